@@ -1,0 +1,145 @@
+/*
+ * hps_mi355x.h -- C ABI of libhps_mi355x.so, the MI355X (gfx950) kernels
+ * behind the uniform-tree HPS build_solver()/solve() hot path.
+ *
+ * The reference (jaxhps) is pure Python/JAX and has no FFI; each entry point
+ * below replaces the batched XLA computation of one reference stage function,
+ * cited per declaration (paths relative to the reference's src/jaxhps/).
+ *
+ * Conventions
+ *   - All array arguments are DEVICE pointers (HBM), row-major, leaf/merge
+ *     batch axis first, exactly the layout of the reference's arrays.
+ *   - dtype: HPS_F64, HPS_F32, HPS_C128, HPS_C64 (complex = interleaved re,im).
+ *     Every floating-point array of a call has that dtype.
+ *   - Calls are asynchronous on the handle's stream; the caller synchronises.
+ *     A handle is bound to one stream and is not thread-safe.
+ *   - Workspace is caller-provided device memory; *_workspace() returns the
+ *     size in bytes.  The library never allocates or frees device memory.
+ *   - Return value: HPS_OK, or a negative error (hps_last_error() explains).
+ *     Numerical breakdown (an exactly zero pivot column) is reported per batch
+ *     element in the device int array `info` (0 = ok, j+1 = column j), the
+ *     LAPACK getrf convention.
+ */
+#ifndef HPS_MI355X_H
+#define HPS_MI355X_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HPS_ABI_VERSION 1
+
+#define HPS_OK 0
+#define HPS_ERR_ARG (-1)
+#define HPS_ERR_HIP (-2)
+#define HPS_ERR_WORKSPACE (-3)
+
+#define HPS_F64 0
+#define HPS_F32 1
+#define HPS_C128 2
+#define HPS_C64 3
+
+#define HPS_KIND_DTN 0
+#define HPS_KIND_ITI 1
+
+typedef struct hps_handle_s* hps_handle_t;
+
+/* Handle bound to a hipStream_t (NULL = default stream). */
+int hps_create(void* stream, hps_handle_t* out);
+int hps_destroy(hps_handle_t h);
+int hps_set_stream(hps_handle_t h, void* stream);
+const char* hps_last_error(hps_handle_t h);
+int hps_version(void);
+
+/* ---- batched dense primitives --------------------------------------------
+ * D[b] = alpha*A[b]@B[b] + beta*C[b]; batch stride 0 broadcasts.  Replaces
+ * the vmapped matmuls/einsums of the reference, e.g. T = Q @ Y
+ * (local_solve/_uniform_2D_DtN.py:267) and the leaf apply einsum
+ * (down_pass/_uniform_2D_DtN.py:114-120). */
+int hps_gemm_batched(hps_handle_t h, int dtype, int M, int N, int K, double alpha, const void* A, long lda,
+                     long sA, const void* B, long ldb, long sB, double beta, const void* C, long ldc, long sC,
+                     void* D, long ldd, long sD, int batch);
+
+/* In-place LU with partial pivoting, P A = L U; perm[b*n+i] = source row of
+ * row i.  Replaces the factorisation inside jnp.linalg.inv
+ * (local_solve/_uniform_2D_DtN.py:257, merge/_schur_complement.py:107,146). */
+size_t hps_getrf_workspace(int n, int batch);
+int hps_getrf_batched(hps_handle_t h, int dtype, int n, void* A, long lda, long sA, int* perm, int* info, int batch,
+                      void* ws, size_t ws_bytes);
+
+/* X = A^{-1} R from hps_getrf_batched's factors (out of place, X != R).
+ * Replaces the products with inv(A) (local_solve/_uniform_2D_DtN.py:260-270). */
+int hps_getrs_batched(hps_handle_t h, int dtype, int n, int nrhs, const void* LU, long lda, long sA, const int* perm,
+                      const void* R, long ldr, long sR, void* X, long ldx, long sX, int batch);
+
+/* ---- leaf stage -----------------------------------------------------------
+ * Batched leaf solve over nl leaves of n = p^d collocation points, nb
+ * Chebyshev boundary points, nq Gauss boundary points and nsrc sources.
+ *   coeffs   (ncoef, nl, n)   coefficient fields that are present
+ *   coef_ops host int[ncoef]: index of each coefficient's operator in
+ *            diffops (nops, n, n), or -1 for the identity (I_coefficients)
+ *   P (nb, nq), Q (nq, n) [DtN: Q, ItI: QH], G (nb, n) [ItI only]
+ *   source   (nl, n, nsrc) or NULL (zero source)
+ * Outputs Y (nl, n, nq), T (nl, nq, nq), v (nl, n, nsrc), h (nl, nq, nsrc),
+ *   lu:   DtN (nl, n-nb, n) = [A_ie | LU(A_ii)];  ItI (nl, n, n) = LU(B)
+ *   perm: DtN (nl, n-nb);  ItI (nl, n);  info (nl).
+ * DtN replaces local_solve_stage_uniform_2D_DtN + get_DtN
+ *   (local_solve/_uniform_2D_DtN.py:9-102,182-273) and the 3D version
+ *   (local_solve/_uniform_3D_DtN.py:13-106);
+ * ItI replaces local_solve_stage_uniform_2D_ItI + get_ItI
+ *   (local_solve/_uniform_2D_ItI.py:10-193).
+ * With nsrc = 0 it is the no-source leaf stage
+ *   (local_solve/_nosource_uniform_2D_{DtN,ItI}.py); Phi is kept as lu/perm. */
+size_t hps_leaf_workspace(int dtype, int kind, int nl, int n, int nb, int nq, int nsrc);
+int hps_leaf_solve(hps_handle_t h, int dtype, int kind, int nl, int n, int nb, int nq, int nsrc, int ncoef,
+                   const int* coef_ops, const void* coeffs, const void* diffops, const void* P, const void* Q,
+                   const void* G, const void* source, void* Y, void* T, void* v, void* hvec, void* lu, int* perm,
+                   int* info, void* ws, size_t ws_bytes);
+
+/* ---- merge stage (one tree level) ------------------------------------------
+ * nm merges of nc children each (4 in 2D, 8 in 3D); children's maps
+ * T_in (nm*nc, nec, nec) and data h_in (nm*nc, nec, nsrc).
+ * Index tables (device int32, built once per level on the host):
+ *   int_rows (ni, 4): for interface equation r, up to two (child, local) pairs
+ *                     whose outgoing data it sums (child -1 = unused)
+ *   int_cols (ni, 4): for interface unknown j, the (child, local) slots it feeds
+ *   ext      (ne, 4): (child, local, -1, 0) per merged exterior node, in the
+ *                     merged node's final order (the reference's roll folded in)
+ * Solves D g_int = -(C g_ext + h_int) with D = gather(int_rows, int_cols)
+ * (+ I when add_identity, the ItI structure I + [[0,D12],[D21,0]]):
+ *   S_out (nm, ni, ne) = -D^{-1} C,   gt_out (nm, ni, nsrc) = -D^{-1} h_int,
+ *   T_out (nm, ne, ne) = A + B S,     h_out (nm, ne, nsrc) = h_ext + B gt,
+ *   D_lu (nm, ni, ni) + perm (nm, ni) = LU of D (kept for the up pass),
+ *   B_out (nm, ne, ni) optional copy of B.
+ * Replaces _uniform_quad_merge_DtN / _ItI with assemble_merge_outputs_*
+ *   (merge/_uniform_2D_DtN.py:207-348, merge/_uniform_2D_ItI.py:183-375,
+ *   merge/_schur_complement.py:7-237) and _uniform_oct_merge_DtN
+ *   (merge/_uniform_3D_DtN.py:128-187, merge/_schur_complement.py:294-774). */
+size_t hps_merge_workspace(int dtype, int nm, int ni, int ne, int nsrc);
+int hps_merge_level(hps_handle_t h, int dtype, int nm, int nc, int nec, int ni, int ne, int nsrc, int add_identity,
+                    const void* T_in, const void* h_in, const int* int_rows, const int* int_cols, const int* ext,
+                    void* S_out, void* T_out, void* h_out, void* gt_out, void* D_lu, int* perm, int* info,
+                    void* B_out, void* ws, size_t ws_bytes);
+
+/* ---- down pass (one tree level) -------------------------------------------
+ * g_int = S g_in + gt; children's boundary data g_out (nm*nc, nec, nsrc) from
+ * child_src (nc*nec ints): >= 0 parent exterior index, < 0 interface index
+ * -(j+1).  Replaces _propagate_down_2D_DtN / _propogate_down_2D_ItI /
+ * _propogate_down_oct_DtN (down_pass/_uniform_2D_DtN.py:126-189,
+ * down_pass/_uniform_2D_ItI.py:122-192, down_pass/_uniform_3D_DtN.py:117-246). */
+size_t hps_down_workspace(int dtype, int nm, int ni, int nsrc);
+int hps_down_level(hps_handle_t h, int dtype, int nm, int nc, int nec, int ni, int ne, int nsrc, const void* S,
+                   const void* gt, const void* g_in, const int* child_src, void* g_out, void* ws, size_t ws_bytes);
+
+/* u = Y g + v per leaf (v may be NULL).  Replaces the leaf einsum of the down
+ * pass (down_pass/_uniform_2D_DtN.py:110-120). */
+int hps_leaf_apply(hps_handle_t h, int dtype, int nl, int n, int nq, int nsrc, const void* Y, const void* g,
+                   const void* v, void* u);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HPS_MI355X_H */

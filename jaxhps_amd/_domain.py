@@ -1,0 +1,49 @@
+"""``Domain``: uniform quadtree/octree discretization (``_domain.py:38-97``)."""
+
+from __future__ import annotations
+
+import numpy as np
+
+from ._discretization_tree import DiscretizationNode2D, DiscretizationNode3D
+from ._grid_creation import (
+    compute_boundary_Gauss_points_uniform_2D,
+    compute_boundary_Gauss_points_uniform_3D,
+    compute_interior_Chebyshev_points_uniform_2D,
+    compute_interior_Chebyshev_points_uniform_3D,
+)
+
+
+class Domain:
+    """Uniform HPS discretization of a box.
+
+    Args mirror the reference: p Chebyshev points per leaf side, q Gauss
+    points per leaf side, the root node, and L levels of uniform refinement.
+    Point arrays are float64 numpy arrays on the host:
+    ``interior_points`` (n_leaves, p^d, d) and ``boundary_points``
+    (n_bdry, d).  Adaptive trees (L=None) are not supported by this package.
+    """
+
+    def __init__(self, p: int, q: int, root, L: int | None = None):
+        self.p = p
+        self.q = q
+        self.root = root
+        self.L = L
+        self.bool_2D = isinstance(root, DiscretizationNode2D)
+        if not (self.bool_2D or isinstance(root, DiscretizationNode3D)):
+            raise TypeError("root must be a DiscretizationNode2D or DiscretizationNode3D")
+        if L is None:
+            raise NotImplementedError(
+                "adaptive discretizations are outside this package's scope; pass L for a uniform tree"
+            )
+        self.bool_uniform = True
+        if self.bool_2D:
+            self.interior_points = compute_interior_Chebyshev_points_uniform_2D(root, L, p)
+            self.boundary_points = compute_boundary_Gauss_points_uniform_2D(root, L, q)
+            self.n_leaves = 4**L
+        else:
+            self.interior_points = compute_interior_Chebyshev_points_uniform_3D(root, L, p)
+            self.boundary_points = compute_boundary_Gauss_points_uniform_3D(root, L, q)
+            self.n_leaves = 8**L
+
+    def __repr__(self) -> str:
+        return f"Domain(p={self.p}, q={self.q}, L={self.L}, {'2D' if self.bool_2D else '3D'}, n_leaves={self.n_leaves})"

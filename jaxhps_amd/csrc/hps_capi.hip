@@ -1,0 +1,667 @@
+// C ABI of libhps_mi355x and the HPS stage kernels behind it.
+//
+// Stage entry points (leaf solve, one merge level, one down-pass level, leaf
+// apply) compose the batched LU/GEMM primitives with three kinds of
+// memory-bound kernels written here:
+//   * leaf operator assembly  A = sum_k diag(c_k) D_k  for a row band,
+//   * index-table gathers that build the merge blocks D, C, B, A and the
+//     interface/exterior data vectors from the children's T and h,
+//   * the down-pass scatter of parent/interface boundary data to children.
+// The index tables (which child and local boundary index each merged node
+// comes from) are computed once on the host per tree level; they encode the
+// reference's block index sets, rolls, flips and row permutation, so one
+// merge code path serves 2D DtN, 2D ItI and 3D DtN.
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "hps_kernels.h"
+#include "../../include/hps_mi355x.h"
+
+namespace hps {
+
+// ---------------------------------------------------------------------------
+// generic batched 2D copy
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void copy2d_kernel(int rows, int cols, double alpha, const T* src, long lds, long ss, T* dst, long ldd,
+                              long sd, int batch) {
+  long total = (long)rows * cols;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    int r = (int)(idx / cols), c = (int)(idx - (long)r * cols);
+    for (int b = blockIdx.y; b < batch; b += gridDim.y) {
+      T v = src ? s_scale(src[(long)b * ss + (long)r * lds + c], alpha) : s_zero<T>();
+      dst[(long)b * sd + (long)r * ldd + c] = v;
+    }
+  }
+}
+
+template <typename T>
+hipError_t copy2d(int rows, int cols, double alpha, const T* src, long lds, long ss, T* dst, long ldd, long sd,
+                  int batch, hipStream_t s) {
+  if (rows <= 0 || cols <= 0 || batch <= 0) return hipSuccess;
+  long total = (long)rows * cols;
+  unsigned gx = ceil_div(total, 256);
+  if (gx > 4096) gx = 4096;
+  unsigned gy = batch < 65535 ? batch : 65535;
+  hipLaunchKernelGGL((copy2d_kernel<T>), dim3(gx, gy), dim3(256), 0, s, rows, cols, alpha, src, lds, ss, dst, ldd,
+                     sd, batch);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// leaf operator assembly: out[l][r][c] = sum_i coef_i[l][row0+r] * D_{op_i}[row0+r][col0+c]
+// op index < 0 means the identity operator.
+// ---------------------------------------------------------------------------
+struct CoefOps {
+  int op[16];
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void assemble_kernel(int n, int nl, int ncoef, CoefOps ops_idx, const T* coeffs,
+                                                       const T* diffops, int row0, int nrows, int col0, int ncols,
+                                                       T* out, long ldo, long so) {
+  const int l = blockIdx.y;
+  long total = (long)nrows * ncols;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    int r = (int)(idx / ncols), c = (int)(idx - (long)r * ncols);
+    int gr = row0 + r, gc = col0 + c;
+    T acc = s_zero<T>();
+    for (int i = 0; i < ncoef; ++i) {
+      T cf = coeffs[((long)i * nl + l) * n + gr];
+      int op = ops_idx.op[i];
+      if (op < 0) {
+        if (gr == gc) acc = s_add(acc, cf);
+      } else {
+        acc = s_add(acc, s_mul(cf, diffops[((long)op * n + gr) * n + gc]));
+      }
+    }
+    out[(long)l * so + (long)r * ldo + c] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// merge gathers
+// rows/cols entries: int4 (c0, l0, c1, l1); c < 0 marks an unused slot.
+// out[m][r][c] = sign * sum_{a,b} [rc_a == cc_b] Tin[m*nc + rc_a][rl_a][cl_b] + (add_id && r == c)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void merge_gather_kernel(const T* Tin, int nc, int nec, const int4* rows, int nr,
+                                                           const int4* cols, int ncl, double sign, int add_id,
+                                                           T* out, long ldo, long so, int nm) {
+  long total = (long)nr * ncl;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    int r = (int)(idx / ncl), c = (int)(idx - (long)r * ncl);
+    int4 re = rows[r];
+    int4 ce = cols[c];
+    for (int m = blockIdx.y; m < nm; m += gridDim.y) {
+      const T* Tm = Tin + (long)m * nc * nec * nec;
+      T acc = s_zero<T>();
+      if (re.x >= 0) {
+        const T* Tc = Tm + (long)re.x * nec * nec + (long)re.y * nec;
+        if (ce.x == re.x) acc = s_add(acc, Tc[ce.y]);
+        if (ce.z == re.x) acc = s_add(acc, Tc[ce.w]);
+      }
+      if (re.z >= 0) {
+        const T* Tc = Tm + (long)re.z * nec * nec + (long)re.w * nec;
+        if (ce.x == re.z) acc = s_add(acc, Tc[ce.y]);
+        if (ce.z == re.z) acc = s_add(acc, Tc[ce.w]);
+      }
+      acc = s_scale(acc, sign);
+      if (add_id && r == c) acc = s_add(acc, s_one<T>());
+      out[(long)m * so + (long)r * ldo + c] = acc;
+    }
+  }
+}
+
+// out[m][r][s] = sign * sum_a h[m*nc + rc_a][rl_a][s]
+template <typename T>
+__global__ __launch_bounds__(256) void vec_gather_kernel(const T* h, int nc, int nec, int nsrc, const int4* rows,
+                                                         int nr, double sign, T* out, int nm) {
+  long total = (long)nr * nsrc;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    int r = (int)(idx / nsrc), s = (int)(idx - (long)r * nsrc);
+    int4 re = rows[r];
+    for (int m = blockIdx.y; m < nm; m += gridDim.y) {
+      const T* hm = h + (long)m * nc * nec * nsrc;
+      T acc = s_zero<T>();
+      if (re.x >= 0) acc = s_add(acc, hm[((long)re.x * nec + re.y) * nsrc + s]);
+      if (re.z >= 0) acc = s_add(acc, hm[((long)re.z * nec + re.w) * nsrc + s]);
+      out[((long)m * nr + r) * nsrc + s] = s_scale(acc, sign);
+    }
+  }
+}
+
+// Down pass scatter: child slot (c, l) takes parent exterior data (src >= 0)
+// or interface data (src < 0 -> index -src-1).
+template <typename T>
+__global__ __launch_bounds__(256) void down_scatter_kernel(int nc, int nec, int ne, int ni, int nsrc,
+                                                           const int* child_src, const T* g_par, const T* g_int,
+                                                           T* g_out, int nm) {
+  long total = (long)nc * nec * nsrc;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    int slot = (int)(idx / nsrc), s = (int)(idx - (long)slot * nsrc);
+    int src = child_src[slot];
+    for (int m = blockIdx.y; m < nm; m += gridDim.y) {
+      T v = src >= 0 ? g_par[((long)m * ne + src) * nsrc + s] : g_int[((long)m * ni + (-src - 1)) * nsrc + s];
+      g_out[((long)m * nc * nec + slot) * nsrc + s] = v;
+    }
+  }
+}
+
+static dim3 grid_for(long total, int batch) {
+  unsigned gx = ceil_div(total, 256);
+  if (gx > 2048) gx = 2048;
+  if (gx == 0) gx = 1;
+  unsigned gy = batch < 65535 ? batch : 65535;
+  return dim3(gx, gy);
+}
+
+}  // namespace hps
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+struct hps_handle_s {
+  hipStream_t stream = nullptr;
+  std::string err;
+};
+
+using namespace hps;
+
+#define HPS_CHECK_HIP(h, expr)                                                      \
+  do {                                                                              \
+    hipError_t _e = (expr);                                                         \
+    if (_e != hipSuccess) {                                                         \
+      (h)->err = std::string(#expr) + ": " + hipGetErrorString(_e);                 \
+      return HPS_ERR_HIP;                                                           \
+    }                                                                               \
+  } while (0)
+
+#define HPS_ARG(h, cond, msg)   \
+  do {                          \
+    if (!(cond)) {              \
+      (h)->err = (msg);         \
+      return HPS_ERR_ARG;       \
+    }                           \
+  } while (0)
+
+static size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+// The entry points below get C linkage from their declarations in
+// hps_mi355x.h; the template helpers between them stay C++.
+
+int hps_create(void* stream, hps_handle_t* out) {
+  if (!out) return HPS_ERR_ARG;
+  hps_handle_s* h = new hps_handle_s();
+  h->stream = (hipStream_t)stream;
+  *out = h;
+  return HPS_OK;
+}
+
+int hps_destroy(hps_handle_t h) {
+  delete h;
+  return HPS_OK;
+}
+
+int hps_set_stream(hps_handle_t h, void* stream) {
+  if (!h) return HPS_ERR_ARG;
+  h->stream = (hipStream_t)stream;
+  return HPS_OK;
+}
+
+const char* hps_last_error(hps_handle_t h) { return h ? h->err.c_str() : "null handle"; }
+
+int hps_version(void) { return HPS_ABI_VERSION; }
+
+// ---------------------------------------------------------------------------
+// primitives
+// ---------------------------------------------------------------------------
+template <typename T>
+static int gemm_t(hps_handle_t h, int M, int N, int K, double alpha, const void* A, long lda, long sA, const void* B,
+                  long ldb, long sB, double beta, const void* C, long ldc, long sC, void* D, long ldd, long sD,
+                  int batch) {
+  GemmArgs<T> g;
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.alpha = alpha;
+  g.beta = beta;
+  g.A = (const T*)A;
+  g.lda = lda;
+  g.sA = sA;
+  g.B = (const T*)B;
+  g.ldb = ldb;
+  g.sB = sB;
+  g.C = (const T*)C;
+  g.ldc = ldc;
+  g.sC = sC;
+  g.D = (T*)D;
+  g.ldd = ldd;
+  g.sD = sD;
+  g.batch = batch;
+  HPS_CHECK_HIP(h, gemm<T>(g, h->stream));
+  return HPS_OK;
+}
+
+#define DISPATCH(dtype, FN, ...)                                  \
+  switch (dtype) {                                                \
+    case HPS_F64: return FN<double>(__VA_ARGS__);                 \
+    case HPS_F32: return FN<float>(__VA_ARGS__);                  \
+    case HPS_C128: return FN<c128>(__VA_ARGS__);                  \
+    case HPS_C64: return FN<c64>(__VA_ARGS__);                    \
+    default: h->err = "bad dtype"; return HPS_ERR_ARG;            \
+  }
+
+int hps_gemm_batched(hps_handle_t h, int dtype, int M, int N, int K, double alpha, const void* A, long lda, long sA,
+                     const void* B, long ldb, long sB, double beta, const void* C, long ldc, long sC, void* D,
+                     long ldd, long sD, int batch) {
+  if (!h) return HPS_ERR_ARG;
+  HPS_ARG(h, M >= 0 && N >= 0 && K >= 0 && batch >= 0, "negative size");
+  HPS_ARG(h, D != nullptr && (K == 0 || (A && B)), "null matrix");
+  HPS_ARG(h, beta == 0.0 || C != nullptr, "beta != 0 needs C");
+  DISPATCH(dtype, gemm_t, h, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch);
+}
+
+size_t hps_getrf_workspace(int n, int batch) { return align_up((size_t)n * batch * sizeof(int)); }
+
+template <typename T>
+static int getrf_t(hps_handle_t h, int n, void* A, long lda, long sA, int* perm, int* info, int batch, void* ws) {
+  HPS_CHECK_HIP(h, getrf_ws<T>(n, (T*)A, lda, sA, perm, (int*)ws, info, batch, h->stream));
+  return HPS_OK;
+}
+
+int hps_getrf_batched(hps_handle_t h, int dtype, int n, void* A, long lda, long sA, int* perm, int* info, int batch,
+                      void* ws, size_t ws_bytes) {
+  if (!h) return HPS_ERR_ARG;
+  HPS_ARG(h, n >= 0 && batch >= 0 && lda >= n, "bad size");
+  HPS_ARG(h, A && perm && info, "null pointer");
+  if (ws_bytes < hps_getrf_workspace(n, batch)) {
+    h->err = "getrf workspace too small";
+    return HPS_ERR_WORKSPACE;
+  }
+  DISPATCH(dtype, getrf_t, h, n, A, lda, sA, perm, info, batch, ws);
+}
+
+template <typename T>
+static int getrs_t(hps_handle_t h, int n, int nrhs, const void* LU, long lda, long sA, const int* perm, const void* R,
+                   long ldr, long sR, void* X, long ldx, long sX, int batch) {
+  HPS_CHECK_HIP(h, getrs_perm<T>(n, nrhs, (const T*)LU, lda, sA, perm, (const T*)R, ldr, sR, (T*)X, ldx, sX, batch,
+                                 h->stream));
+  return HPS_OK;
+}
+
+int hps_getrs_batched(hps_handle_t h, int dtype, int n, int nrhs, const void* LU, long lda, long sA, const int* perm,
+                      const void* R, long ldr, long sR, void* X, long ldx, long sX, int batch) {
+  if (!h) return HPS_ERR_ARG;
+  HPS_ARG(h, n >= 0 && nrhs >= 0 && batch >= 0, "bad size");
+  HPS_ARG(h, LU && perm && R && X, "null pointer");
+  HPS_ARG(h, R != X, "getrs is out of place: R and X must differ");
+  DISPATCH(dtype, getrs_t, h, n, nrhs, LU, lda, sA, perm, R, ldr, sR, X, ldx, sX, batch);
+}
+
+// ---------------------------------------------------------------------------
+// leaf solve
+// ---------------------------------------------------------------------------
+size_t hps_leaf_workspace(int dtype, int kind, int nl, int n, int nb, int nq, int nsrc) {
+  size_t es = dtype_size(dtype);
+  int nrows = kind == HPS_KIND_DTN ? n - nb : n;
+  size_t rhs = align_up((size_t)nl * nrows * (nq + nsrc) * es);
+  return 2 * rhs + align_up((size_t)nl * nrows * sizeof(int));
+}
+
+template <typename T>
+static int leaf_t(hps_handle_t h, int kind, int nl, int n, int nb, int nq, int nsrc, int ncoef, const int* coef_ops,
+                  const void* coeffs, const void* diffops, const void* Pv, const void* Qv, const void* Gv,
+                  const void* source, void* Yv, void* Tv, void* vv, void* hv, void* luv, int* perm, int* info,
+                  void* ws) {
+  hipStream_t s = h->stream;
+  const T* P = (const T*)Pv;
+  const T* Q = (const T*)Qv;
+  T* Y = (T*)Yv;
+  T* Tm = (T*)Tv;
+  T* lu = (T*)luv;
+  const int ni = n - nb;
+  const int nrows = kind == HPS_KIND_DTN ? ni : n;
+  const int w = nq + nsrc;
+  size_t es = sizeof(T);
+  char* wp = (char*)ws;
+  T* R = (T*)wp;
+  wp += align_up((size_t)nl * nrows * w * es);
+  T* X = (T*)wp;
+  wp += align_up((size_t)nl * nrows * w * es);
+  int* ipiv = (int*)wp;
+
+  CoefOps co;
+  for (int i = 0; i < 16; ++i) co.op[i] = i < ncoef ? coef_ops[i] : 0;
+
+  // 1. assemble the operator rows the factorisation needs
+  const long ld_lu = n;
+  const long s_lu = (long)nrows * n;
+  int arow0 = nb, an = ni;
+  T* aout = kind == HPS_KIND_DTN ? lu : lu + (long)nb * n;
+  {
+    dim3 g = grid_for((long)an * n, 1);
+    g.y = nl;
+    hipLaunchKernelGGL((assemble_kernel<T>), g, dim3(256), 0, s, n, nl, ncoef, co, (const T*)coeffs,
+                       (const T*)diffops, arow0, an, 0, n, aout, ld_lu, s_lu);
+    HPS_CHECK_HIP(h, hipGetLastError());
+  }
+  if (kind == HPS_KIND_ITI) {
+    // incoming-impedance rows G on top of B
+    HPS_CHECK_HIP(h, copy2d<T>(nb, n, 1.0, (const T*)Gv, n, 0, lu, ld_lu, s_lu, nl, s));
+  }
+
+  // 2. right-hand sides
+  if (kind == HPS_KIND_DTN) {
+    // R[:, :, :nq] = A_ie P
+    GemmArgs<T> g;
+    g.M = ni;
+    g.N = nq;
+    g.K = nb;
+    g.A = lu;
+    g.lda = ld_lu;
+    g.sA = s_lu;
+    g.B = P;
+    g.ldb = nq;
+    g.sB = 0;
+    g.D = R;
+    g.ldd = w;
+    g.sD = (long)ni * w;
+    g.batch = nl;
+    HPS_CHECK_HIP(h, gemm<T>(g, s));
+    if (nsrc > 0)
+      HPS_CHECK_HIP(h, copy2d<T>(ni, nsrc, 1.0, source ? (const T*)source + (long)nb * nsrc : nullptr, nsrc,
+                                 (long)n * nsrc, R + nq, w, (long)ni * w, nl, s));
+  } else {
+    HPS_CHECK_HIP(h, copy2d<T>(n, w, 1.0, nullptr, 0, 0, R, w, (long)n * w, nl, s));
+    HPS_CHECK_HIP(h, copy2d<T>(nb, nq, 1.0, P, nq, 0, R, w, (long)n * w, nl, s));
+    if (nsrc > 0 && source)
+      HPS_CHECK_HIP(h, copy2d<T>(ni, nsrc, 1.0, (const T*)source + (long)nb * nsrc, nsrc, (long)n * nsrc,
+                                 R + (long)nb * w + nq, w, (long)n * w, nl, s));
+  }
+
+  // 3. factor and solve
+  T* Afac = kind == HPS_KIND_DTN ? lu + nb : lu;
+  HPS_CHECK_HIP(h, getrf_ws<T>(nrows, Afac, ld_lu, s_lu, perm, ipiv, info, nl, s));
+  HPS_CHECK_HIP(h, getrs_perm<T>(nrows, w, Afac, ld_lu, s_lu, perm, R, w, (long)nrows * w, X, w, (long)nrows * w, nl, s));
+
+  // 4. unpack Y and v
+  if (kind == HPS_KIND_DTN) {
+    HPS_CHECK_HIP(h, copy2d<T>(nb, nq, 1.0, P, nq, 0, Y, nq, (long)n * nq, nl, s));
+    HPS_CHECK_HIP(h, copy2d<T>(ni, nq, -1.0, X, w, (long)ni * w, Y + (long)nb * nq, nq, (long)n * nq, nl, s));
+    if (nsrc > 0) {
+      HPS_CHECK_HIP(h, copy2d<T>(nb, nsrc, 1.0, nullptr, 0, 0, (T*)vv, nsrc, (long)n * nsrc, nl, s));
+      HPS_CHECK_HIP(h, copy2d<T>(ni, nsrc, 1.0, X + nq, w, (long)ni * w, (T*)vv + (long)nb * nsrc, nsrc,
+                                 (long)n * nsrc, nl, s));
+    }
+  } else {
+    HPS_CHECK_HIP(h, copy2d<T>(n, nq, 1.0, X, w, (long)n * w, Y, nq, (long)n * nq, nl, s));
+    if (nsrc > 0)
+      HPS_CHECK_HIP(h, copy2d<T>(n, nsrc, 1.0, X + nq, w, (long)n * w, (T*)vv, nsrc, (long)n * nsrc, nl, s));
+  }
+
+  // 5. T = Q Y, h = Q v
+  {
+    GemmArgs<T> g;
+    g.M = nq;
+    g.N = nq;
+    g.K = n;
+    g.A = Q;
+    g.lda = n;
+    g.sA = 0;
+    g.B = Y;
+    g.ldb = nq;
+    g.sB = (long)n * nq;
+    g.D = Tm;
+    g.ldd = nq;
+    g.sD = (long)nq * nq;
+    g.batch = nl;
+    HPS_CHECK_HIP(h, gemm<T>(g, s));
+  }
+  if (nsrc > 0) {
+    GemmArgs<T> g;
+    g.M = nq;
+    g.N = nsrc;
+    g.K = n;
+    g.A = Q;
+    g.lda = n;
+    g.sA = 0;
+    g.B = (const T*)vv;
+    g.ldb = nsrc;
+    g.sB = (long)n * nsrc;
+    g.D = (T*)hv;
+    g.ldd = nsrc;
+    g.sD = (long)nq * nsrc;
+    g.batch = nl;
+    HPS_CHECK_HIP(h, gemm<T>(g, s));
+  }
+  return HPS_OK;
+}
+
+int hps_leaf_solve(hps_handle_t h, int dtype, int kind, int nl, int n, int nb, int nq, int nsrc, int ncoef,
+                   const int* coef_ops, const void* coeffs, const void* diffops, const void* P, const void* Q,
+                   const void* G, const void* source, void* Y, void* T, void* v, void* hvec, void* lu, int* perm,
+                   int* info, void* ws, size_t ws_bytes) {
+  if (!h) return HPS_ERR_ARG;
+  HPS_ARG(h, kind == HPS_KIND_DTN || kind == HPS_KIND_ITI, "kind must be DtN or ItI");
+  HPS_ARG(h, nl > 0 && n > nb && nb > 0 && nq > 0 && nsrc >= 0, "bad leaf sizes");
+  HPS_ARG(h, ncoef > 0 && ncoef <= 16 && coef_ops, "1..16 coefficients required");
+  HPS_ARG(h, coeffs && diffops && P && Q && Y && T && lu && perm && info, "null pointer");
+  HPS_ARG(h, kind == HPS_KIND_DTN || G, "ItI needs G");
+  HPS_ARG(h, nsrc == 0 || (v && hvec), "v/h required when nsrc > 0");
+  if (ws_bytes < hps_leaf_workspace(dtype, kind, nl, n, nb, nq, nsrc)) {
+    h->err = "leaf workspace too small";
+    return HPS_ERR_WORKSPACE;
+  }
+  DISPATCH(dtype, leaf_t, h, kind, nl, n, nb, nq, nsrc, ncoef, coef_ops, coeffs, diffops, P, Q, G, source, Y, T, v,
+           hvec, lu, perm, info, ws);
+}
+
+// ---------------------------------------------------------------------------
+// merge level
+// ---------------------------------------------------------------------------
+size_t hps_merge_workspace(int dtype, int nm, int ni, int ne, int nsrc) {
+  size_t es = dtype_size(dtype);
+  size_t big = (size_t)nm * ni * ne * es;
+  return align_up(big) + align_up((size_t)nm * ni * (nsrc > 0 ? nsrc : 1) * es) + align_up((size_t)nm * ni * sizeof(int));
+}
+
+template <typename T>
+static int merge_t(hps_handle_t h, int nm, int nc, int nec, int ni, int ne, int nsrc, int add_identity,
+                   const void* T_in, const void* h_in, const int* int_rows, const int* int_cols, const int* ext,
+                   void* S_out, void* T_out, void* h_out, void* gt_out, void* D_lu, int* perm, int* info,
+                   void* B_out, void* ws) {
+  hipStream_t s = h->stream;
+  const T* Tin = (const T*)T_in;
+  const int4* rows = (const int4*)int_rows;
+  const int4* cols = (const int4*)int_cols;
+  const int4* ex = (const int4*)ext;
+  size_t es = sizeof(T);
+  char* wp = (char*)ws;
+  T* W1 = (T*)wp;  // -C, later B
+  wp += align_up((size_t)nm * ni * ne * es);
+  T* W2 = (T*)wp;  // -h_int
+  wp += align_up((size_t)nm * ni * (nsrc > 0 ? nsrc : 1) * es);
+  int* ipiv = (int*)wp;
+
+  // D (+ I for ItI) -> D_lu, then factor
+  hipLaunchKernelGGL((merge_gather_kernel<T>), grid_for((long)ni * ni, nm), dim3(256), 0, s, Tin, nc, nec, rows, ni,
+                     cols, ni, 1.0, add_identity, (T*)D_lu, (long)ni, (long)ni * ni, nm);
+  HPS_CHECK_HIP(h, hipGetLastError());
+  HPS_CHECK_HIP(h, getrf_ws<T>(ni, (T*)D_lu, ni, (long)ni * ni, perm, ipiv, info, nm, s));
+
+  // S = -D^{-1} C
+  hipLaunchKernelGGL((merge_gather_kernel<T>), grid_for((long)ni * ne, nm), dim3(256), 0, s, Tin, nc, nec, rows, ni,
+                     ex, ne, -1.0, 0, W1, (long)ne, (long)ni * ne, nm);
+  HPS_CHECK_HIP(h, hipGetLastError());
+  HPS_CHECK_HIP(h, getrs_perm<T>(ni, ne, (const T*)D_lu, ni, (long)ni * ni, perm, W1, ne, (long)ni * ne, (T*)S_out,
+                                 ne, (long)ni * ne, nm, s));
+  // g_tilde = -D^{-1} h_int
+  if (nsrc > 0) {
+    hipLaunchKernelGGL((vec_gather_kernel<T>), grid_for((long)ni * nsrc, nm), dim3(256), 0, s, (const T*)h_in, nc,
+                       nec, nsrc, rows, ni, -1.0, W2, nm);
+    HPS_CHECK_HIP(h, hipGetLastError());
+    HPS_CHECK_HIP(h, getrs_perm<T>(ni, nsrc, (const T*)D_lu, ni, (long)ni * ni, perm, W2, nsrc, (long)ni * nsrc,
+                                   (T*)gt_out, nsrc, (long)ni * nsrc, nm, s));
+  }
+  // B (exterior rows x interface unknowns)
+  T* Bm = B_out ? (T*)B_out : W1;
+  hipLaunchKernelGGL((merge_gather_kernel<T>), grid_for((long)ne * ni, nm), dim3(256), 0, s, Tin, nc, nec, ex, ne,
+                     cols, ni, 1.0, 0, Bm, (long)ni, (long)ne * ni, nm);
+  HPS_CHECK_HIP(h, hipGetLastError());
+  // T = A + B S
+  if (T_out) {
+    hipLaunchKernelGGL((merge_gather_kernel<T>), grid_for((long)ne * ne, nm), dim3(256), 0, s, Tin, nc, nec, ex, ne,
+                       ex, ne, 1.0, 0, (T*)T_out, (long)ne, (long)ne * ne, nm);
+    HPS_CHECK_HIP(h, hipGetLastError());
+    GemmArgs<T> g;
+    g.M = ne;
+    g.N = ne;
+    g.K = ni;
+    g.alpha = 1.0;
+    g.beta = 1.0;
+    g.A = Bm;
+    g.lda = ni;
+    g.sA = (long)ne * ni;
+    g.B = (const T*)S_out;
+    g.ldb = ne;
+    g.sB = (long)ni * ne;
+    g.C = (const T*)T_out;
+    g.ldc = ne;
+    g.sC = (long)ne * ne;
+    g.D = (T*)T_out;
+    g.ldd = ne;
+    g.sD = (long)ne * ne;
+    g.batch = nm;
+    HPS_CHECK_HIP(h, gemm<T>(g, s));
+  }
+  // h_out = h_ext + B g_tilde
+  if (nsrc > 0 && h_out) {
+    hipLaunchKernelGGL((vec_gather_kernel<T>), grid_for((long)ne * nsrc, nm), dim3(256), 0, s, (const T*)h_in, nc,
+                       nec, nsrc, ex, ne, 1.0, (T*)h_out, nm);
+    HPS_CHECK_HIP(h, hipGetLastError());
+    GemmArgs<T> g;
+    g.M = ne;
+    g.N = nsrc;
+    g.K = ni;
+    g.alpha = 1.0;
+    g.beta = 1.0;
+    g.A = Bm;
+    g.lda = ni;
+    g.sA = (long)ne * ni;
+    g.B = (const T*)gt_out;
+    g.ldb = nsrc;
+    g.sB = (long)ni * nsrc;
+    g.C = (const T*)h_out;
+    g.ldc = nsrc;
+    g.sC = (long)ne * nsrc;
+    g.D = (T*)h_out;
+    g.ldd = nsrc;
+    g.sD = (long)ne * nsrc;
+    g.batch = nm;
+    HPS_CHECK_HIP(h, gemm<T>(g, s));
+  }
+  return HPS_OK;
+}
+
+int hps_merge_level(hps_handle_t h, int dtype, int nm, int nc, int nec, int ni, int ne, int nsrc, int add_identity,
+                    const void* T_in, const void* h_in, const int* int_rows, const int* int_cols, const int* ext,
+                    void* S_out, void* T_out, void* h_out, void* gt_out, void* D_lu, int* perm, int* info,
+                    void* B_out, void* ws, size_t ws_bytes) {
+  if (!h) return HPS_ERR_ARG;
+  HPS_ARG(h, nm > 0 && nc > 0 && nec > 0 && ni > 0 && ne > 0 && nsrc >= 0, "bad merge sizes");
+  HPS_ARG(h, T_in && int_rows && int_cols && ext && S_out && D_lu && perm && info, "null pointer");
+  HPS_ARG(h, nsrc == 0 || (h_in && gt_out), "h_in/gt_out required when nsrc > 0");
+  if (ws_bytes < hps_merge_workspace(dtype, nm, ni, ne, nsrc)) {
+    h->err = "merge workspace too small";
+    return HPS_ERR_WORKSPACE;
+  }
+  DISPATCH(dtype, merge_t, h, nm, nc, nec, ni, ne, nsrc, add_identity, T_in, h_in, int_rows, int_cols, ext, S_out,
+           T_out, h_out, gt_out, D_lu, perm, info, B_out, ws);
+}
+
+// ---------------------------------------------------------------------------
+// down pass level and leaf apply
+// ---------------------------------------------------------------------------
+size_t hps_down_workspace(int dtype, int nm, int ni, int nsrc) {
+  return align_up((size_t)nm * ni * nsrc * dtype_size(dtype));
+}
+
+template <typename T>
+static int down_t(hps_handle_t h, int nm, int nc, int nec, int ni, int ne, int nsrc, const void* S, const void* gt,
+                  const void* g_in, const int* child_src, void* g_out, void* ws) {
+  hipStream_t s = h->stream;
+  T* W = (T*)ws;
+  GemmArgs<T> g;
+  g.M = ni;
+  g.N = nsrc;
+  g.K = ne;
+  g.alpha = 1.0;
+  g.beta = gt ? 1.0 : 0.0;
+  g.A = (const T*)S;
+  g.lda = ne;
+  g.sA = (long)ni * ne;
+  g.B = (const T*)g_in;
+  g.ldb = nsrc;
+  g.sB = (long)ne * nsrc;
+  g.C = (const T*)gt;
+  g.ldc = nsrc;
+  g.sC = (long)ni * nsrc;
+  g.D = W;
+  g.ldd = nsrc;
+  g.sD = (long)ni * nsrc;
+  g.batch = nm;
+  HPS_CHECK_HIP(h, gemm<T>(g, s));
+  hipLaunchKernelGGL((down_scatter_kernel<T>), grid_for((long)nc * nec * nsrc, nm), dim3(256), 0, s, nc, nec, ne, ni,
+                     nsrc, child_src, (const T*)g_in, W, (T*)g_out, nm);
+  HPS_CHECK_HIP(h, hipGetLastError());
+  return HPS_OK;
+}
+
+int hps_down_level(hps_handle_t h, int dtype, int nm, int nc, int nec, int ni, int ne, int nsrc, const void* S,
+                   const void* gt, const void* g_in, const int* child_src, void* g_out, void* ws, size_t ws_bytes) {
+  if (!h) return HPS_ERR_ARG;
+  HPS_ARG(h, nm > 0 && nc > 0 && nec > 0 && ni > 0 && ne > 0 && nsrc > 0, "bad down-pass sizes");
+  HPS_ARG(h, S && g_in && child_src && g_out, "null pointer");
+  if (ws_bytes < hps_down_workspace(dtype, nm, ni, nsrc)) {
+    h->err = "down workspace too small";
+    return HPS_ERR_WORKSPACE;
+  }
+  DISPATCH(dtype, down_t, h, nm, nc, nec, ni, ne, nsrc, S, gt, g_in, child_src, g_out, ws);
+}
+
+template <typename T>
+static int apply_t(hps_handle_t h, int nl, int n, int nq, int nsrc, const void* Y, const void* g, const void* v,
+                   void* u) {
+  GemmArgs<T> a;
+  a.M = n;
+  a.N = nsrc;
+  a.K = nq;
+  a.alpha = 1.0;
+  a.beta = v ? 1.0 : 0.0;
+  a.A = (const T*)Y;
+  a.lda = nq;
+  a.sA = (long)n * nq;
+  a.B = (const T*)g;
+  a.ldb = nsrc;
+  a.sB = (long)nq * nsrc;
+  a.C = (const T*)v;
+  a.ldc = nsrc;
+  a.sC = (long)n * nsrc;
+  a.D = (T*)u;
+  a.ldd = nsrc;
+  a.sD = (long)n * nsrc;
+  a.batch = nl;
+  HPS_CHECK_HIP(h, gemm<T>(a, h->stream));
+  return HPS_OK;
+}
+
+int hps_leaf_apply(hps_handle_t h, int dtype, int nl, int n, int nq, int nsrc, const void* Y, const void* g,
+                   const void* v, void* u) {
+  if (!h) return HPS_ERR_ARG;
+  HPS_ARG(h, nl > 0 && n > 0 && nq > 0 && nsrc > 0, "bad sizes");
+  HPS_ARG(h, Y && g && u, "null pointer");
+  DISPATCH(dtype, apply_t, h, nl, n, nq, nsrc, Y, g, v, u);
+}
+

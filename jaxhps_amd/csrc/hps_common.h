@@ -1,0 +1,128 @@
+// Common definitions for the HPS MI355X (gfx950) kernels.
+//
+// Scalar types: double, float, and interleaved complex (re, im) of each.
+// All matrices are row-major with an explicit leading dimension (row stride)
+// and a batch stride; a batch stride of 0 broadcasts one matrix to the batch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+
+namespace hps {
+
+struct c64 {
+  float re, im;
+};
+struct c128 {
+  double re, im;
+};
+
+template <typename T> struct real_of { using type = T; };
+template <> struct real_of<c64> { using type = float; };
+template <> struct real_of<c128> { using type = double; };
+
+template <typename T> struct is_cplx { static constexpr bool value = false; };
+template <> struct is_cplx<c64> { static constexpr bool value = true; };
+template <> struct is_cplx<c128> { static constexpr bool value = true; };
+
+// ---- scalar helpers (device + host) ---------------------------------------
+__host__ __device__ inline double s_re(double x) { return x; }
+__host__ __device__ inline float s_re(float x) { return x; }
+__host__ __device__ inline double s_re(c128 x) { return x.re; }
+__host__ __device__ inline float s_re(c64 x) { return x.re; }
+__host__ __device__ inline double s_im(double) { return 0.0; }
+__host__ __device__ inline float s_im(float) { return 0.0f; }
+__host__ __device__ inline double s_im(c128 x) { return x.im; }
+__host__ __device__ inline float s_im(c64 x) { return x.im; }
+
+template <typename T> __host__ __device__ inline T s_zero() { return T(0); }
+template <> __host__ __device__ inline c64 s_zero<c64>() { return c64{0.f, 0.f}; }
+template <> __host__ __device__ inline c128 s_zero<c128>() { return c128{0.0, 0.0}; }
+
+template <typename T> __host__ __device__ inline T s_one() { return T(1); }
+template <> __host__ __device__ inline c64 s_one<c64>() { return c64{1.f, 0.f}; }
+template <> __host__ __device__ inline c128 s_one<c128>() { return c128{1.0, 0.0}; }
+
+__host__ __device__ inline double s_add(double a, double b) { return a + b; }
+__host__ __device__ inline float s_add(float a, float b) { return a + b; }
+__host__ __device__ inline c128 s_add(c128 a, c128 b) { return c128{a.re + b.re, a.im + b.im}; }
+__host__ __device__ inline c64 s_add(c64 a, c64 b) { return c64{a.re + b.re, a.im + b.im}; }
+
+__host__ __device__ inline double s_sub(double a, double b) { return a - b; }
+__host__ __device__ inline float s_sub(float a, float b) { return a - b; }
+__host__ __device__ inline c128 s_sub(c128 a, c128 b) { return c128{a.re - b.re, a.im - b.im}; }
+__host__ __device__ inline c64 s_sub(c64 a, c64 b) { return c64{a.re - b.re, a.im - b.im}; }
+
+__host__ __device__ inline double s_mul(double a, double b) { return a * b; }
+__host__ __device__ inline float s_mul(float a, float b) { return a * b; }
+__host__ __device__ inline c128 s_mul(c128 a, c128 b) {
+  return c128{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__host__ __device__ inline c64 s_mul(c64 a, c64 b) {
+  return c64{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+// a - b*c
+template <typename T> __host__ __device__ inline T s_fnms(T a, T b, T c) { return s_sub(a, s_mul(b, c)); }
+
+__host__ __device__ inline double s_scale(double a, double s) { return a * s; }
+__host__ __device__ inline float s_scale(float a, double s) { return a * (float)s; }
+__host__ __device__ inline c128 s_scale(c128 a, double s) { return c128{a.re * s, a.im * s}; }
+__host__ __device__ inline c64 s_scale(c64 a, double s) { return c64{a.re * (float)s, a.im * (float)s}; }
+
+__host__ __device__ inline double s_neg(double a) { return -a; }
+__host__ __device__ inline float s_neg(float a) { return -a; }
+__host__ __device__ inline c128 s_neg(c128 a) { return c128{-a.re, -a.im}; }
+__host__ __device__ inline c64 s_neg(c64 a) { return c64{-a.re, -a.im}; }
+
+// 1 / a
+__host__ __device__ inline double s_inv(double a) { return 1.0 / a; }
+__host__ __device__ inline float s_inv(float a) { return 1.0f / a; }
+__host__ __device__ inline c128 s_inv(c128 a) {
+  // Smith's algorithm for a robust complex reciprocal.
+  if (fabs(a.re) >= fabs(a.im)) {
+    double r = a.im / a.re, d = a.re + a.im * r;
+    return c128{1.0 / d, -r / d};
+  } else {
+    double r = a.re / a.im, d = a.re * r + a.im;
+    return c128{r / d, -1.0 / d};
+  }
+}
+__host__ __device__ inline c64 s_inv(c64 a) {
+  if (fabsf(a.re) >= fabsf(a.im)) {
+    float r = a.im / a.re, d = a.re + a.im * r;
+    return c64{1.0f / d, -r / d};
+  } else {
+    float r = a.re / a.im, d = a.re * r + a.im;
+    return c64{r / d, -1.0f / d};
+  }
+}
+
+// pivot magnitude |re| + |im| (the BLAS i*amax convention)
+__host__ __device__ inline double s_abs1(double a) { return fabs(a); }
+__host__ __device__ inline float s_abs1(float a) { return fabsf(a); }
+__host__ __device__ inline double s_abs1(c128 a) { return fabs(a.re) + fabs(a.im); }
+__host__ __device__ inline float s_abs1(c64 a) { return fabsf(a.re) + fabsf(a.im); }
+
+__host__ __device__ inline bool s_is_zero(double a) { return a == 0.0; }
+__host__ __device__ inline bool s_is_zero(float a) { return a == 0.0f; }
+__host__ __device__ inline bool s_is_zero(c128 a) { return a.re == 0.0 && a.im == 0.0; }
+__host__ __device__ inline bool s_is_zero(c64 a) { return a.re == 0.0f && a.im == 0.0f; }
+
+// Precision codes shared with the C ABI (include/hps_mi355x.h).
+enum Dtype : int { HPS_F64 = 0, HPS_F32 = 1, HPS_C128 = 2, HPS_C64 = 3 };
+
+inline size_t dtype_size(int dt) {
+  switch (dt) {
+    case HPS_F64: return 8;
+    case HPS_F32: return 4;
+    case HPS_C128: return 16;
+    case HPS_C64: return 8;
+  }
+  return 0;
+}
+
+inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace hps

@@ -1,0 +1,49 @@
+// Internal kernel launchers shared by the translation units of libhps_mi355x.
+#pragma once
+
+#include "hps_common.h"
+
+namespace hps {
+
+// D[b] = alpha * A[b] @ B[b] + beta * C[b]   (row-major, per batch element)
+// A: M x K (lda), B: K x N (ldb), C/D: M x N (ldc/ldd).  s* are batch strides
+// (0 broadcasts).  C may be null when beta == 0.  C and D may alias.
+template <typename T>
+struct GemmArgs {
+  int M = 0, N = 0, K = 0;
+  double alpha = 1.0, beta = 0.0;
+  const T* A = nullptr;
+  long lda = 0, sA = 0;
+  const T* B = nullptr;
+  long ldb = 0, sB = 0;
+  const T* C = nullptr;
+  long ldc = 0, sC = 0;
+  T* D = nullptr;
+  long ldd = 0, sD = 0;
+  int batch = 1;
+};
+
+template <typename T>
+hipError_t gemm(const GemmArgs<T>& a, hipStream_t s);
+
+// In-place batched LU with partial pivoting of the n x n block of each A[b]
+// (row stride lda, batch stride sA): P A = L U.  perm[b*n + i] is the row of
+// A that became row i of P A.  ipiv is scratch (batch*n ints, LAPACK-style
+// 0-based swap sequence).  info[b] = 0, or j+1 for the first column j whose
+// pivot candidates are all exactly zero.
+template <typename T>
+hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, int* ipiv, int* info, int batch, hipStream_t s);
+
+// X = A^{-1} R from getrf_ws's factors.  R (n x nrhs, row stride ldr, batch
+// stride sR) is read only; X must not alias R.
+template <typename T>
+hipError_t getrs_perm(int n, int nrhs, const T* LU, long lda, long sA, const int* perm, const T* R, long ldr,
+                      long sR, T* X, long ldx, long sX, int batch, hipStream_t s);
+
+// dst[b][r][c] = alpha * src[b][r][c]; src == nullptr writes zeros; a batch
+// stride of 0 on src broadcasts.
+template <typename T>
+hipError_t copy2d(int rows, int cols, double alpha, const T* src, long lds, long ss, T* dst, long ldd, long sd,
+                  int batch, hipStream_t s);
+
+}  // namespace hps

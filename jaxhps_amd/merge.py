@@ -1,0 +1,93 @@
+"""Merge stage, named and shaped like the reference's ``merge`` package.
+
+Every tree level is one ``hps_merge_level`` call (batched over the level's
+merges); the index tables in ``_tables`` carry the reference's block
+structure, rolls, flips and row permutation.
+"""
+
+from __future__ import annotations
+
+from typing import List
+
+import torch
+
+from ._lib import raise_on_info
+from ._stages import merge_level
+from ._tables import quad_merge_tables
+
+
+def _levels(T_arr, h_arr, l, kind, nc, table_fn, keep_B=False):
+    """Run max(l-1, 0) batched levels plus the root merge, as the reference's
+    merge stages do (``merge/_uniform_2D_DtN.py:90-203``)."""
+    nec = T_arr.shape[-1]
+    T = T_arr.reshape(-1, nec, nec)
+    multi = h_arr is not None and h_arr.ndim == T_arr.ndim
+    if h_arr is None:
+        h = None
+    else:
+        h = h_arr.reshape(T.shape[0], nec, -1) if multi else h_arr.reshape(T.shape[0], nec, 1)
+    out = []
+    n_levels = max(l - 1, 0) + 1
+    for lvl in range(n_levels):
+        if lvl == n_levels - 1:
+            T = T[:nc]
+            h = h[:nc] if h is not None else None
+        nec = T.shape[-1]
+        tb = table_fn(nec)
+        res = merge_level(T.contiguous(), h.contiguous() if h is not None else None, tb, kind == "iti", keep_B=keep_B)
+        out.append(res)
+        T, h = res.T, res.h
+    return out, multi
+
+
+def _pack_outputs(levels, multi, return_T, what):
+    for r in levels:
+        raise_on_info(r.info, what)
+    S_lst = [r.S for r in levels]
+    if multi:
+        g_lst = [r.gt for r in levels]
+    else:
+        g_lst = [r.gt[..., 0] for r in levels]
+    if return_T:
+        return S_lst, g_lst, levels[-1].T[0]
+    return S_lst, g_lst
+
+
+def merge_stage_uniform_2D_DtN(
+    T_arr, h_arr, l: int, device=None, host_device=None, subtree_recomp: bool = False, return_T: bool = False
+):
+    """DtN 4:1 merges (``merge/_uniform_2D_DtN.py:13-203``).
+
+    Returns S_lst (level j: (4^(l-1-j), 4m, 8m); root (1, 4m, 8m)),
+    g_tilde_lst and optionally the top T.  subtree_recomp=True returns
+    (T_top[None], h_top[None]) like the reference."""
+    levels, multi = _levels(T_arr, h_arr, l, "dtn", 4, lambda nec: quad_merge_tables(nec // 4, "dtn"))
+    if subtree_recomp:
+        top = levels[-1]
+        hh = top.h if multi else top.h[..., 0]
+        return top.T, hh
+    return _pack_outputs(levels, multi, return_T, "merge_stage_uniform_2D_DtN")
+
+
+def merge_stage_uniform_2D_ItI(
+    T_arr,
+    h_arr,
+    l: int,
+    device=None,
+    host_device=None,
+    subtree_recomp: bool = False,
+    return_T: bool = False,
+    return_h: bool = False,
+):
+    """ItI 4:1 merges (``merge/_uniform_2D_ItI.py:19-179``).  S rows and g_tilde
+    come out in the reference's order a5, b5, b6, c6, c7, d7, d8, a8."""
+    levels, multi = _levels(T_arr, h_arr, l, "iti", 4, lambda nec: quad_merge_tables(nec // 4, "iti"))
+    if subtree_recomp:
+        top = levels[-1]
+        hh = top.h if multi else top.h[..., 0]
+        return top.T, hh
+    out = _pack_outputs(levels, multi, return_T, "merge_stage_uniform_2D_ItI")
+    if return_h:
+        top = levels[-1]
+        out = tuple(out) + ((top.h[0] if multi else top.h[0, :, 0]),)
+    return out

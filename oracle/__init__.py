@@ -1,0 +1,32 @@
+"""ORACLE -- CPU FP64 restatement of jaxhps's uniform HPS build/solve.
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and
+the CPU-baseline leg of ``bench.py`` may import, call or link anything in
+this package, and only as the checker (or the timed CPU baseline) -- never as
+the product path.  The product (``jaxhps_amd``) must fail loudly when its HIP
+library is missing; it never falls back to this code.
+
+Modules
+-------
+spectral       leaf operators and point sets, built independently from
+               coordinates and textbook formulas
+hps2d          stage-by-stage restatement of the reference's 2D DtN / ItI
+               leaf solve, merges, down pass and ItI up pass
+hps3d          3D DtN leaf/merge/down pass restatement
+global_system  the global HPS discrete system solved densely (cross-check)
+
+Pinning (what checks each function beyond re-typing it):
+* spectral: textbook D-matrices n=2,3,4 and Chebyshev/Gauss points from the
+  reference's own tests; exactness on polynomials; agreement with the
+  product's independently written operators.
+* hps2d leaf/merge/down: the reference's closed-form accuracy cases at
+  atol 1e-12 (polynomials) / 1e-8 (Helmholtz); global_system at 1e-11.
+* hps2d up pass / no-source: equality with the with-source path, as the
+  reference's ``test_nosource_accuracy.py`` checks.
+* hps3d: manufactured polynomial solutions (exact up to rounding) and the
+  3D global system on tiny inputs.
+"""
+
+from . import spectral, hps2d, global_system  # noqa: F401
+
+__all__ = ["spectral", "hps2d", "global_system"]
