@@ -1,0 +1,175 @@
+#!/usr/bin/env python
+"""Benchmark: HPS build_solver + solve throughput (DOF/s) on one MI355X.
+
+A step is one build_solver() + solve() of the workload (all right-hand
+sides), with coefficients, source and boundary data already in HBM.  DOF =
+n_leaves * p^d.  Prints ONE JSON line (rank 0).
+
+Default workload: 2D Helmholtz, ItI merges, k = 100, p = 16, L = 7 (16384
+leaves, 4,194,304 DOF), complex128 ("fp64" path).  --config B selects the 2D
+variable-coefficient DtN workload (p = 16, L = 6, 16 RHS).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="C", choices=["B", "C"])
+    ap.add_argument("--L", type=int, default=None, help="override tree depth (testing only)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile-only", action="store_true", help="skip the CPU baseline and extra passes")
+    return ap.parse_args()
+
+
+def make_workload(cfg, L=None):
+    from jaxhps_amd import workloads as wl
+
+    if cfg == "C":
+        dom, kw, g = wl.config_C(L=L or 7)
+        desc = dict(workload="2D Helmholtz ItI k=100 p=16", L=dom.L, n_leaves=dom.n_leaves, p=16, q=14, nsrc=1)
+    else:
+        dom, kw, g = wl.config_B(L=L or 6)
+        desc = dict(workload="2D var-coeff elliptic DtN p=16", L=dom.L, n_leaves=dom.n_leaves, p=16, q=14, nsrc=16)
+    return dom, kw, g, desc
+
+
+def cpu_baseline(cfg):
+    """The FP64 oracle (numpy/LAPACK on the host cores) on a bounded sample of
+    the same workload: the same p and coefficients at L = 4 (256 leaves)."""
+    import numpy as np
+
+    from jaxhps_amd import workloads as wl
+    from oracle import hps2d
+    from oracle import spectral as osp
+
+    L = 4
+    if cfg == "C":
+        dom, kw, g = wl.config_C(L=L)
+        kind = "iti"
+    else:
+        dom, kw, g = wl.config_B(L=L)
+        kind = "dtn"
+    p, q = dom.p, dom.q
+    half = 2.0 / 2 ** (L + 1)
+    ops = osp.leaf_ops_2D(p, q, half, eta=kw.get("eta"))
+    names = {"D_xx_coefficients": "D_xx", "D_yy_coefficients": "D_yy", "D_x_coefficients": "D_x",
+             "D_y_coefficients": "D_y", "I_coefficients": "I"}
+    coeffs = {names[k]: v for k, v in kw.items() if k in names}
+    src = kw["source"]
+    src = src if src.ndim == 3 else src[..., None]
+    gg = g if g.ndim == 2 else g[:, None]
+
+    def run():
+        A = hps2d.assemble_diff_operator(coeffs, ops)
+        if kind == "dtn":
+            Y, T, v, h = hps2d.leaf_dtn(A, src, ops["P"], ops["Q"])
+        else:
+            T, Y, h, v = hps2d.leaf_iti(A, src.astype(complex), ops["P_iti"], ops["QH"], ops["G"])
+        S, gl, _, _ = hps2d.merge_stage(T, h, L, kind)
+        u, _ = hps2d.down_pass(gg, S, gl, Y, v, kind)
+        return u
+
+    run()
+    times = []
+    t_end = time.time() + 20.0
+    while len(times) < 3 and (time.time() < t_end or not times):
+        t0 = time.perf_counter()
+        run()
+        times.append(time.perf_counter() - t0)
+    t = sorted(times)[len(times) // 2]
+    dof = dom.n_leaves * p * p
+    cores = len(os.sched_getaffinity(0))
+    return {
+        "value": dof / t,
+        "unit": "DOF/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"FP64 numpy oracle, full build+solve at L={L} ({dom.n_leaves} leaves, {dof} DOF), "
+        f"median of {len(times)} runs, BLAS threads <= {cores}",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(
+            f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+            "(the multi-GPU design is documented, not built)",
+            file=sys.stderr,
+        )
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import jaxhps_amd as jh
+    from jaxhps_amd import _lib, profiling
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    dom, kw, g, desc = make_workload(args.config, args.L)
+    # inputs resident in HBM before the timed region
+    dkw = {}
+    for k, v in kw.items():
+        if isinstance(v, np.ndarray):
+            dkw[k] = torch.as_tensor(v).to(dev)
+        else:
+            dkw[k] = v
+    pde = jh.PDEProblem(dom, **dkw)
+    g_dev = torch.as_tensor(g).to(dev)
+
+    def step():
+        jh.build_solver(pde)
+        return jh.solve(pde, g_dev)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        u = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) / args.steps * 1e3
+    dof = dom.n_leaves * dom.p ** (2 if dom.bool_2D else 3)
+
+    out = {
+        "metric": "build+solve DOF/s (FP64 GPU path; mixed-precision path not yet built)",
+        "value": dof / (ms / 1e3),
+        "unit": "DOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128" if args.config == "C" else "f64",
+        "data": "synthetic (seeded coefficient fields and boundary data of the config)",
+        "config": desc,
+    }
+    if not args.profile_only:
+        # per-stage breakdown and dominant-kernel roofline from one instrumented step
+        prof = profiling.instrumented_step(pde, g_dev)
+        out["stages_ms"] = prof["stages_ms"]
+        out["roofline"] = prof["roofline"]
+        out["kernels"] = prof["kernels"]
+    if not args.no_cpu_baseline and not args.profile_only:
+        out["cpu_baseline"] = cpu_baseline(args.config)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -63,16 +63,20 @@ int hps_gemm_batched(hps_handle_t h, int dtype, int M, int N, int K, double alph
                      void* D, long ldd, long sD, int batch);
 
 /* In-place LU with partial pivoting, P A = L U; perm[b*n+i] = source row of
- * row i.  Replaces the factorisation inside jnp.linalg.inv
+ * row i; dinv[b] (hps_dinv_size(n) elements per matrix) = packed inverses of
+ * the 32x32 diagonal blocks (strict lower: inv(L_kk), upper: inv(U_kk)),
+ * which turn every triangular solve into parallel block products.
+ * n <= 12288.  Replaces the factorisation inside jnp.linalg.inv
  * (local_solve/_uniform_2D_DtN.py:257, merge/_schur_complement.py:107,146). */
 size_t hps_getrf_workspace(int n, int batch);
-int hps_getrf_batched(hps_handle_t h, int dtype, int n, void* A, long lda, long sA, int* perm, int* info, int batch,
-                      void* ws, size_t ws_bytes);
+size_t hps_dinv_size(int n);
+int hps_getrf_batched(hps_handle_t h, int dtype, int n, void* A, long lda, long sA, int* perm, void* dinv, int* info,
+                      int batch, void* ws, size_t ws_bytes);
 
 /* X = A^{-1} R from hps_getrf_batched's factors (out of place, X != R).
  * Replaces the products with inv(A) (local_solve/_uniform_2D_DtN.py:260-270). */
 int hps_getrs_batched(hps_handle_t h, int dtype, int n, int nrhs, const void* LU, long lda, long sA, const int* perm,
-                      const void* R, long ldr, long sR, void* X, long ldx, long sX, int batch);
+                      const void* dinv, const void* R, long ldr, long sR, void* X, long ldx, long sX, int batch);
 
 /* ---- leaf stage -----------------------------------------------------------
  * Batched leaf solve over nl leaves of n = p^d collocation points, nb
@@ -84,7 +88,8 @@ int hps_getrs_batched(hps_handle_t h, int dtype, int n, int nrhs, const void* LU
  *   source   (nl, n, nsrc) or NULL (zero source)
  * Outputs Y (nl, n, nq), T (nl, nq, nq), v (nl, n, nsrc), h (nl, nq, nsrc),
  *   lu:   DtN (nl, n-nb, n) = [A_ie | LU(A_ii)];  ItI (nl, n, n) = LU(B)
- *   perm: DtN (nl, n-nb);  ItI (nl, n);  info (nl).
+ *   perm: DtN (nl, n-nb);  ItI (nl, n);  dinv: (nl, hps_dinv_size(rows));
+ *   info (nl).
  * DtN replaces local_solve_stage_uniform_2D_DtN + get_DtN
  *   (local_solve/_uniform_2D_DtN.py:9-102,182-273) and the 3D version
  *   (local_solve/_uniform_3D_DtN.py:13-106);
@@ -96,7 +101,7 @@ size_t hps_leaf_workspace(int dtype, int kind, int nl, int n, int nb, int nq, in
 int hps_leaf_solve(hps_handle_t h, int dtype, int kind, int nl, int n, int nb, int nq, int nsrc, int ncoef,
                    const int* coef_ops, const void* coeffs, const void* diffops, const void* P, const void* Q,
                    const void* G, const void* source, void* Y, void* T, void* v, void* hvec, void* lu, int* perm,
-                   int* info, void* ws, size_t ws_bytes);
+                   void* dinv, int* info, void* ws, size_t ws_bytes);
 
 /* ---- merge stage (one tree level) ------------------------------------------
  * nm merges of nc children each (4 in 2D, 8 in 3D); children's maps
@@ -107,21 +112,32 @@ int hps_leaf_solve(hps_handle_t h, int dtype, int kind, int nl, int n, int nb, i
  *   int_cols (ni, 4): for interface unknown j, the (child, local) slots it feeds
  *   ext      (ne, 4): (child, local, -1, 0) per merged exterior node, in the
  *                     merged node's final order (the reference's roll folded in)
- * Solves D g_int = -(C g_ext + h_int) with D = gather(int_rows, int_cols)
- * (+ I when add_identity, the ItI structure I + [[0,D12],[D21,0]]):
- *   S_out (nm, ni, ne) = -D^{-1} C,   gt_out (nm, ni, nsrc) = -D^{-1} h_int,
- *   T_out (nm, ne, ne) = A + B S,     h_out (nm, ne, nsrc) = h_ext + B gt,
- *   D_lu (nm, ni, ni) + perm (nm, ni) = LU of D (kept for the up pass),
- *   B_out (nm, ne, ni) optional copy of B.
+ *   cols_final (ni, 4): int_cols in the final unknown order (B's columns);
+ *                     NULL = int_cols
+ *   final_of_internal (ni): final row of each internal unknown (ItI only)
+ * The interface system D g_int = -(C g_ext + h_int), D = gather(rows, cols):
+ *   DtN: D is dense; F = LU(D), F (nm, ni, ni).
+ *   ItI: unknowns in internal order [F-half; G-half] give
+ *        D = I + [[0, D12], [D21, 0]]; F = LU(W), W = I - D12 D21
+ *        (nm, ni/2, ni/2), and D^{-1} is applied through the Schur complement
+ *        (merge/_schur_complement.py:79-114).  D12/D21_out (nm, ni/2, ni/2)
+ *        optionally keep the blocks.
+ * Outputs (final order):  S_out (nm, ni, ne) = -D^{-1} C,
+ *   gt_out (nm, ni, nsrc) = -D^{-1} h_int,  T_out (nm, ne, ne) = A + B S,
+ *   h_out (nm, ne, nsrc) = h_ext + B gt,  perm/F_dinv/info of F,
+ *   B_out (nm, ne, ni) optional copy of B.  T_out/h_out may be NULL.
+ * upd (host int[4*n_upd]): the nonzero blocks (r0, nr, k0, nk) of B; the
+ *   products B S and B gt use only these (n_upd = 0: B dense).
  * Replaces _uniform_quad_merge_DtN / _ItI with assemble_merge_outputs_*
  *   (merge/_uniform_2D_DtN.py:207-348, merge/_uniform_2D_ItI.py:183-375,
  *   merge/_schur_complement.py:7-237) and _uniform_oct_merge_DtN
  *   (merge/_uniform_3D_DtN.py:128-187, merge/_schur_complement.py:294-774). */
-size_t hps_merge_workspace(int dtype, int nm, int ni, int ne, int nsrc);
-int hps_merge_level(hps_handle_t h, int dtype, int nm, int nc, int nec, int ni, int ne, int nsrc, int add_identity,
+size_t hps_merge_workspace(int dtype, int kind, int nm, int ni, int ne, int nsrc);
+int hps_merge_level(hps_handle_t h, int dtype, int kind, int nm, int nc, int nec, int ni, int ne, int nsrc,
                     const void* T_in, const void* h_in, const int* int_rows, const int* int_cols, const int* ext,
-                    void* S_out, void* T_out, void* h_out, void* gt_out, void* D_lu, int* perm, int* info,
-                    void* B_out, void* ws, size_t ws_bytes);
+                    const int* cols_final, const int* final_of_internal, int n_upd, const int* upd, void* S_out,
+                    void* T_out, void* h_out, void* gt_out, void* F_lu, int* perm, void* F_dinv, int* info,
+                    void* D12_out, void* D21_out, void* B_out, void* ws, size_t ws_bytes);
 
 /* ---- down pass (one tree level) -------------------------------------------
  * g_int = S g_in + gt; children's boundary data g_out (nm*nc, nec, nsrc) from
@@ -137,6 +153,17 @@ int hps_down_level(hps_handle_t h, int dtype, int nm, int nc, int nec, int ni, i
  * pass (down_pass/_uniform_2D_DtN.py:110-120). */
 int hps_leaf_apply(hps_handle_t h, int dtype, int nl, int n, int nq, int nsrc, const void* Y, const void* g,
                    const void* v, void* u);
+
+/* ---- optional kernel timing ----------------------------------------------
+ * When enabled, every kernel launch of the library is bracketed by HIP events
+ * on its stream and attributed to a kernel class ("gemm_mfma<c128>", ...)
+ * with the launch's algorithmic flop and byte counts.  hps_prof_query
+ * synchronises on the recorded events.  Process-wide; not thread-safe with
+ * respect to concurrent enable/reset.  No counterpart in the reference. */
+int hps_prof_enable(int enable);
+int hps_prof_reset(void);
+int hps_prof_count(void);
+int hps_prof_query(int idx, char* name, int name_len, long* launches, double* ms, double* flops, double* bytes);
 
 #ifdef __cplusplus
 }

@@ -39,21 +39,27 @@ SIGNATURES = {
     "hps_version": (_I, []),
     "hps_gemm_batched": (_I, [_P, _I, _I, _I, _I, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _P, _L, _L, _I]),
     "hps_getrf_workspace": (_S, [_I, _I]),
-    "hps_getrf_batched": (_I, [_P, _I, _I, _P, _L, _L, _P, _P, _I, _P, _S]),
-    "hps_getrs_batched": (_I, [_P, _I, _I, _I, _P, _L, _L, _P, _P, _L, _L, _P, _L, _L, _I]),
+    "hps_dinv_size": (_S, [_I]),
+    "hps_getrf_batched": (_I, [_P, _I, _I, _P, _L, _L, _P, _P, _P, _I, _P, _S]),
+    "hps_getrs_batched": (_I, [_P, _I, _I, _I, _P, _L, _L, _P, _P, _P, _L, _L, _P, _L, _L, _I]),
     "hps_leaf_workspace": (_S, [_I, _I, _I, _I, _I, _I, _I]),
     "hps_leaf_solve": (
         _I,
-        [_P, _I, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _S],
+        [_P, _I, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _S],
     ),
-    "hps_merge_workspace": (_S, [_I, _I, _I, _I, _I]),
+    "hps_merge_workspace": (_S, [_I, _I, _I, _I, _I, _I]),
     "hps_merge_level": (
         _I,
-        [_P, _I, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _S],
+        [_P, _I, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+         _P, _P, _P, _P, _S],
     ),
     "hps_down_workspace": (_S, [_I, _I, _I, _I]),
     "hps_down_level": (_I, [_P, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _S]),
     "hps_leaf_apply": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P]),
+    "hps_prof_enable": (_I, [_I]),
+    "hps_prof_reset": (_I, []),
+    "hps_prof_count": (_I, []),
+    "hps_prof_query": (_I, [_I, ctypes.c_char_p, _I, ctypes.POINTER(_L), ctypes.POINTER(_D), ctypes.POINTER(_D), ctypes.POINTER(_D)]),
 }
 
 
@@ -172,20 +178,27 @@ def gemm(A, B, C=None, alpha=1.0, beta=0.0, out=None):
     return out
 
 
+def dinv_size(n: int) -> int:
+    return int(load_library().hps_dinv_size(n))
+
+
 def getrf(A):
-    """In-place batched LU of (batch, n, n) A.  Returns (perm, info)."""
+    """In-place batched LU of (batch, n, n) A.  Returns (perm, dinv, info)."""
     lib = load_library()
     h = handle()
     b, n, _ = A.shape
     perm = torch.empty((b, n), dtype=torch.int32, device=A.device)
+    dinv = torch.empty((b, dinv_size(n)), dtype=A.dtype, device=A.device)
     info = torch.empty((b,), dtype=torch.int32, device=A.device)
     ws = workspace(lib.hps_getrf_workspace(n, b), A.device)
-    rc = lib.hps_getrf_batched(h, dtype_code(A.dtype), n, ptr(A), n, n * n, ptr(perm), ptr(info), b, ptr(ws), ws.numel())
+    rc = lib.hps_getrf_batched(
+        h, dtype_code(A.dtype), n, ptr(A), n, n * n, ptr(perm), ptr(dinv), ptr(info), b, ptr(ws), ws.numel()
+    )
     check(rc, h, "hps_getrf_batched")
-    return perm, info
+    return perm, dinv, info
 
 
-def getrs(LU, perm, R):
+def getrs(LU, perm, dinv, R):
     """X = A^{-1} R for (batch, n, nrhs) R from getrf's factors."""
     lib = load_library()
     h = handle()
@@ -193,7 +206,8 @@ def getrs(LU, perm, R):
     nrhs = R.shape[2]
     X = torch.empty_like(R)
     rc = lib.hps_getrs_batched(
-        h, dtype_code(LU.dtype), n, nrhs, ptr(LU), n, n * n, ptr(perm), ptr(R), nrhs, n * nrhs, ptr(X), nrhs, n * nrhs, b
+        h, dtype_code(LU.dtype), n, nrhs, ptr(LU), n, n * n, ptr(perm), ptr(dinv), ptr(R), nrhs, n * nrhs, ptr(X),
+        nrhs, n * nrhs, b,
     )
     check(rc, h, "hps_getrs_batched")
     return X

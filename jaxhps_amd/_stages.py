@@ -64,6 +64,7 @@ class LeafResult:
     h: Optional[torch.Tensor]
     lu: torch.Tensor
     perm: torch.Tensor
+    dinv: torch.Tensor
     info: torch.Tensor
     kind: int
 
@@ -104,16 +105,17 @@ def leaf_solve(pde, kind: int, dtype, device, source=None, nsrc_zero=False) -> L
     hv = torch.empty((nl, nq, nsrc), dtype=dtype, device=device) if nsrc else None
     lu = torch.empty((nl, nrows, n), dtype=dtype, device=device)
     perm = torch.empty((nl, nrows), dtype=torch.int32, device=device)
+    dinv = torch.empty((nl, _lib.dinv_size(nrows)), dtype=dtype, device=device)
     info = torch.empty((nl,), dtype=torch.int32, device=device)
     ws = workspace(lib.hps_leaf_workspace(dtype_code(dtype), kind, nl, n, nb, nq, nsrc), device)
     cop = coef_ops.ctypes.data_as(ctypes.c_void_p)
     rc = lib.hps_leaf_solve(
         h, dtype_code(dtype), kind, nl, n, nb, nq, nsrc, len(clist), cop,
         ptr(coeffs), ptr(ops["diffops"]), ptr(P), ptr(Q), ptr(ops.get("G")), ptr(src),
-        ptr(Y), ptr(T), ptr(v), ptr(hv), ptr(lu), ptr(perm), ptr(info), ptr(ws), ws.numel(),
+        ptr(Y), ptr(T), ptr(v), ptr(hv), ptr(lu), ptr(perm), ptr(dinv), ptr(info), ptr(ws), ws.numel(),
     )
     check(rc, h, "hps_leaf_solve")
-    return LeafResult(Y, T, v, hv, lu, perm, info, kind)
+    return LeafResult(Y, T, v, hv, lu, perm, dinv, info, kind)
 
 
 @dataclass
@@ -122,40 +124,53 @@ class MergeLevel:
     T: Optional[torch.Tensor]
     h: Optional[torch.Tensor]
     gt: Optional[torch.Tensor]
-    D_lu: torch.Tensor
+    F_lu: torch.Tensor          # LU of D (DtN) or of W = I - D12 D21 (ItI)
     perm: torch.Tensor
+    dinv: torch.Tensor
     info: torch.Tensor
+    D12: Optional[torch.Tensor]
+    D21: Optional[torch.Tensor]
     B: Optional[torch.Tensor]
     tables: MergeTables
 
 
-def merge_level(T_in, h_in, tables: MergeTables, add_identity: bool, keep_B=False, want_T=True) -> MergeLevel:
-    """One level of 4:1 / 8:1 merges.  T_in (nm*nc, nec, nec), h_in (nm*nc, nec, nsrc) or None."""
+def merge_level(T_in, h_in, tables: MergeTables, keep_factors=False, want_T=True) -> MergeLevel:
+    """One level of 4:1 / 8:1 merges.  T_in (nm*nc, nec, nec), h_in (nm*nc, nec, nsrc) or None.
+    keep_factors also keeps B (and D12/D21 for ItI) for the up pass."""
     lib = load_library()
     hd = _lib.handle()
     dev = T_in.device
     dt = T_in.dtype
     nc, nec, ni, ne = tables.nc, tables.nec, tables.ni, tables.ne
+    kind = HPS_KIND_ITI if tables.kind == "iti" else HPS_KIND_DTN
     nm = T_in.shape[0] // nc
     assert T_in.shape == (nm * nc, nec, nec), (T_in.shape, nm, nc, nec)
     nsrc = 0 if h_in is None else h_in.shape[-1]
-    rows, cols, ext, _ = tables.device(dev)
+    nf = ni // 2 if kind == HPS_KIND_ITI else ni
+    rows, cols, ext, _, cols_final, final_of_internal = tables.device(dev)
     S = torch.empty((nm, ni, ne), dtype=dt, device=dev)
     T = torch.empty((nm, ne, ne), dtype=dt, device=dev) if want_T else None
-    ho = torch.empty((nm, ne, nsrc), dtype=dt, device=dev) if nsrc else None
+    ho = torch.empty((nm, ne, nsrc), dtype=dt, device=dev) if (nsrc and want_T) else None
     gt = torch.empty((nm, ni, nsrc), dtype=dt, device=dev) if nsrc else None
-    D_lu = torch.empty((nm, ni, ni), dtype=dt, device=dev)
-    perm = torch.empty((nm, ni), dtype=torch.int32, device=dev)
+    F_lu = torch.empty((nm, nf, nf), dtype=dt, device=dev)
+    perm = torch.empty((nm, nf), dtype=torch.int32, device=dev)
+    dinv = torch.empty((nm, _lib.dinv_size(nf)), dtype=dt, device=dev)
     info = torch.empty((nm,), dtype=torch.int32, device=dev)
-    B = torch.empty((nm, ne, ni), dtype=dt, device=dev) if keep_B else None
-    ws = workspace(lib.hps_merge_workspace(dtype_code(dt), nm, ni, ne, nsrc), dev)
+    keep12 = keep_factors and kind == HPS_KIND_ITI
+    D12 = torch.empty((nm, nf, nf), dtype=dt, device=dev) if keep12 else None
+    D21 = torch.empty((nm, nf, nf), dtype=dt, device=dev) if keep12 else None
+    B = torch.empty((nm, ne, ni), dtype=dt, device=dev) if keep_factors else None
+    ws = workspace(lib.hps_merge_workspace(dtype_code(dt), kind, nm, ni, ne, nsrc), dev)
+    upd = np.ascontiguousarray(tables.upd, dtype=np.int32)
     rc = lib.hps_merge_level(
-        hd, dtype_code(dt), nm, nc, nec, ni, ne, nsrc, int(add_identity),
-        ptr(T_in), ptr(h_in), ptr(rows), ptr(cols), ptr(ext),
-        ptr(S), ptr(T), ptr(ho), ptr(gt), ptr(D_lu), ptr(perm), ptr(info), ptr(B), ptr(ws), ws.numel(),
+        hd, dtype_code(dt), kind, nm, nc, nec, ni, ne, nsrc,
+        ptr(T_in), ptr(h_in), ptr(rows), ptr(cols), ptr(ext), ptr(cols_final), ptr(final_of_internal),
+        upd.shape[0], upd.ctypes.data_as(ctypes.c_void_p),
+        ptr(S), ptr(T), ptr(ho), ptr(gt), ptr(F_lu), ptr(perm), ptr(dinv), ptr(info),
+        ptr(D12), ptr(D21), ptr(B), ptr(ws), ws.numel(),
     )
     check(rc, hd, "hps_merge_level")
-    return MergeLevel(S, T, ho, gt, D_lu, perm, info, B, tables)
+    return MergeLevel(S, T, ho, gt, F_lu, perm, dinv, info, D12, D21, B, tables)
 
 
 def down_level(S, gt, g_in, tables: MergeTables):
@@ -167,7 +182,7 @@ def down_level(S, gt, g_in, tables: MergeTables):
     nc, nec, ni, ne = tables.nc, tables.nec, tables.ni, tables.ne
     nm = S.shape[0]
     nsrc = g_in.shape[-1]
-    _, _, _, src = tables.device(dev)
+    src = tables.device(dev)[3]
     g_out = torch.empty((nm * nc, nec, nsrc), dtype=dt, device=dev)
     ws = workspace(lib.hps_down_workspace(dtype_code(dt), nm, ni, nsrc), dev)
     rc = lib.hps_down_level(

@@ -62,7 +62,17 @@ def _ext_2D(m: int):
 
 
 class MergeTables:
-    def __init__(self, nc, nec, int_rows, int_cols, ext, child_src):
+    """Index tables of one merge level.
+
+    int_rows/int_cols are in the *internal* unknown order the factorisation
+    uses; cols_final is int_cols in the final (stored) order and
+    final_of_internal maps internal -> final rows (identity for DtN).  upd
+    lists the nonzero (r0, nr, k0, nk) blocks of B (exterior x unknowns).
+    """
+
+    def __init__(self, kind, nc, nec, int_rows, int_cols, ext, child_src, cols_final=None, final_of_internal=None,
+                 block=None):
+        self.kind = kind
         self.nc = nc
         self.nec = nec
         self.ni = int_rows.shape[0]
@@ -71,6 +81,11 @@ class MergeTables:
         self.int_cols = int_cols
         self.ext = ext
         self.child_src = child_src
+        self.cols_final = int_cols if cols_final is None else cols_final
+        self.final_of_internal = (
+            np.arange(self.ni, dtype=np.int32) if final_of_internal is None else final_of_internal.astype(np.int32)
+        )
+        self.upd = _sparse_blocks(self.ext, self.cols_final, block) if block else np.zeros((0, 4), np.int32)
         self._dev = {}
 
     def device(self, dev):
@@ -78,9 +93,41 @@ class MergeTables:
         if key not in self._dev:
             self._dev[key] = tuple(
                 torch.from_numpy(np.ascontiguousarray(x, dtype=np.int32)).to(dev)
-                for x in (self.int_rows, self.int_cols, self.ext, self.child_src)
+                for x in (self.int_rows, self.int_cols, self.ext, self.child_src, self.cols_final,
+                          self.final_of_internal)
             )
         return self._dev[key]
+
+
+def _children(entry):
+    return {int(entry[0])} | ({int(entry[2])} if entry[2] >= 0 else set())
+
+
+def _sparse_blocks(ext, cols_final, blk):
+    """Nonzero blocks of B = gather(ext, cols_final) at block size blk, merged
+    into rectangles: consecutive exterior blocks with the same set of unknown
+    blocks share a rectangle; consecutive unknown blocks form one run."""
+    ne, ni = ext.shape[0], cols_final.shape[0]
+    assert ne % blk == 0 and ni % blk == 0
+    eb = [set().union(*[_children(ext[i]) for i in range(b * blk, (b + 1) * blk)]) for b in range(ne // blk)]
+    ub = [set().union(*[_children(cols_final[i]) for i in range(b * blk, (b + 1) * blk)]) for b in range(ni // blk)]
+    sets = [tuple(u for u in range(len(ub)) if eb[e] & ub[u]) for e in range(len(eb))]
+    rects = []
+    e = 0
+    while e < len(sets):
+        e1 = e
+        while e1 + 1 < len(sets) and sets[e1 + 1] == sets[e]:
+            e1 += 1
+        us = sets[e]
+        i = 0
+        while i < len(us):
+            j = i
+            while j + 1 < len(us) and us[j + 1] == us[j] + 1:
+                j += 1
+            rects.append((e * blk, (e1 - e + 1) * blk, us[i] * blk, (us[j] - us[i] + 1) * blk))
+            i = j + 1
+        e = e1 + 1
+    return np.array(rects, dtype=np.int32).reshape(-1, 4)
 
 
 def _pack(pairs):
@@ -117,6 +164,9 @@ def quad_merge_tables(m: int, kind: str) -> MergeTables:
         rows = _pack(nodes)
         cols = rows.copy()
         unknown_slots = [list(p) for p in nodes]
+        ext_arr = _pack([[e] for e in ext])
+        src = _child_src(4, nec, ext, unknown_slots)
+        return MergeTables("dtn", 4, nec, rows, cols, ext_arr, src, block=m)
     elif kind == "iti":
         # (child, local) of the unknown and of its partner across the interface
         a5 = [(p[0], p[1]) for p in i5]
@@ -127,12 +177,19 @@ def quad_merge_tables(m: int, kind: str) -> MergeTables:
         d7 = [(p[1], p[0]) for p in i7]
         d8 = [(p[0], p[1]) for p in i8]
         a8 = [(p[1], p[0]) for p in i8]
-        order = a5 + b5 + b6 + c6 + c7 + d7 + d8 + a8
-        cols = _pack([[own] for own, _ in order])
-        rows = _pack([[partner] for _, partner in order])
-        unknown_slots = [[own] for own, _ in order]
+        final = a5 + b5 + b6 + c6 + c7 + d7 + d8 + a8
+        # internal order: unknowns of children a, c (F half) then b, d (G half);
+        # the equation of every F unknown involves only G unknowns and vice
+        # versa, so D = I + [[0, D12], [D21, 0]]
+        internal = a5 + a8 + c6 + c7 + b5 + b6 + d7 + d8
+        pos = {own: i for i, (own, _) in enumerate(final)}
+        final_of_internal = np.array([pos[own] for own, _ in internal], dtype=np.int32)
+        cols = _pack([[own] for own, _ in internal])
+        rows = _pack([[partner] for _, partner in internal])
+        cols_final = _pack([[own] for own, _ in final])
+        ext_arr = _pack([[e] for e in ext])
+        src = _child_src(4, nec, ext, [[own] for own, _ in final])
+        return MergeTables("iti", 4, nec, rows, cols, ext_arr, src, cols_final=cols_final,
+                           final_of_internal=final_of_internal, block=m)
     else:
         raise ValueError(kind)
-    ext_arr = _pack([[e] for e in ext])
-    src = _child_src(4, nec, ext, unknown_slots)
-    return MergeTables(4, nec, rows, cols, ext_arr, src)

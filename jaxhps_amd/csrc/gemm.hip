@@ -7,7 +7,10 @@
 // * gemm_skinny_kernel: N <= 16 (down-pass / leaf-apply GEMV shapes).  One
 //   wave per output row, lanes stride the K dimension with coalesced loads
 //   of A's row, then a wave reduction.  These shapes are HBM-bound on A.
+#include <string>
+
 #include "hps_kernels.h"
+#include "hps_prof.h"
 
 namespace hps {
 
@@ -203,12 +206,13 @@ __global__ __launch_bounds__(256) void gemm_skinny_kernel(GemmArgs<T> p) {
 template <typename T>
 hipError_t gemm(const GemmArgs<T>& a, hipStream_t s) {
   if (a.M <= 0 || a.N <= 0 || a.batch <= 0) return hipSuccess;
-  if (a.K <= 0) {
-    // D = beta * C: reuse the skinny path with K = 0 only for small N;
-    // general case handled by the MFMA kernel (the k-loop is empty).
-  }
   unsigned gy = (unsigned)(a.batch < 65535 ? a.batch : 65535);
+  const double flops = 2.0 * a.M * a.N * (double)a.K * a.batch * flop_mult<T>();
+  const double bytes = sizeof(T) * ((double)a.M * a.K * (a.sA ? a.batch : 1) + (double)a.K * a.N * (a.sB ? a.batch : 1) +
+                                    (double)a.M * a.N * a.batch * (a.beta != 0.0 ? 2 : 1));
   if (a.N <= 16) {
+    static const std::string nm = std::string("gemm_skinny<") + tname<T>() + ">";
+    prof::Scope ps(s, nm.c_str(), flops, bytes);
     dim3 grid(ceil_div(a.M, 4), gy);
     if (a.N <= 1)
       hipLaunchKernelGGL((gemm_skinny_kernel<T, 1>), grid, dim3(256), 0, s, a);
@@ -217,6 +221,8 @@ hipError_t gemm(const GemmArgs<T>& a, hipStream_t s) {
     else
       hipLaunchKernelGGL((gemm_skinny_kernel<T, 16>), grid, dim3(256), 0, s, a);
   } else {
+    static const std::string nm = std::string("gemm_mfma<") + tname<T>() + ">";
+    prof::Scope ps(s, nm.c_str(), flops, bytes);
     int tiles_n = ceil_div(a.N, GBN);
     int tiles_m = ceil_div(a.M, GBM);
     dim3 grid((unsigned)(tiles_m * tiles_n), gy);

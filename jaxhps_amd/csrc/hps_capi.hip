@@ -16,7 +16,14 @@
 #include <string>
 
 #include "hps_kernels.h"
+#include "hps_prof.h"
 #include "../../include/hps_mi355x.h"
+
+// Open a timing scope named "<label><dtype>" for the next launch (no-op unless
+// profiling is enabled).
+#define HPS_PROF(label, T, flops, bytes)                                                 \
+  static const std::string _pn_##__LINE__ = std::string(label "<") + tname<T>() + ">"; \
+  prof::Scope _ps_##__LINE__(s, _pn_##__LINE__.c_str(), (flops), (bytes))
 
 namespace hps {
 
@@ -44,6 +51,7 @@ hipError_t copy2d(int rows, int cols, double alpha, const T* src, long lds, long
   unsigned gx = ceil_div(total, 256);
   if (gx > 4096) gx = 4096;
   unsigned gy = batch < 65535 ? batch : 65535;
+  HPS_PROF("copy2d", T, 0.0, (src ? 2.0 : 1.0) * sizeof(T) * (double)total * batch);
   hipLaunchKernelGGL((copy2d_kernel<T>), dim3(gx, gy), dim3(256), 0, s, rows, cols, alpha, src, lds, ss, dst, ldd,
                      sd, batch);
   return hipGetLastError();
@@ -263,41 +271,44 @@ int hps_gemm_batched(hps_handle_t h, int dtype, int M, int N, int K, double alph
   DISPATCH(dtype, gemm_t, h, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch);
 }
 
-size_t hps_getrf_workspace(int n, int batch) { return align_up((size_t)n * batch * sizeof(int)); }
+size_t hps_getrf_workspace(int n, int batch) { return align_up(getrf_scratch_bytes(n, batch)); }
+
+size_t hps_dinv_size(int n) { return dinv_elems(n); }
 
 template <typename T>
-static int getrf_t(hps_handle_t h, int n, void* A, long lda, long sA, int* perm, int* info, int batch, void* ws) {
-  HPS_CHECK_HIP(h, getrf_ws<T>(n, (T*)A, lda, sA, perm, (int*)ws, info, batch, h->stream));
+static int getrf_t(hps_handle_t h, int n, void* A, long lda, long sA, int* perm, void* dinv, int* info, int batch,
+                   void* ws) {
+  HPS_CHECK_HIP(h, getrf_ws<T>(n, (T*)A, lda, sA, perm, (T*)dinv, (int*)ws, info, batch, h->stream));
   return HPS_OK;
 }
 
-int hps_getrf_batched(hps_handle_t h, int dtype, int n, void* A, long lda, long sA, int* perm, int* info, int batch,
-                      void* ws, size_t ws_bytes) {
+int hps_getrf_batched(hps_handle_t h, int dtype, int n, void* A, long lda, long sA, int* perm, void* dinv, int* info,
+                      int batch, void* ws, size_t ws_bytes) {
   if (!h) return HPS_ERR_ARG;
-  HPS_ARG(h, n >= 0 && batch >= 0 && lda >= n, "bad size");
-  HPS_ARG(h, A && perm && info, "null pointer");
+  HPS_ARG(h, n >= 0 && n <= 12288 && batch >= 0 && lda >= n, "bad size (0 <= n <= 12288, lda >= n)");
+  HPS_ARG(h, A && perm && dinv && info, "null pointer");
   if (ws_bytes < hps_getrf_workspace(n, batch)) {
     h->err = "getrf workspace too small";
     return HPS_ERR_WORKSPACE;
   }
-  DISPATCH(dtype, getrf_t, h, n, A, lda, sA, perm, info, batch, ws);
+  DISPATCH(dtype, getrf_t, h, n, A, lda, sA, perm, dinv, info, batch, ws);
 }
 
 template <typename T>
-static int getrs_t(hps_handle_t h, int n, int nrhs, const void* LU, long lda, long sA, const int* perm, const void* R,
-                   long ldr, long sR, void* X, long ldx, long sX, int batch) {
-  HPS_CHECK_HIP(h, getrs_perm<T>(n, nrhs, (const T*)LU, lda, sA, perm, (const T*)R, ldr, sR, (T*)X, ldx, sX, batch,
-                                 h->stream));
+static int getrs_t(hps_handle_t h, int n, int nrhs, const void* LU, long lda, long sA, const int* perm,
+                   const void* dinv, const void* R, long ldr, long sR, void* X, long ldx, long sX, int batch) {
+  HPS_CHECK_HIP(h, getrs_perm<T>(n, nrhs, (const T*)LU, lda, sA, perm, (const T*)dinv, (const T*)R, ldr, sR, (T*)X,
+                                 ldx, sX, batch, h->stream));
   return HPS_OK;
 }
 
 int hps_getrs_batched(hps_handle_t h, int dtype, int n, int nrhs, const void* LU, long lda, long sA, const int* perm,
-                      const void* R, long ldr, long sR, void* X, long ldx, long sX, int batch) {
+                      const void* dinv, const void* R, long ldr, long sR, void* X, long ldx, long sX, int batch) {
   if (!h) return HPS_ERR_ARG;
   HPS_ARG(h, n >= 0 && nrhs >= 0 && batch >= 0, "bad size");
-  HPS_ARG(h, LU && perm && R && X, "null pointer");
+  HPS_ARG(h, LU && perm && dinv && R && X, "null pointer");
   HPS_ARG(h, R != X, "getrs is out of place: R and X must differ");
-  DISPATCH(dtype, getrs_t, h, n, nrhs, LU, lda, sA, perm, R, ldr, sR, X, ldx, sX, batch);
+  DISPATCH(dtype, getrs_t, h, n, nrhs, LU, lda, sA, perm, dinv, R, ldr, sR, X, ldx, sX, batch);
 }
 
 // ---------------------------------------------------------------------------
@@ -307,14 +318,14 @@ size_t hps_leaf_workspace(int dtype, int kind, int nl, int n, int nb, int nq, in
   size_t es = dtype_size(dtype);
   int nrows = kind == HPS_KIND_DTN ? n - nb : n;
   size_t rhs = align_up((size_t)nl * nrows * (nq + nsrc) * es);
-  return 2 * rhs + align_up((size_t)nl * nrows * sizeof(int));
+  return 2 * rhs + align_up(getrf_scratch_bytes(nrows, nl));
 }
 
 template <typename T>
 static int leaf_t(hps_handle_t h, int kind, int nl, int n, int nb, int nq, int nsrc, int ncoef, const int* coef_ops,
                   const void* coeffs, const void* diffops, const void* Pv, const void* Qv, const void* Gv,
-                  const void* source, void* Yv, void* Tv, void* vv, void* hv, void* luv, int* perm, int* info,
-                  void* ws) {
+                  const void* source, void* Yv, void* Tv, void* vv, void* hv, void* luv, int* perm, void* dinvv,
+                  int* info, void* ws) {
   hipStream_t s = h->stream;
   const T* P = (const T*)Pv;
   const T* Q = (const T*)Qv;
@@ -330,7 +341,8 @@ static int leaf_t(hps_handle_t h, int kind, int nl, int n, int nb, int nq, int n
   wp += align_up((size_t)nl * nrows * w * es);
   T* X = (T*)wp;
   wp += align_up((size_t)nl * nrows * w * es);
-  int* ipiv = (int*)wp;
+  int* scratch = (int*)wp;
+  T* dinv = (T*)dinvv;
 
   CoefOps co;
   for (int i = 0; i < 16; ++i) co.op[i] = i < ncoef ? coef_ops[i] : 0;
@@ -341,6 +353,8 @@ static int leaf_t(hps_handle_t h, int kind, int nl, int n, int nb, int nq, int n
   int arow0 = nb, an = ni;
   T* aout = kind == HPS_KIND_DTN ? lu : lu + (long)nb * n;
   {
+    HPS_PROF("leaf_assemble", T, flop_mult<T>() * 2.0 * ncoef * (double)an * n * nl,
+             sizeof(T) * ((double)an * n * nl + (double)ncoef * n * nl));
     dim3 g = grid_for((long)an * n, 1);
     g.y = nl;
     hipLaunchKernelGGL((assemble_kernel<T>), g, dim3(256), 0, s, n, nl, ncoef, co, (const T*)coeffs,
@@ -383,8 +397,9 @@ static int leaf_t(hps_handle_t h, int kind, int nl, int n, int nb, int nq, int n
 
   // 3. factor and solve
   T* Afac = kind == HPS_KIND_DTN ? lu + nb : lu;
-  HPS_CHECK_HIP(h, getrf_ws<T>(nrows, Afac, ld_lu, s_lu, perm, ipiv, info, nl, s));
-  HPS_CHECK_HIP(h, getrs_perm<T>(nrows, w, Afac, ld_lu, s_lu, perm, R, w, (long)nrows * w, X, w, (long)nrows * w, nl, s));
+  HPS_CHECK_HIP(h, getrf_ws<T>(nrows, Afac, ld_lu, s_lu, perm, dinv, scratch, info, nl, s));
+  HPS_CHECK_HIP(h, getrs_perm<T>(nrows, w, Afac, ld_lu, s_lu, perm, dinv, R, w, (long)nrows * w, X, w,
+                                 (long)nrows * w, nl, s));
 
   // 4. unpack Y and v
   if (kind == HPS_KIND_DTN) {
@@ -442,12 +457,12 @@ static int leaf_t(hps_handle_t h, int kind, int nl, int n, int nb, int nq, int n
 int hps_leaf_solve(hps_handle_t h, int dtype, int kind, int nl, int n, int nb, int nq, int nsrc, int ncoef,
                    const int* coef_ops, const void* coeffs, const void* diffops, const void* P, const void* Q,
                    const void* G, const void* source, void* Y, void* T, void* v, void* hvec, void* lu, int* perm,
-                   int* info, void* ws, size_t ws_bytes) {
+                   void* dinv, int* info, void* ws, size_t ws_bytes) {
   if (!h) return HPS_ERR_ARG;
   HPS_ARG(h, kind == HPS_KIND_DTN || kind == HPS_KIND_ITI, "kind must be DtN or ItI");
   HPS_ARG(h, nl > 0 && n > nb && nb > 0 && nq > 0 && nsrc >= 0, "bad leaf sizes");
   HPS_ARG(h, ncoef > 0 && ncoef <= 16 && coef_ops, "1..16 coefficients required");
-  HPS_ARG(h, coeffs && diffops && P && Q && Y && T && lu && perm && info, "null pointer");
+  HPS_ARG(h, coeffs && diffops && P && Q && Y && T && lu && perm && dinv && info, "null pointer");
   HPS_ARG(h, kind == HPS_KIND_DTN || G, "ItI needs G");
   HPS_ARG(h, nsrc == 0 || (v && hvec), "v/h required when nsrc > 0");
   if (ws_bytes < hps_leaf_workspace(dtype, kind, nl, n, nb, nq, nsrc)) {
@@ -455,130 +470,244 @@ int hps_leaf_solve(hps_handle_t h, int dtype, int kind, int nl, int n, int nb, i
     return HPS_ERR_WORKSPACE;
   }
   DISPATCH(dtype, leaf_t, h, kind, nl, n, nb, nq, nsrc, ncoef, coef_ops, coeffs, diffops, P, Q, G, source, Y, T, v,
-           hvec, lu, perm, info, ws);
+           hvec, lu, perm, dinv, info, ws);
 }
 
 // ---------------------------------------------------------------------------
 // merge level
 // ---------------------------------------------------------------------------
-size_t hps_merge_workspace(int dtype, int nm, int ni, int ne, int nsrc) {
+// Workspace layout (bytes, each region 256-aligned):
+//   Y   ni x ne      right-hand sides -C (internal unknown order), solutions
+//   Z   nF x ne      ItI: Y_F - D12 Y_G
+//   Yh  ni x nsrc    right-hand sides -h_int
+//   Zh  nF x nsrc
+//   D12, D21 nF x nF ItI blocks (when the caller does not keep them)
+//   B   ne x ni      (when the caller does not keep it)
+//   getrf scratch
+size_t hps_merge_workspace(int dtype, int kind, int nm, int ni, int ne, int nsrc) {
   size_t es = dtype_size(dtype);
-  size_t big = (size_t)nm * ni * ne * es;
-  return align_up(big) + align_up((size_t)nm * ni * (nsrc > 0 ? nsrc : 1) * es) + align_up((size_t)nm * ni * sizeof(int));
+  const int nF = kind == HPS_KIND_ITI ? ni / 2 : ni;
+  const int nfac = kind == HPS_KIND_ITI ? nF : ni;
+  const int ns = nsrc > 0 ? nsrc : 1;
+  size_t tot = 0;
+  tot += align_up((size_t)nm * ni * ne * es);
+  tot += kind == HPS_KIND_ITI ? align_up((size_t)nm * nF * ne * es) : 0;
+  tot += align_up((size_t)nm * ni * ns * es);
+  tot += kind == HPS_KIND_ITI ? align_up((size_t)nm * nF * ns * es) : 0;
+  tot += kind == HPS_KIND_ITI ? 2 * align_up((size_t)nm * nF * nF * es) : 0;
+  tot += align_up((size_t)nm * ne * ni * es);
+  tot += align_up(getrf_scratch_bytes(nfac, nm));
+  return tot;
+}
+
+namespace hps {
+
+template <typename T>
+__global__ void add_identity_kernel(T* A, int n, long sA, int nm) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int m = blockIdx.y;
+  if (i < n && m < nm) A[(long)m * sA + (long)i * n + i] = s_add(A[(long)m * sA + (long)i * n + i], s_one<T>());
+}
+
+// dst[m][map[i]][c] = src[m][i][c]  (row scatter from the internal to the final unknown order)
+template <typename T>
+__global__ void row_scatter_kernel(const T* src, T* dst, const int* map, int rows, int cols, int nm) {
+  long total = (long)rows * cols;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    int r = (int)(idx / cols), c = (int)(idx - (long)r * cols);
+    for (int m = blockIdx.y; m < nm; m += gridDim.y)
+      dst[((long)m * rows + map[r]) * cols + c] = src[((long)m * rows + r) * cols + c];
+  }
 }
 
 template <typename T>
-static int merge_t(hps_handle_t h, int nm, int nc, int nec, int ni, int ne, int nsrc, int add_identity,
-                   const void* T_in, const void* h_in, const int* int_rows, const int* int_cols, const int* ext,
-                   void* S_out, void* T_out, void* h_out, void* gt_out, void* D_lu, int* perm, int* info,
-                   void* B_out, void* ws) {
+static hipError_t gather_launch(hipStream_t s, const char* label, const T* Tin, int nc, int nec, const int4* rows,
+                                int nr, const int4* cols, int ncl, double sign, int add_id, T* out, long ldo, long so,
+                                int nm) {
+  static const std::string nm_ = std::string("merge_gather<") + tname<T>() + ">";
+  prof::Scope ps(s, nm_.c_str(), 0.0, 2.0 * sizeof(T) * (double)nr * ncl * nm);
+  hipLaunchKernelGGL((merge_gather_kernel<T>), grid_for((long)nr * ncl, nm), dim3(256), 0, s, Tin, nc, nec, rows, nr,
+                     cols, ncl, sign, add_id, out, ldo, so, nm);
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t gemm_call(hipStream_t s, int M, int N, int K, double alpha, const T* A, long lda, long sA,
+                            const T* B, long ldb, long sB, double beta, const T* C, long ldc, long sC, T* D, long ldd,
+                            long sD, int batch) {
+  GemmArgs<T> g;
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.alpha = alpha;
+  g.beta = beta;
+  g.A = A;
+  g.lda = lda;
+  g.sA = sA;
+  g.B = B;
+  g.ldb = ldb;
+  g.sB = sB;
+  g.C = C;
+  g.ldc = ldc;
+  g.sC = sC;
+  g.D = D;
+  g.ldd = ldd;
+  g.sD = sD;
+  g.batch = batch;
+  return gemm<T>(g, s);
+}
+
+}  // namespace hps
+
+template <typename T>
+static int merge_t(hps_handle_t h, int kind, int nm, int nc, int nec, int ni, int ne, int nsrc, const void* T_in,
+                   const void* h_in, const int* int_rows, const int* int_cols, const int* ext,
+                   const int* cols_final, const int* final_of_internal, int n_upd, const int* upd, void* S_out,
+                   void* T_out, void* h_out, void* gt_out, void* F_lu, int* perm, void* F_dinv, int* info,
+                   void* D12_out, void* D21_out, void* B_out, void* ws) {
   hipStream_t s = h->stream;
   const T* Tin = (const T*)T_in;
   const int4* rows = (const int4*)int_rows;
   const int4* cols = (const int4*)int_cols;
   const int4* ex = (const int4*)ext;
-  size_t es = sizeof(T);
+  const int4* colsF = (const int4*)(cols_final ? cols_final : int_cols);
+  const bool iti = kind == HPS_KIND_ITI;
+  const int nF = iti ? ni / 2 : ni;
+  const int ns = nsrc > 0 ? nsrc : 1;
+  const size_t es = sizeof(T);
   char* wp = (char*)ws;
-  T* W1 = (T*)wp;  // -C, later B
-  wp += align_up((size_t)nm * ni * ne * es);
-  T* W2 = (T*)wp;  // -h_int
-  wp += align_up((size_t)nm * ni * (nsrc > 0 ? nsrc : 1) * es);
-  int* ipiv = (int*)wp;
+  auto take = [&](size_t bytes) {
+    char* p = wp;
+    wp += align_up(bytes);
+    return p;
+  };
+  T* Y = (T*)take((size_t)nm * ni * ne * es);
+  T* Z = iti ? (T*)take((size_t)nm * nF * ne * es) : nullptr;
+  T* Yh = (T*)take((size_t)nm * ni * ns * es);
+  T* Zh = iti ? (T*)take((size_t)nm * nF * ns * es) : nullptr;
+  T* D12w = iti ? (T*)take((size_t)nm * nF * nF * es) : nullptr;
+  T* D21w = iti ? (T*)take((size_t)nm * nF * nF * es) : nullptr;
+  T* Bw = (T*)take((size_t)nm * ne * ni * es);
+  int* scratch = (int*)wp;
+  T* Fl = (T*)F_lu;
+  T* dinv = (T*)F_dinv;
+  T* D12 = D12_out ? (T*)D12_out : D12w;
+  T* D21 = D21_out ? (T*)D21_out : D21w;
+  T* Bm = B_out ? (T*)B_out : Bw;
+  const long sNF2 = (long)nF * nF;
 
-  // D (+ I for ItI) -> D_lu, then factor
-  hipLaunchKernelGGL((merge_gather_kernel<T>), grid_for((long)ni * ni, nm), dim3(256), 0, s, Tin, nc, nec, rows, ni,
-                     cols, ni, 1.0, add_identity, (T*)D_lu, (long)ni, (long)ni * ni, nm);
-  HPS_CHECK_HIP(h, hipGetLastError());
-  HPS_CHECK_HIP(h, getrf_ws<T>(ni, (T*)D_lu, ni, (long)ni * ni, perm, ipiv, info, nm, s));
+  // ---- factor ----------------------------------------------------------
+  if (!iti) {
+    HPS_CHECK_HIP(h, gather_launch<T>(s, "D", Tin, nc, nec, rows, ni, cols, ni, 1.0, 0, Fl, ni, (long)ni * ni, nm));
+  } else {
+    HPS_CHECK_HIP(h, gather_launch<T>(s, "D12", Tin, nc, nec, rows, nF, cols + nF, nF, 1.0, 0, D12, nF, sNF2, nm));
+    HPS_CHECK_HIP(h, gather_launch<T>(s, "D21", Tin, nc, nec, rows + nF, nF, cols, nF, 1.0, 0, D21, nF, sNF2, nm));
+    // W = I - D12 D21
+    HPS_CHECK_HIP(h, gemm_call<T>(s, nF, nF, nF, -1.0, D12, nF, sNF2, D21, nF, sNF2, 0.0, nullptr, 0, 0, Fl, nF,
+                                  sNF2, nm));
+    hipLaunchKernelGGL((add_identity_kernel<T>), dim3(ceil_div(nF, 256), nm), dim3(256), 0, s, Fl, nF, sNF2, nm);
+    HPS_CHECK_HIP(h, hipGetLastError());
+  }
+  HPS_CHECK_HIP(h, getrf_ws<T>(nF, Fl, nF, sNF2, perm, dinv, scratch, info, nm, s));
 
-  // S = -D^{-1} C
-  hipLaunchKernelGGL((merge_gather_kernel<T>), grid_for((long)ni * ne, nm), dim3(256), 0, s, Tin, nc, nec, rows, ni,
-                     ex, ne, -1.0, 0, W1, (long)ne, (long)ni * ne, nm);
-  HPS_CHECK_HIP(h, hipGetLastError());
-  HPS_CHECK_HIP(h, getrs_perm<T>(ni, ne, (const T*)D_lu, ni, (long)ni * ni, perm, W1, ne, (long)ni * ne, (T*)S_out,
-                                 ne, (long)ni * ne, nm, s));
-  // g_tilde = -D^{-1} h_int
-  if (nsrc > 0) {
-    hipLaunchKernelGGL((vec_gather_kernel<T>), grid_for((long)ni * nsrc, nm), dim3(256), 0, s, (const T*)h_in, nc,
-                       nec, nsrc, rows, ni, -1.0, W2, nm);
-    HPS_CHECK_HIP(h, hipGetLastError());
-    HPS_CHECK_HIP(h, getrs_perm<T>(ni, nsrc, (const T*)D_lu, ni, (long)ni * ni, perm, W2, nsrc, (long)ni * nsrc,
-                                   (T*)gt_out, nsrc, (long)ni * nsrc, nm, s));
+  // ---- solves: X = -D^{-1} [C | h_int] -------------------------------------
+  // one pass for the C columns (width ne) and one for h (width nsrc)
+  for (int pass = 0; pass < 2; ++pass) {
+    const int w = pass == 0 ? ne : nsrc;
+    if (w <= 0) continue;
+    T* Yp = pass == 0 ? Y : Yh;
+    T* Zp = pass == 0 ? Z : Zh;
+    T* out = (T*)(pass == 0 ? S_out : gt_out);
+    if (pass == 0) {
+      HPS_CHECK_HIP(h, gather_launch<T>(s, "C", Tin, nc, nec, rows, ni, ex, ne, -1.0, 0, Yp, ne, (long)ni * ne, nm));
+    } else {
+      static const std::string nmh = std::string("merge_gather_h<") + tname<T>() + ">";
+      prof::Scope ps(s, nmh.c_str(), 0.0, 2.0 * es * (double)ni * nsrc * nm);
+      hipLaunchKernelGGL((vec_gather_kernel<T>), grid_for((long)ni * nsrc, nm), dim3(256), 0, s, (const T*)h_in, nc,
+                         nec, nsrc, rows, ni, -1.0, Yp, nm);
+      HPS_CHECK_HIP(h, hipGetLastError());
+    }
+    const long sY = (long)ni * w;
+    if (!iti) {
+      HPS_CHECK_HIP(h, getrs_perm<T>(ni, w, Fl, ni, (long)ni * ni, perm, dinv, Yp, w, sY, out, w, sY, nm, s));
+    } else {
+      const long sZ = (long)nF * w;
+      // Z = Y_F - D12 Y_G
+      HPS_CHECK_HIP(h, gemm_call<T>(s, nF, w, nF, -1.0, D12, nF, sNF2, Yp + (long)nF * w, w, sY, 1.0, Yp, w, sY, Zp, w,
+                                    sZ, nm));
+      // X_F = W^{-1} Z  (into Y_F)
+      HPS_CHECK_HIP(h, getrs_perm<T>(nF, w, Fl, nF, sNF2, perm, dinv, Zp, w, sZ, Yp, w, sY, nm, s));
+      // X_G = Y_G - D21 X_F  (in place)
+      HPS_CHECK_HIP(h, gemm_call<T>(s, nF, w, nF, -1.0, D21, nF, sNF2, Yp, w, sY, 1.0, Yp + (long)nF * w, w, sY,
+                                    Yp + (long)nF * w, w, sY, nm));
+      // internal -> final unknown order
+      static const std::string nms = std::string("row_scatter<") + tname<T>() + ">";
+      prof::Scope ps(s, nms.c_str(), 0.0, 2.0 * es * (double)ni * w * nm);
+      hipLaunchKernelGGL((row_scatter_kernel<T>), grid_for((long)ni * w, nm), dim3(256), 0, s, Yp, out,
+                         final_of_internal, ni, w, nm);
+      HPS_CHECK_HIP(h, hipGetLastError());
+    }
   }
-  // B (exterior rows x interface unknowns)
-  T* Bm = B_out ? (T*)B_out : W1;
-  hipLaunchKernelGGL((merge_gather_kernel<T>), grid_for((long)ne * ni, nm), dim3(256), 0, s, Tin, nc, nec, ex, ne,
-                     cols, ni, 1.0, 0, Bm, (long)ni, (long)ne * ni, nm);
-  HPS_CHECK_HIP(h, hipGetLastError());
-  // T = A + B S
-  if (T_out) {
-    hipLaunchKernelGGL((merge_gather_kernel<T>), grid_for((long)ne * ne, nm), dim3(256), 0, s, Tin, nc, nec, ex, ne,
-                       ex, ne, 1.0, 0, (T*)T_out, (long)ne, (long)ne * ne, nm);
-    HPS_CHECK_HIP(h, hipGetLastError());
-    GemmArgs<T> g;
-    g.M = ne;
-    g.N = ne;
-    g.K = ni;
-    g.alpha = 1.0;
-    g.beta = 1.0;
-    g.A = Bm;
-    g.lda = ni;
-    g.sA = (long)ne * ni;
-    g.B = (const T*)S_out;
-    g.ldb = ne;
-    g.sB = (long)ni * ne;
-    g.C = (const T*)T_out;
-    g.ldc = ne;
-    g.sC = (long)ne * ne;
-    g.D = (T*)T_out;
-    g.ldd = ne;
-    g.sD = (long)ne * ne;
-    g.batch = nm;
-    HPS_CHECK_HIP(h, gemm<T>(g, s));
+
+  // ---- T = A + B S, h_out = h_ext + B g_tilde (block-sparse B) ---------------
+  if (T_out || (nsrc > 0 && h_out) || B_out) {
+    HPS_CHECK_HIP(h, gather_launch<T>(s, "B", Tin, nc, nec, ex, ne, colsF, ni, 1.0, 0, Bm, ni, (long)ne * ni, nm));
   }
-  // h_out = h_ext + B g_tilde
-  if (nsrc > 0 && h_out) {
-    hipLaunchKernelGGL((vec_gather_kernel<T>), grid_for((long)ne * nsrc, nm), dim3(256), 0, s, (const T*)h_in, nc,
-                       nec, nsrc, ex, ne, 1.0, (T*)h_out, nm);
-    HPS_CHECK_HIP(h, hipGetLastError());
-    GemmArgs<T> g;
-    g.M = ne;
-    g.N = nsrc;
-    g.K = ni;
-    g.alpha = 1.0;
-    g.beta = 1.0;
-    g.A = Bm;
-    g.lda = ni;
-    g.sA = (long)ne * ni;
-    g.B = (const T*)gt_out;
-    g.ldb = nsrc;
-    g.sB = (long)ni * nsrc;
-    g.C = (const T*)h_out;
-    g.ldc = nsrc;
-    g.sC = (long)ne * nsrc;
-    g.D = (T*)h_out;
-    g.ldd = nsrc;
-    g.sD = (long)ne * nsrc;
-    g.batch = nm;
-    HPS_CHECK_HIP(h, gemm<T>(g, s));
+  for (int pass = 0; pass < 2; ++pass) {
+    const int w = pass == 0 ? ne : nsrc;
+    T* outp = (T*)(pass == 0 ? T_out : h_out);
+    const T* Sp = (const T*)(pass == 0 ? S_out : gt_out);
+    if (!outp || w <= 0) continue;
+    if (pass == 0) {
+      HPS_CHECK_HIP(h, gather_launch<T>(s, "A", Tin, nc, nec, ex, ne, ex, ne, 1.0, 0, outp, ne, (long)ne * ne, nm));
+    } else {
+      static const std::string nmh = std::string("merge_gather_h<") + tname<T>() + ">";
+      prof::Scope ps(s, nmh.c_str(), 0.0, 2.0 * es * (double)ne * nsrc * nm);
+      hipLaunchKernelGGL((vec_gather_kernel<T>), grid_for((long)ne * nsrc, nm), dim3(256), 0, s, (const T*)h_in, nc,
+                         nec, nsrc, ex, ne, 1.0, outp, nm);
+      HPS_CHECK_HIP(h, hipGetLastError());
+    }
+    const long sO = (long)ne * w;
+    const long sS = (long)ni * w;
+    if (n_upd <= 0) {
+      HPS_CHECK_HIP(h, gemm_call<T>(s, ne, w, ni, 1.0, Bm, ni, (long)ne * ni, Sp, w, sS, 1.0, outp, w, sO, outp, w, sO,
+                                    nm));
+    } else {
+      for (int u = 0; u < n_upd; ++u) {
+        const int r0 = upd[4 * u], nr = upd[4 * u + 1], k0 = upd[4 * u + 2], nk = upd[4 * u + 3];
+        HPS_CHECK_HIP(h, gemm_call<T>(s, nr, w, nk, 1.0, Bm + (long)r0 * ni + k0, ni, (long)ne * ni,
+                                      Sp + (long)k0 * w, w, sS, 1.0, outp + (long)r0 * w, w, sO,
+                                      outp + (long)r0 * w, w, sO, nm));
+      }
+    }
   }
   return HPS_OK;
 }
 
-int hps_merge_level(hps_handle_t h, int dtype, int nm, int nc, int nec, int ni, int ne, int nsrc, int add_identity,
+int hps_merge_level(hps_handle_t h, int dtype, int kind, int nm, int nc, int nec, int ni, int ne, int nsrc,
                     const void* T_in, const void* h_in, const int* int_rows, const int* int_cols, const int* ext,
-                    void* S_out, void* T_out, void* h_out, void* gt_out, void* D_lu, int* perm, int* info,
-                    void* B_out, void* ws, size_t ws_bytes) {
+                    const int* cols_final, const int* final_of_internal, int n_upd, const int* upd, void* S_out,
+                    void* T_out, void* h_out, void* gt_out, void* F_lu, int* perm, void* F_dinv, int* info,
+                    void* D12_out, void* D21_out, void* B_out, void* ws, size_t ws_bytes) {
   if (!h) return HPS_ERR_ARG;
+  HPS_ARG(h, kind == HPS_KIND_DTN || kind == HPS_KIND_ITI, "kind must be DtN or ItI");
   HPS_ARG(h, nm > 0 && nc > 0 && nec > 0 && ni > 0 && ne > 0 && nsrc >= 0, "bad merge sizes");
-  HPS_ARG(h, T_in && int_rows && int_cols && ext && S_out && D_lu && perm && info, "null pointer");
+  HPS_ARG(h, kind == HPS_KIND_DTN || ni % 2 == 0, "ItI merges need an even number of interface unknowns");
+  HPS_ARG(h, T_in && int_rows && int_cols && ext && S_out && F_lu && perm && F_dinv && info, "null pointer");
+  HPS_ARG(h, kind == HPS_KIND_DTN || final_of_internal, "ItI merges need the internal->final row map");
   HPS_ARG(h, nsrc == 0 || (h_in && gt_out), "h_in/gt_out required when nsrc > 0");
-  if (ws_bytes < hps_merge_workspace(dtype, nm, ni, ne, nsrc)) {
+  HPS_ARG(h, n_upd >= 0 && (n_upd == 0 || upd), "bad update block list");
+  for (int u = 0; u < n_upd; ++u)
+    HPS_ARG(h, upd[4 * u] >= 0 && upd[4 * u + 1] > 0 && upd[4 * u] + upd[4 * u + 1] <= ne && upd[4 * u + 2] >= 0 &&
+                   upd[4 * u + 3] > 0 && upd[4 * u + 2] + upd[4 * u + 3] <= ni,
+            "update block out of range");
+  if (ws_bytes < hps_merge_workspace(dtype, kind, nm, ni, ne, nsrc)) {
     h->err = "merge workspace too small";
     return HPS_ERR_WORKSPACE;
   }
-  DISPATCH(dtype, merge_t, h, nm, nc, nec, ni, ne, nsrc, add_identity, T_in, h_in, int_rows, int_cols, ext, S_out,
-           T_out, h_out, gt_out, D_lu, perm, info, B_out, ws);
+  DISPATCH(dtype, merge_t, h, kind, nm, nc, nec, ni, ne, nsrc, T_in, h_in, int_rows, int_cols, ext, cols_final,
+           final_of_internal, n_upd, upd, S_out, T_out, h_out, gt_out, F_lu, perm, F_dinv, info, D12_out, D21_out,
+           B_out, ws);
 }
 
 // ---------------------------------------------------------------------------
@@ -613,8 +742,12 @@ static int down_t(hps_handle_t h, int nm, int nc, int nec, int ni, int ne, int n
   g.sD = (long)ni * nsrc;
   g.batch = nm;
   HPS_CHECK_HIP(h, gemm<T>(g, s));
-  hipLaunchKernelGGL((down_scatter_kernel<T>), grid_for((long)nc * nec * nsrc, nm), dim3(256), 0, s, nc, nec, ne, ni,
+  {
+    HPS_PROF("down_scatter", T, 0.0, 2.0 * sizeof(T) * (double)((double)nc * nec * nsrc) * nm);
+    hipLaunchKernelGGL((down_scatter_kernel<T>), grid_for((long)nc * nec * nsrc, nm), dim3(256), 0, s, nc, nec, ne, ni,
                      nsrc, child_src, (const T*)g_in, W, (T*)g_out, nm);
+  }
+  
   HPS_CHECK_HIP(h, hipGetLastError());
   return HPS_OK;
 }

@@ -123,6 +123,13 @@ inline size_t dtype_size(int dt) {
   return 0;
 }
 
+template <typename T> inline const char* tname();
+template <> inline const char* tname<double>() { return "f64"; }
+template <> inline const char* tname<float>() { return "f32"; }
+template <> inline const char* tname<c128>() { return "c128"; }
+template <> inline const char* tname<c64>() { return "c64"; }
+template <typename T> constexpr double flop_mult() { return is_cplx<T>::value ? 4.0 : 1.0; }
+
 inline unsigned ceil_div(long a, long b) { return (unsigned)((a + b - 1) / b); }
 
 }  // namespace hps

@@ -28,17 +28,21 @@ hipError_t gemm(const GemmArgs<T>& a, hipStream_t s);
 
 // In-place batched LU with partial pivoting of the n x n block of each A[b]
 // (row stride lda, batch stride sA): P A = L U.  perm[b*n + i] is the row of
-// A that became row i of P A.  ipiv is scratch (batch*n ints, LAPACK-style
-// 0-based swap sequence).  info[b] = 0, or j+1 for the first column j whose
-// pivot candidates are all exactly zero.
+// A that became row i of P A.  dinv[b] (dinv_elems(n) values) receives the
+// packed inverses of the 32 x 32 diagonal blocks.  scratch holds
+// getrf_scratch_bytes(n, batch) bytes.  info[b] = 0, or j+1 for the first
+// column j whose pivot candidates are all exactly zero.
+size_t getrf_scratch_bytes(int n, int batch);
+size_t dinv_elems(int n);
 template <typename T>
-hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, int* ipiv, int* info, int batch, hipStream_t s);
+hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, T* dinv, int* scratch, int* info, int batch,
+                    hipStream_t s);
 
 // X = A^{-1} R from getrf_ws's factors.  R (n x nrhs, row stride ldr, batch
 // stride sR) is read only; X must not alias R.
 template <typename T>
-hipError_t getrs_perm(int n, int nrhs, const T* LU, long lda, long sA, const int* perm, const T* R, long ldr,
-                      long sR, T* X, long ldx, long sX, int batch, hipStream_t s);
+hipError_t getrs_perm(int n, int nrhs, const T* LU, long lda, long sA, const int* perm, const T* dinv, const T* R,
+                      long ldr, long sR, T* X, long ldx, long sX, int batch, hipStream_t s);
 
 // dst[b][r][c] = alpha * src[b][r][c]; src == nullptr writes zeros; a batch
 // stride of 0 on src broadcasts.

@@ -1,25 +1,37 @@
 // Batched dense LU with partial pivoting and the matching solve, gfx950.
 //
+// Factor format: P A = L U stored LAPACK-style in A, plus
+//   perm[b][i]        row of A that became row i of P A,
+//   dinv[b][blk]      for every NB x NB diagonal block, the packed inverses
+//                     (strict lower part: inv(L11); upper part: inv(U11)),
+// so every triangular solve is a parallel block product instead of a
+// sequential substitution.
+//
 // getrf: blocked right-looking, panel width NB = 32.
-//   1. panel kernel (one workgroup per matrix): factors the (n-k) x nb panel
-//      column by column.  The pivot search is a wave-level argmax (shuffle
-//      reduction inside each 64-wide wave, then across waves through LDS).
-//      The panel is staged in LDS when it fits (leaf-sized matrices), and is
-//      worked on in place in global memory (L2-resident) otherwise.
-//   2. swap+TRSM kernel: applies the panel's row swaps to every column outside
-//      the panel and forms U12 = L11^{-1} A12 (L11 staged in LDS).
-//   3. trailing update A22 -= L21 U12 through the batched MFMA GEMM.
-//   The kernel also maintains perm[] (row i of P*A is row perm[i] of A), so
-//   the solve applies the whole permutation as one gather.
-// getrs: X = A^{-1} R out of place: gather rows by perm, blocked unit-lower
-//   forward substitution and upper back substitution (LDS-staged diagonal
-//   blocks, MFMA GEMM updates between blocks).
+//   1. panel (one workgroup per matrix), column by column with a wave-level
+//      argmax (shuffle reduction inside each 64-wide wave, then across waves
+//      through LDS):
+//        - short panels are staged whole in LDS;
+//        - tall panels (merge matrices) keep 4-column sub-panels in
+//          registers (rows distributed over 1024 threads) and update each
+//          sub-panel left-looking from the columns already factored.
+//      The kernel ends by inverting the diagonal block into dinv and writing
+//      the panel's net row permutation as a (row, source) table.
+//   2. swap + U12: applies that table to every column outside the panel in
+//      parallel and forms U12 = inv(L11) A12 as a block product.
+//   3. trailing update A22 -= L21 U12 with the batched MFMA GEMM.
+// getrs: gather rows by perm; per block X_k = inv(L_kk) X_k, then the MFMA
+//   GEMM update below; backward X_k = inv(U_kk) X_k, GEMM update above.
+#include <string>
+
 #include "hps_kernels.h"
+#include "hps_prof.h"
 
 namespace hps {
 
 constexpr int LU_NB = 32;
 constexpr size_t LU_PANEL_LDS_MAX = 150 * 1024;
+constexpr int SWTAB = 1 + 4 * LU_NB;  // count, then up to 2*NB (row, src) int pairs
 
 __global__ void fill_int_kernel(int* p, long n, int v) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -39,175 +51,497 @@ __device__ __forceinline__ void argmax_merge(R& bv, int& bi, R v, int i) {
   }
 }
 
-// One workgroup per batch element.  P points at A[k][k]; panel has m rows and
-// nb columns with row stride ldp (LDS: NB, global: lda).
-template <typename T, bool USE_LDS, int THREADS>
-__global__ __launch_bounds__(THREADS) void getrf_panel_kernel(int n, T* A, long lda, long sA, int k, int nb,
-                                                              int* perm, int* ipiv, int* info) {
+// Block-wide argmax of (value, index) pairs; every thread gets the winner.
+template <typename R, int THREADS>
+__device__ __forceinline__ int block_argmax(R bv, int bi, R* red_v, int* red_i, int* s_out, R* s_val) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    R ov = __shfl_xor(bv, off, 64);
+    int oi = __shfl_xor(bi, off, 64);
+    argmax_merge(bv, bi, ov, oi);
+  }
+  if (lane == 0) {
+    red_v[wave] = bv;
+    red_i[wave] = bi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    R fv = red_v[0];
+    int fi = red_i[0];
+    for (int w = 1; w < THREADS / 64; ++w) argmax_merge(fv, fi, red_v[w], red_i[w]);
+    *s_out = fi;
+    *s_val = fv;
+  }
+  __syncthreads();
+  return *s_out;
+}
+
+// Record pivot j of the panel: swap sequence and breakdown flag.  The row
+// permutation is assembled once from the swap sequence after the last panel.
+__device__ __forceinline__ void record_pivot(int k, int j, int fi, bool zero, int* ipivb, int* info, int b) {
+  ipivb[k + j] = k + fi;
+  if (zero && info[b] == 0) info[b] = k + j + 1;
+}
+
+// perm = the composition of all row swaps: one workgroup per matrix, the
+// permutation built in LDS.
+__global__ __launch_bounds__(256) void perm_from_ipiv_kernel(int n, const int* ipiv, int* perm) {
+  extern __shared__ int sp[];
+  const int b = blockIdx.x;
+  const int* ib = ipiv + (long)b * n;
+  for (int i = threadIdx.x; i < n; i += 256) sp[i] = i;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < n; ++i) {
+      int p = ib[i];
+      if (p != i) {
+        int t = sp[i];
+        sp[i] = sp[p];
+        sp[p] = t;
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += 256) perm[(long)b * n + i] = sp[i];
+}
+
+// After the panel: (row, source) table of its net row permutation, from the
+// swap sequence ipiv[k .. k+nb).  Single thread.
+__device__ void build_swap_table(int k, int nb, const int* ipivb, int* tab) {
+  int rows[2 * LU_NB], src[2 * LU_NB];
+  int cnt = 0;
+  for (int j = 0; j < nb; ++j) {
+    int a = k + j, bb = ipivb[k + j];
+    if (a == bb) continue;
+    int ia = -1, ib = -1;
+    for (int t = 0; t < cnt; ++t) {
+      if (rows[t] == a) ia = t;
+      if (rows[t] == bb) ib = t;
+    }
+    if (ia < 0) {
+      rows[cnt] = a;
+      src[cnt] = a;
+      ia = cnt++;
+    }
+    if (ib < 0) {
+      rows[cnt] = bb;
+      src[cnt] = bb;
+      ib = cnt++;
+    }
+    int t = src[ia];
+    src[ia] = src[ib];
+    src[ib] = t;
+  }
+  tab[0] = cnt;
+  for (int t = 0; t < cnt; ++t) {
+    tab[1 + 2 * t] = rows[t];
+    tab[2 + 2 * t] = src[t];
+  }
+}
+
+// Invert the factored nb x nb diagonal block D (row stride ldd) into the
+// packed form out (row stride ldo): strict lower = inv(L11), upper = inv(U11).
+template <typename T>
+__device__ void invert_diag_block(const T* D, int ldd, T* out, int ldo, int nb) {
+  const int t = threadIdx.x;
+  if (t < nb) {
+    // column t of inv(L11): unit lower forward substitution
+    const int c = t;
+    T y[LU_NB];
+#pragma unroll
+    for (int i = 0; i < LU_NB; ++i) y[i] = s_zero<T>();
+    for (int i = c; i < nb; ++i) {
+      T acc = i == c ? s_one<T>() : s_zero<T>();
+      for (int j = c; j < i; ++j) acc = s_fnms(acc, D[i * ldd + j], y[j]);
+      y[i] = acc;
+    }
+    for (int i = c + 1; i < nb; ++i) out[i * ldo + c] = y[i];
+  } else if (t >= 64 && t < 64 + nb) {
+    // column c of inv(U11): back substitution
+    const int c = t - 64;
+    T x[LU_NB];
+#pragma unroll
+    for (int i = 0; i < LU_NB; ++i) x[i] = s_zero<T>();
+    for (int i = c; i >= 0; --i) {
+      T acc = i == c ? s_one<T>() : s_zero<T>();
+      for (int j = i + 1; j <= c; ++j) acc = s_fnms(acc, D[i * ldd + j], x[j]);
+      x[i] = s_mul(acc, s_inv(D[i * ldd + i]));
+    }
+    for (int i = 0; i <= c; ++i) out[i * ldo + c] = x[i];
+  }
+}
+
+// Diagonal-block inverse -> dinv, and the panel's swap table.  D is the final
+// factored diagonal block in LDS; Out is LDS scratch of nb rows.
+template <typename T>
+__device__ void finish_panel(const T* D, int ldd, T* Out, int ldo, int k, int nb, int n, int b, const int* ipivb,
+                             T* dinv, int* swtab) {
+  __syncthreads();
+  invert_diag_block<T>(D, ldd, Out, ldo, nb);
+  if (threadIdx.x == 128) build_swap_table(k, nb, ipivb, swtab + (long)b * SWTAB);
+  __syncthreads();
+  const int nblk = (n + LU_NB - 1) / LU_NB;
+  T* db = dinv + ((long)b * nblk + k / LU_NB) * LU_NB * LU_NB;
+  for (int idx = threadIdx.x; idx < LU_NB * LU_NB; idx += blockDim.x) {
+    int r = idx / LU_NB, c = idx - r * LU_NB;
+    db[idx] = (r < nb && c < nb) ? Out[r * ldo + c] : s_zero<T>();
+  }
+}
+
+// Short panels: the whole (n-k) x nb panel in LDS.
+template <typename T, int THREADS>
+__global__ __launch_bounds__(THREADS) void getrf_panel_lds_kernel(int n, T* A, long lda, long sA, int k, int nb,
+                                                                  int* ipiv, int* info, T* dinv, int* swtab) {
   using R = typename real_of<T>::type;
   extern __shared__ char smem_raw[];
   __shared__ R red_v[THREADS / 64];
   __shared__ int red_i[THREADS / 64];
   __shared__ int s_piv;
+  __shared__ R s_val;
   const int b = blockIdx.x;
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
   const int m = n - k;
   T* Ag = A + (long)b * sA + (long)k * lda + k;
-  T* P;
-  long ldp;
-  if constexpr (USE_LDS) {
-    P = reinterpret_cast<T*>(smem_raw);
-    ldp = LU_NB;
-    for (int idx = tid; idx < m * nb; idx += THREADS) {
-      int r = idx / nb, c = idx - r * nb;
-      P[r * LU_NB + c] = Ag[(long)r * lda + c];
-    }
-    __syncthreads();
-  } else {
-    P = Ag;
-    ldp = lda;
+  T* P = reinterpret_cast<T*>(smem_raw);
+  for (int idx = tid; idx < m * nb; idx += THREADS) {
+    int r = idx / nb, c = idx - r * nb;
+    P[r * LU_NB + c] = Ag[(long)r * lda + c];
   }
-  int* permb = perm + (long)b * n;
+  __syncthreads();
   int* ipivb = ipiv + (long)b * n;
-
   for (int j = 0; j < nb; ++j) {
     R bv = R(-1);
     int bi = 0x7fffffff;
-    for (int r = j + tid; r < m; r += THREADS) argmax_merge(bv, bi, s_abs1(P[(long)r * ldp + j]), r);
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      R ov = __shfl_xor(bv, off, 64);
-      int oi = __shfl_xor(bi, off, 64);
-      argmax_merge(bv, bi, ov, oi);
-    }
-    if (lane == 0) {
-      red_v[wave] = bv;
-      red_i[wave] = bi;
-    }
-    __syncthreads();
-    if (tid == 0) {
-      R fv = red_v[0];
-      int fi = red_i[0];
-      for (int w = 1; w < THREADS / 64; ++w) argmax_merge(fv, fi, red_v[w], red_i[w]);
-      // NaN entries never win a comparison; keep the diagonal then so the
-      // pivot row always lies inside the panel.
-      if (fi < j || fi >= m) fi = j;
-      s_piv = fi;
-      ipivb[k + j] = k + fi;
-      if (fi != j) {
-        int t = permb[k + j];
-        permb[k + j] = permb[k + fi];
-        permb[k + fi] = t;
-      }
-      if (fv == R(0) && info[b] == 0) info[b] = k + j + 1;
-    }
-    __syncthreads();
-    const int pr = s_piv;
+    for (int r = j + tid; r < m; r += THREADS) argmax_merge(bv, bi, s_abs1(P[r * LU_NB + j]), r);
+    int pr = block_argmax<R, THREADS>(bv, bi, red_v, red_i, &s_piv, &s_val);
+    // NaN entries never win a comparison; keep the diagonal then so the
+    // pivot row always lies inside the panel.
+    if (pr < j || pr >= m) pr = j;
+    if (tid == 0) record_pivot(k, j, pr, s_val == R(0), ipivb, info, b);
     if (pr != j) {
       for (int c = tid; c < nb; c += THREADS) {
-        T t = P[(long)j * ldp + c];
-        P[(long)j * ldp + c] = P[(long)pr * ldp + c];
-        P[(long)pr * ldp + c] = t;
+        T t = P[j * LU_NB + c];
+        P[j * LU_NB + c] = P[pr * LU_NB + c];
+        P[pr * LU_NB + c] = t;
       }
     }
     __syncthreads();
-    T piv = P[(long)j * ldp + j];
+    T piv = P[j * LU_NB + j];
     if (!s_is_zero(piv)) {
       T inv = s_inv(piv);
       for (int r = j + 1 + tid; r < m; r += THREADS) {
-        T l = s_mul(P[(long)r * ldp + j], inv);
-        P[(long)r * ldp + j] = l;
-        for (int c = j + 1; c < nb; ++c) P[(long)r * ldp + c] = s_fnms(P[(long)r * ldp + c], l, P[(long)j * ldp + c]);
+        T l = s_mul(P[r * LU_NB + j], inv);
+        P[r * LU_NB + j] = l;
+        for (int c = j + 1; c < nb; ++c) P[r * LU_NB + c] = s_fnms(P[r * LU_NB + c], l, P[j * LU_NB + c]);
       }
     }
     __syncthreads();
   }
-  if constexpr (USE_LDS) {
-    for (int idx = tid; idx < m * nb; idx += THREADS) {
-      int r = idx / nb, c = idx - r * nb;
-      Ag[(long)r * lda + c] = P[r * LU_NB + c];
-    }
+  for (int idx = tid; idx < m * nb; idx += THREADS) {
+    int r = idx / nb, c = idx - r * nb;
+    Ag[(long)r * lda + c] = P[r * LU_NB + c];
   }
+  // the dynamic allocation holds m + LU_NB rows: the extra rows take the inverse
+  finish_panel<T>(P, LU_NB, P + (long)m * LU_NB, LU_NB, k, nb, n, b, ipivb, dinv, swtab);
 }
 
-// Row swaps of panel k applied to all columns outside the panel, and
-// U12 = L11^{-1} A12 for the columns right of it.  One thread per column.
-template <typename T>
-__global__ __launch_bounds__(256) void getrf_swap_trsm_kernel(int n, T* A, long lda, long sA, int k, int nb,
-                                                              const int* ipiv) {
-  __shared__ T L11[LU_NB][LU_NB + 1];
-  __shared__ int spiv[LU_NB];
-  const int b = blockIdx.y;
-  T* Ab = A + (long)b * sA;
-  for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
-    int r = idx / nb, c = idx - r * nb;
-    L11[r][c] = Ab[(long)(k + r) * lda + k + c];
-  }
-  for (int j = threadIdx.x; j < nb; j += blockDim.x) spiv[j] = ipiv[(long)b * n + k + j];
+// Tall panels: rows distributed cyclically over THREADS (thread t owns panel
+// rows t + THREADS*i, i < RPT); 4-column sub-panels live in registers and are
+// brought up to date left-looking before they are factored.  Rows are
+// addressed through an LDS row map (logical row -> memory row), so a pivot
+// swap costs a register exchange and two LDS writes; the panel is put into
+// physical order once at the end (only the <= 2*nb rows the swaps touched
+// move).
+template <typename T, int RPT, int THREADS>
+__global__ __launch_bounds__(THREADS) void getrf_panel_reg_kernel(int n, T* A, long lda, long sA, int k, int nb,
+                                                                  int* ipiv, int* info, T* dinv, int* swtab) {
+  using R = typename real_of<T>::type;
+  constexpr int SW = 4;
+  __shared__ R red_v[THREADS / 64];
+  __shared__ int red_i[THREADS / 64];
+  __shared__ int s_piv;
+  __shared__ R s_val;
+  __shared__ int rowmap[RPT * THREADS];
+  __shared__ T Lt[LU_NB][LU_NB + 1];  // top nb logical rows of the panel
+  __shared__ T Ut[LU_NB][SW];         // U rows above the current sub-panel
+  __shared__ T xa[SW], xb[SW];        // row exchange buffers
+  __shared__ T prow[SW];
+  __shared__ T Dout[LU_NB][LU_NB + 1];
+  __shared__ T fix[2 * LU_NB][LU_NB];
+  __shared__ int fix_rows[2 * LU_NB];
+  __shared__ int fix_cnt;
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int m = n - k;
+  T* P = A + (long)b * sA + (long)k * lda + k;
+  int* ipivb = ipiv + (long)b * n;
+  for (int r = tid; r < m; r += THREADS) rowmap[r] = r;
   __syncthreads();
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  int c = t < k ? t : t + nb;
-  if (c >= n) return;
-  for (int j = 0; j < nb; ++j) {
-    int pr = spiv[j];
-    if (pr != k + j) {
-      T x = Ab[(long)(k + j) * lda + c];
-      Ab[(long)(k + j) * lda + c] = Ab[(long)pr * lda + c];
-      Ab[(long)pr * lda + c] = x;
-    }
-  }
-  if (c >= k + nb) {
-    T x[LU_NB];
-#pragma unroll
-    for (int j = 0; j < LU_NB; ++j)
-      if (j < nb) x[j] = Ab[(long)(k + j) * lda + c];
-#pragma unroll
-    for (int j = 0; j < LU_NB; ++j) {
-      if (j < nb) {
-        for (int i = 0; i < j; ++i) x[j] = s_fnms(x[j], L11[j][i], x[i]);
-        Ab[(long)(k + j) * lda + c] = x[j];
+
+  T x[RPT][SW];
+  for (int c0 = 0; c0 < nb; c0 += SW) {
+    const int sw = nb - c0 < SW ? nb - c0 : SW;
+    // U rows above: forward substitution with the unit-lower top block
+    if (tid < sw) {
+      const int c = c0 + tid;
+      for (int j = 0; j < c0; ++j) {
+        T* Pj = P + (long)rowmap[j] * lda;
+        T acc = Pj[c];
+        for (int i = 0; i < j; ++i) acc = s_fnms(acc, Lt[j][i], Ut[i][tid]);
+        Ut[j][tid] = acc;
+        Pj[c] = acc;
       }
     }
+    __syncthreads();
+    // load my rows of the sub-panel and apply the left-looking update
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+      const int r = tid + THREADS * i;
+#pragma unroll
+      for (int q = 0; q < SW; ++q) x[i][q] = s_zero<T>();
+      if (r >= c0 && r < m) {
+        const T* Pr = P + (long)rowmap[r] * lda;
+#pragma unroll
+        for (int q = 0; q < SW; ++q)
+          if (q < sw) x[i][q] = Pr[c0 + q];
+        int j = 0;
+        for (; j + 4 <= c0; j += 4) {
+          T l0 = Pr[j], l1 = Pr[j + 1], l2 = Pr[j + 2], l3 = Pr[j + 3];
+#pragma unroll
+          for (int q = 0; q < SW; ++q) {
+            x[i][q] = s_fnms(x[i][q], l0, Ut[j][q]);
+            x[i][q] = s_fnms(x[i][q], l1, Ut[j + 1][q]);
+            x[i][q] = s_fnms(x[i][q], l2, Ut[j + 2][q]);
+            x[i][q] = s_fnms(x[i][q], l3, Ut[j + 3][q]);
+          }
+        }
+        for (; j < c0; ++j) {
+          T l = Pr[j];
+#pragma unroll
+          for (int q = 0; q < SW; ++q) x[i][q] = s_fnms(x[i][q], l, Ut[j][q]);
+        }
+      }
+    }
+    // factor the sub-panel columns
+    for (int q = 0; q < sw; ++q) {
+      const int j = c0 + q;
+      R bv = R(-1);
+      int bi = 0x7fffffff;
+#pragma unroll
+      for (int i = 0; i < RPT; ++i) {
+        const int r = tid + THREADS * i;
+        if (r >= j && r < m) argmax_merge(bv, bi, s_abs1(x[i][q]), r);
+      }
+      int pr = block_argmax<R, THREADS>(bv, bi, red_v, red_i, &s_piv, &s_val);
+      if (pr < j || pr >= m) pr = j;
+      if (tid == 0) record_pivot(k, j, pr, s_val == R(0), ipivb, info, b);
+      if (pr != j) {
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) {
+          const int r = tid + THREADS * i;
+          if (r == j)
+#pragma unroll
+            for (int t = 0; t < SW; ++t) xa[t] = x[i][t];
+          if (r == pr)
+#pragma unroll
+            for (int t = 0; t < SW; ++t) xb[t] = x[i][t];
+        }
+        if (tid == 0) {
+          int t = rowmap[j];
+          rowmap[j] = rowmap[pr];
+          rowmap[pr] = t;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) {
+          const int r = tid + THREADS * i;
+          if (r == j)
+#pragma unroll
+            for (int t = 0; t < SW; ++t) x[i][t] = xb[t];
+          if (r == pr)
+#pragma unroll
+            for (int t = 0; t < SW; ++t) x[i][t] = xa[t];
+        }
+      }
+      // pivot row broadcast
+#pragma unroll
+      for (int i = 0; i < RPT; ++i) {
+        const int r = tid + THREADS * i;
+        if (r == j)
+#pragma unroll
+          for (int t = 0; t < SW; ++t) prow[t] = x[i][t];
+      }
+      __syncthreads();
+      T piv = prow[q];
+      if (!s_is_zero(piv)) {
+        T inv = s_inv(piv);
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) {
+          const int r = tid + THREADS * i;
+          if (r > j && r < m) {
+            T l = s_mul(x[i][q], inv);
+            x[i][q] = l;
+#pragma unroll
+            for (int t = 0; t < SW; ++t)
+              if (t > q && t < sw) x[i][t] = s_fnms(x[i][t], l, prow[t]);
+          }
+        }
+      }
+      __syncthreads();
+    }
+    // write the sub-panel back; refresh the top block
+#pragma unroll
+    for (int i = 0; i < RPT; ++i) {
+      const int r = tid + THREADS * i;
+      if (r >= c0 && r < m) {
+        T* Pr = P + (long)rowmap[r] * lda;
+#pragma unroll
+        for (int q = 0; q < SW; ++q)
+          if (q < sw) Pr[c0 + q] = x[i][q];
+      }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < nb * (c0 + sw); idx += THREADS) {
+      int r = idx / (c0 + sw), c = idx - r * (c0 + sw);
+      Lt[r][c] = P[(long)rowmap[r] * lda + c];
+    }
+    __syncthreads();
+  }
+  // physical order: move the rows whose memory row differs from their position
+  if (tid == 0) fix_cnt = 0;
+  __syncthreads();
+  for (int r = tid; r < m; r += THREADS) {
+    if (rowmap[r] != r) {
+      int slot = atomicAdd(&fix_cnt, 1);
+      if (slot < 2 * LU_NB) fix_rows[slot] = r;
+    }
+  }
+  __syncthreads();
+  const int cnt = fix_cnt < 2 * LU_NB ? fix_cnt : 2 * LU_NB;
+  for (int idx = tid; idx < cnt * nb; idx += THREADS) {
+    int a = idx / nb, c = idx - a * nb;
+    fix[a][c] = P[(long)rowmap[fix_rows[a]] * lda + c];
+  }
+  __syncthreads();
+  for (int idx = tid; idx < cnt * nb; idx += THREADS) {
+    int a = idx / nb, c = idx - a * nb;
+    P[(long)fix_rows[a] * lda + c] = fix[a][c];
+  }
+  // Lt holds the whole factored diagonal block
+  finish_panel<T>(&Lt[0][0], LU_NB + 1, &Dout[0][0], LU_NB + 1, k, nb, n, b, ipivb, dinv, swtab);
+}
+
+// Apply the panel's row permutation table to columns outside the panel and,
+// for the trailing columns, U12 = inv(L11) A12.  256 threads = 32 rows x 8
+// columns per workgroup.
+template <typename T>
+__global__ __launch_bounds__(256) void getrf_swap_u12_kernel(int n, T* A, long lda, long sA, int k, int nb,
+                                                             const T* dinv, const int* swtab) {
+  __shared__ T Li[LU_NB][LU_NB + 1];
+  __shared__ T vals[2 * LU_NB][8];
+  __shared__ T X[LU_NB][8];
+  __shared__ int tab[SWTAB];
+  const int b = blockIdx.y;
+  T* Ab = A + (long)b * sA;
+  const int others = n - nb;
+  const int cbase = blockIdx.x * 8;
+  const int tid = threadIdx.x;
+  const int cc = tid & 7, rr = tid >> 3;
+  const int t = cbase + cc;
+  const int c = t < k ? t : t + nb;  // column index in A
+  const bool valid = t < others;
+  for (int i = tid; i < SWTAB; i += 256) tab[i] = swtab[(long)b * SWTAB + i];
+  __syncthreads();
+  const int cnt = tab[0];
+  // gather the permuted rows, then scatter
+  for (int a = rr; a < cnt; a += 32)
+    if (valid) vals[a][cc] = Ab[(long)tab[2 + 2 * a] * lda + c];
+  __syncthreads();
+  for (int a = rr; a < cnt; a += 32)
+    if (valid) Ab[(long)tab[1 + 2 * a] * lda + c] = vals[a][cc];
+  __syncthreads();
+  if (cbase >= k) {  // trailing slab (slabs never straddle k: k is a multiple of 8)
+    const int nblk = (n + LU_NB - 1) / LU_NB;
+    const T* db = dinv + ((long)b * nblk + k / LU_NB) * LU_NB * LU_NB;
+    for (int idx = tid; idx < LU_NB * LU_NB; idx += 256) {
+      int r = idx / LU_NB, q = idx - r * LU_NB;
+      Li[r][q] = q < r ? db[idx] : (q == r ? s_one<T>() : s_zero<T>());
+    }
+    X[rr][cc] = (valid && rr < nb) ? Ab[(long)(k + rr) * lda + c] : s_zero<T>();
+    __syncthreads();
+    if (valid && rr < nb) {
+      T acc = s_zero<T>();
+      for (int q = 0; q <= rr; ++q) acc = s_add(acc, s_mul(Li[rr][q], X[q][cc]));
+      Ab[(long)(k + rr) * lda + c] = acc;
+    }
   }
 }
 
+template <typename T, int RPT>
+static void launch_reg_panel(int n, T* A, long lda, long sA, int k, int nb, int* ipiv, int* info, T* dinv,
+                             int* swtab, int batch, hipStream_t s) {
+  hipLaunchKernelGGL((getrf_panel_reg_kernel<T, RPT, 1024>), dim3(batch), dim3(1024), 0, s, n, A, lda, sA, k, nb,
+                     ipiv, info, dinv, swtab);
+}
+
 template <typename T>
-static hipError_t launch_panel(int n, T* A, long lda, long sA, int k, int nb, int* perm, int* ipiv, int* info,
-                               int batch, hipStream_t s) {
-  int m = n - k;
-  size_t lds = (size_t)m * LU_NB * sizeof(T);
+static hipError_t launch_panel(int n, T* A, long lda, long sA, int k, int nb, int* ipiv, int* info, T* dinv,
+                               int* swtab, int batch, hipStream_t s) {
+  const int m = n - k;
+  const size_t lds = (size_t)(m + LU_NB) * LU_NB * sizeof(T);
+  static const std::string nm = std::string("getrf_panel<") + tname<T>() + ">";
+  const double pf = flop_mult<T>() * batch * ((double)m * nb * nb - (double)nb * nb * nb / 3.0);
+  prof::Scope ps(s, nm.c_str(), pf, 2.0 * sizeof(T) * (double)m * nb * batch);
   if (lds <= LU_PANEL_LDS_MAX) {
     static bool attr_set = false;
     if (!attr_set) {
-      hipFuncSetAttribute((const void*)getrf_panel_kernel<T, true, 256>,
-                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)LU_PANEL_LDS_MAX);
+      hipFuncSetAttribute((const void*)getrf_panel_lds_kernel<T, 256>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)LU_PANEL_LDS_MAX);
       attr_set = true;
     }
-    hipLaunchKernelGGL((getrf_panel_kernel<T, true, 256>), dim3(batch), dim3(256), lds, s, n, A, lda, sA, k, nb,
-                       perm, ipiv, info);
+    hipLaunchKernelGGL((getrf_panel_lds_kernel<T, 256>), dim3(batch), dim3(256), lds, s, n, A, lda, sA, k, nb, ipiv,
+                       info, dinv, swtab);
   } else {
-    hipLaunchKernelGGL((getrf_panel_kernel<T, false, 1024>), dim3(batch), dim3(1024), 0, s, n, A, lda, sA, k, nb,
-                       perm, ipiv, info);
+    const int rpt = (m + 1023) / 1024;
+    if (rpt <= 1)
+      launch_reg_panel<T, 1>(n, A, lda, sA, k, nb, ipiv, info, dinv, swtab, batch, s);
+    else if (rpt <= 2)
+      launch_reg_panel<T, 2>(n, A, lda, sA, k, nb, ipiv, info, dinv, swtab, batch, s);
+    else if (rpt <= 4)
+      launch_reg_panel<T, 4>(n, A, lda, sA, k, nb, ipiv, info, dinv, swtab, batch, s);
+    else if (rpt <= 8)
+      launch_reg_panel<T, 8>(n, A, lda, sA, k, nb, ipiv, info, dinv, swtab, batch, s);
+    else if (rpt <= 12)
+      launch_reg_panel<T, 12>(n, A, lda, sA, k, nb, ipiv, info, dinv, swtab, batch, s);
+    else
+      return hipErrorInvalidValue;  // n > 12288: beyond this library's matrix sizes
   }
   return hipGetLastError();
 }
 
-// perm/info outputs; ipiv is scratch of batch*n ints.
+size_t getrf_scratch_bytes(int n, int batch) {
+  return ((size_t)batch * n + (size_t)batch * SWTAB) * sizeof(int);
+}
+
 template <typename T>
-hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, int* ipiv, int* info, int batch, hipStream_t s) {
+hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, T* dinv, int* scratch, int* info, int batch,
+                    hipStream_t s) {
   if (n <= 0 || batch <= 0) return hipSuccess;
-  long total = (long)batch * n;
-  hipLaunchKernelGGL(iota_kernel, dim3(ceil_div(total, 256)), dim3(256), 0, s, perm, n, total);
+  int* ipiv = scratch;
+  int* swtab = scratch + (long)batch * n;
   hipLaunchKernelGGL(fill_int_kernel, dim3(ceil_div(batch, 256)), dim3(256), 0, s, info, (long)batch, 0);
   for (int k = 0; k < n; k += LU_NB) {
     int nb = n - k < LU_NB ? n - k : LU_NB;
-    hipError_t e = launch_panel<T>(n, A, lda, sA, k, nb, perm, ipiv, info, batch, s);
+    hipError_t e = launch_panel<T>(n, A, lda, sA, k, nb, ipiv, info, dinv, swtab, batch, s);
     if (e != hipSuccess) return e;
     int others = n - nb;
     if (others > 0) {
-      dim3 grid(ceil_div(others, 256), batch);
-      hipLaunchKernelGGL((getrf_swap_trsm_kernel<T>), grid, dim3(256), 0, s, n, A, lda, sA, k, nb, ipiv);
+      static const std::string nm = std::string("getrf_swap_u12<") + tname<T>() + ">";
+      prof::Scope ps(s, nm.c_str(), flop_mult<T>() * batch * (double)(n - k - nb) * nb * nb,
+                     3.0 * sizeof(T) * (double)nb * others * batch);
+      dim3 grid(ceil_div(others, 8), batch);
+      hipLaunchKernelGGL((getrf_swap_u12_kernel<T>), grid, dim3(256), 0, s, n, A, lda, sA, k, nb, dinv, swtab);
     }
     int rem = n - k - nb;
     if (rem > 0) {
@@ -234,6 +568,7 @@ hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, int* ipiv, int* i
       if (e != hipSuccess) return e;
     }
   }
+  hipLaunchKernelGGL(perm_from_ipiv_kernel, dim3(batch), dim3(256), (size_t)n * sizeof(int), s, n, ipiv, perm);
   return hipGetLastError();
 }
 
@@ -249,58 +584,60 @@ __global__ void gather_rows_kernel(int n, int nrhs, const int* perm, const T* R,
   X[(long)b * sX + (long)i * ldx + c] = R[(long)b * sR + (long)src * ldr + c];
 }
 
+// X[k:k+nb, :] = tri(Dinv_k) X[k:k+nb, :] in place; tri = unit lower (forward)
+// or upper (backward).  256 threads = 32 rows x 8 columns per workgroup.
 template <typename T, bool UPPER>
-__global__ __launch_bounds__(256) void trsm_block_kernel(int nrhs, const T* LU, long lda, long sA, T* X, long ldx,
-                                                         long sX, int k, int nb) {
-  __shared__ T Ls[LU_NB][LU_NB + 1];
+__global__ __launch_bounds__(256) void trsm_inv_kernel(int n, int nrhs, const T* dinv, T* X, long ldx, long sX, int k,
+                                                       int nb) {
+  __shared__ T Di[LU_NB][LU_NB + 1];
+  __shared__ T Xs[LU_NB][8];
   const int b = blockIdx.y;
-  const T* Lb = LU + (long)b * sA;
-  for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
-    int r = idx / nb, c = idx - r * nb;
-    Ls[r][c] = Lb[(long)(k + r) * lda + k + c];
+  const int tid = threadIdx.x;
+  const int cc = tid & 7, rr = tid >> 3;
+  const int c = blockIdx.x * 8 + cc;
+  const int nblk = (n + LU_NB - 1) / LU_NB;
+  const T* db = dinv + ((long)b * nblk + k / LU_NB) * LU_NB * LU_NB;
+  for (int idx = tid; idx < LU_NB * LU_NB; idx += 256) {
+    int r = idx / LU_NB, q = idx - r * LU_NB;
+    T v;
+    if (UPPER)
+      v = q >= r ? db[idx] : s_zero<T>();
+    else
+      v = q < r ? db[idx] : (q == r ? s_one<T>() : s_zero<T>());
+    Di[r][q] = v;
   }
-  __syncthreads();
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nrhs) return;
   T* Xb = X + (long)b * sX;
-  T x[LU_NB];
-#pragma unroll
-  for (int j = 0; j < LU_NB; ++j)
-    if (j < nb) x[j] = Xb[(long)(k + j) * ldx + c];
-  if constexpr (!UPPER) {
-#pragma unroll
-    for (int j = 0; j < LU_NB; ++j)
-      if (j < nb)
-        for (int i = 0; i < j; ++i) x[j] = s_fnms(x[j], Ls[j][i], x[i]);
-  } else {
-#pragma unroll
-    for (int jj = LU_NB - 1; jj >= 0; --jj) {
-      if (jj < nb) {
-        for (int i = jj + 1; i < nb; ++i) x[jj] = s_fnms(x[jj], Ls[jj][i], x[i]);
-        x[jj] = s_mul(x[jj], s_inv(Ls[jj][jj]));
-      }
-    }
+  Xs[rr][cc] = (c < nrhs && rr < nb) ? Xb[(long)(k + rr) * ldx + c] : s_zero<T>();
+  __syncthreads();
+  if (c < nrhs && rr < nb) {
+    T acc = s_zero<T>();
+    for (int q = 0; q < nb; ++q) acc = s_add(acc, s_mul(Di[rr][q], Xs[q][cc]));
+    Xb[(long)(k + rr) * ldx + c] = acc;
   }
-#pragma unroll
-  for (int j = 0; j < LU_NB; ++j)
-    if (j < nb) Xb[(long)(k + j) * ldx + c] = x[j];
 }
 
 template <typename T>
-hipError_t getrs_perm(int n, int nrhs, const T* LU, long lda, long sA, const int* perm, const T* R, long ldr,
-                      long sR, T* X, long ldx, long sX, int batch, hipStream_t s) {
+hipError_t getrs_perm(int n, int nrhs, const T* LU, long lda, long sA, const int* perm, const T* dinv, const T* R,
+                      long ldr, long sR, T* X, long ldx, long sX, int batch, hipStream_t s) {
   if (n <= 0 || nrhs <= 0 || batch <= 0) return hipSuccess;
-  // batch goes in grid.z (<= 65535 in practice for this library's shapes)
   for (int b0 = 0; b0 < batch; b0 += 65535) {
+    static const std::string nm = std::string("getrs_gather<") + tname<T>() + ">";
+    prof::Scope ps(s, nm.c_str(), 0.0, 2.0 * sizeof(T) * (double)n * nrhs * batch);
     int bc = batch - b0 < 65535 ? batch - b0 : 65535;
     dim3 grid(ceil_div(nrhs, 64), n, bc);
     hipLaunchKernelGGL((gather_rows_kernel<T>), grid, dim3(64), 0, s, n, nrhs, perm + (long)b0 * n,
                        R + (long)b0 * sR, ldr, sR, X + (long)b0 * sX, ldx, sX);
   }
-  dim3 tgrid(ceil_div(nrhs, 256), batch);
+  const int nblk = (n + LU_NB - 1) / LU_NB;
+  dim3 tgrid(ceil_div(nrhs, 8), batch);
   for (int k = 0; k < n; k += LU_NB) {
     int nb = n - k < LU_NB ? n - k : LU_NB;
-    hipLaunchKernelGGL((trsm_block_kernel<T, false>), tgrid, dim3(256), 0, s, nrhs, LU, lda, sA, X, ldx, sX, k, nb);
+    {
+      static const std::string nm = std::string("trsm_lower<") + tname<T>() + ">";
+      prof::Scope ps(s, nm.c_str(), flop_mult<T>() * batch * (double)nb * nb * nrhs,
+                     2.0 * sizeof(T) * (double)nb * nrhs * batch);
+      hipLaunchKernelGGL((trsm_inv_kernel<T, false>), tgrid, dim3(256), 0, s, n, nrhs, dinv, X, ldx, sX, k, nb);
+    }
     int rem = n - k - nb;
     if (rem > 0) {
       GemmArgs<T> g;
@@ -326,11 +663,15 @@ hipError_t getrs_perm(int n, int nrhs, const T* LU, long lda, long sA, const int
       if (e != hipSuccess) return e;
     }
   }
-  int nblk = (n + LU_NB - 1) / LU_NB;
   for (int bi = nblk - 1; bi >= 0; --bi) {
     int k = bi * LU_NB;
     int nb = n - k < LU_NB ? n - k : LU_NB;
-    hipLaunchKernelGGL((trsm_block_kernel<T, true>), tgrid, dim3(256), 0, s, nrhs, LU, lda, sA, X, ldx, sX, k, nb);
+    {
+      static const std::string nm = std::string("trsm_upper<") + tname<T>() + ">";
+      prof::Scope ps(s, nm.c_str(), flop_mult<T>() * batch * (double)nb * nb * nrhs,
+                     2.0 * sizeof(T) * (double)nb * nrhs * batch);
+      hipLaunchKernelGGL((trsm_inv_kernel<T, true>), tgrid, dim3(256), 0, s, n, nrhs, dinv, X, ldx, sX, k, nb);
+    }
     if (k > 0) {
       GemmArgs<T> g;
       g.M = k;
@@ -358,10 +699,12 @@ hipError_t getrs_perm(int n, int nrhs, const T* LU, long lda, long sA, const int
   return hipGetLastError();
 }
 
-#define HPS_INST_LU(T)                                                                                        \
-  template hipError_t getrf_ws<T>(int, T*, long, long, int*, int*, int*, int, hipStream_t);                   \
-  template hipError_t getrs_perm<T>(int, int, const T*, long, long, const int*, const T*, long, long, T*, long, \
-                                    long, int, hipStream_t);
+size_t dinv_elems(int n) { return (size_t)((n + LU_NB - 1) / LU_NB) * LU_NB * LU_NB; }
+
+#define HPS_INST_LU(T)                                                                                          \
+  template hipError_t getrf_ws<T>(int, T*, long, long, int*, T*, int*, int*, int, hipStream_t);                 \
+  template hipError_t getrs_perm<T>(int, int, const T*, long, long, const int*, const T*, const T*, long, long, \
+                                    T*, long, long, int, hipStream_t);
 HPS_INST_LU(double)
 HPS_INST_LU(float)
 HPS_INST_LU(c128)

@@ -34,7 +34,7 @@ def _levels(T_arr, h_arr, l, kind, nc, table_fn, keep_B=False):
             h = h[:nc] if h is not None else None
         nec = T.shape[-1]
         tb = table_fn(nec)
-        res = merge_level(T.contiguous(), h.contiguous() if h is not None else None, tb, kind == "iti", keep_B=keep_B)
+        res = merge_level(T.contiguous(), h.contiguous() if h is not None else None, tb, keep_factors=keep_B)
         out.append(res)
         T, h = res.T, res.h
     return out, multi

@@ -55,7 +55,9 @@ def test_gemm_broadcast_operand(gpu, dtype):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
-@pytest.mark.parametrize("n,batch", [(5, 3), (36, 4), (196, 8), (256, 2), (33, 1), (700, 1)])
+@pytest.mark.parametrize(
+    "n,batch", [(5, 3), (36, 4), (196, 8), (256, 2), (33, 1), (700, 1), (300, 2), (1100, 1), (2500, 2), (4200, 1)]
+)
 def test_getrf_getrs_solve(gpu, dtype, n, batch):
     from jaxhps_amd import _lib
 
@@ -64,9 +66,9 @@ def test_getrf_getrs_solve(gpu, dtype, n, batch):
     A = _rand((batch, n, n), dtype, g)
     R = _rand((batch, n, 7), dtype, g)
     LU = A.to(gpu).contiguous()
-    perm, info = _lib.getrf(LU)
+    perm, dinv, info = _lib.getrf(LU)
     assert int(info.abs().sum()) == 0
-    X = _lib.getrs(LU, perm, R.to(gpu).contiguous())
+    X = _lib.getrs(LU, perm, dinv, R.to(gpu).contiguous())
     ref = torch.linalg.solve(_ref64(A), _ref64(R))
     rel = ((_ref64(X) - ref).abs().max() / ref.abs().max()).item()
     cond_scale = n
@@ -86,7 +88,7 @@ def test_getrf_reports_singular(gpu):
     A = torch.randn(3, 20, 20, dtype=torch.float64)
     A[1, :, 4] = 0.0  # column 4 of element 1 all zero
     LU = A.to(gpu).contiguous()
-    perm, info = _lib.getrf(LU)
+    perm, dinv, info = _lib.getrf(LU)
     info = info.cpu().tolist()
     assert info[0] == 0 and info[2] == 0
     assert info[1] == 5
