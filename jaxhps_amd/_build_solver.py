@@ -26,8 +26,11 @@ def build_solver(pde_problem, return_top_T: bool = False, compute_device=None, h
         raise NotImplementedError("the no-source build (solve(..., source=...)) is not implemented yet")
     if not dom.bool_2D:
         raise NotImplementedError("3D build is not implemented yet")
+    from . import profiling
+
     kind = HPS_KIND_ITI if pde_problem.use_ItI else HPS_KIND_DTN
-    res, v, h = _leaf(pde_problem, kind, compute_device)
+    with profiling.stage("leaf"):
+        res, v, h = _leaf(pde_problem, kind, compute_device)
     raise_on_info(res.info, "leaf stage")
     pde_problem.Y = res.Y
     pde_problem.v = v

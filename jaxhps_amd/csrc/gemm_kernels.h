@@ -1,0 +1,218 @@
+// GEMM kernel templates for gfx950 (included by gemm.hip).
+#pragma once
+
+#include "hps_kernels.h"
+
+namespace hps {
+
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+// Result layout of the 16x16x4 MFMAs (measured on gfx950 with an in-bounds
+// single-wave probe): register r of lane l holds
+//   f64: C[l/16 + 4r][l%16]        f32: C[4(l/16) + r][l%16]
+// while lane l supplies A[l%16][l/16] and B[l/16][l%16] for both.
+template <typename R> struct Mfma;
+template <> struct Mfma<double> {
+  using acc_t = dbl4;
+  static __device__ __forceinline__ int out_row(int lk, int r) { return lk + 4 * r; }
+  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct Mfma<float> {
+  using acc_t = flt4;
+  static __device__ __forceinline__ int out_row(int lk, int r) { return 4 * lk + r; }
+  static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+template <typename T> __device__ __forceinline__ T make_val(typename real_of<T>::type re, typename real_of<T>::type im);
+template <> __device__ __forceinline__ double make_val<double>(double re, double) { return re; }
+template <> __device__ __forceinline__ float make_val<float>(float re, float) { return re; }
+template <> __device__ __forceinline__ c128 make_val<c128>(double re, double im) { return c128{re, im}; }
+template <> __device__ __forceinline__ c64 make_val<c64>(float re, float im) { return c64{re, im}; }
+
+constexpr int GBK = 16;
+
+// Block tile (2*TM*16) x (2*TN*16) x 16; 4 waves in a 2x2 arrangement, each
+// owning TM x TN MFMA 16x16 blocks.  The next K-tile is fetched from global
+// memory into registers while the current one is multiplied out of LDS.
+template <typename T, int TM, int TN, int MINB = 1>
+__global__ __launch_bounds__(256, MINB) void gemm_mfma_kernel(GemmArgs<T> p, int tiles_n) {
+  using R = typename real_of<T>::type;
+  using M = Mfma<R>;
+  using acc_t = typename M::acc_t;
+  constexpr bool CPLX = is_cplx<T>::value;
+  constexpr int BM = 2 * TM * 16, BN = 2 * TN * 16;
+  constexpr int LA = BM * GBK / 256, LB = GBK * BN / 256;
+  __shared__ R As_re[GBK][BM + 1];
+  __shared__ R Bs_re[GBK][BN + 1];
+  __shared__ R As_im[CPLX ? GBK : 1][CPLX ? BM + 1 : 1];
+  __shared__ R Bs_im[CPLX ? GBK : 1][CPLX ? BN + 1 : 1];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int tm = blockIdx.x / tiles_n, tn = blockIdx.x % tiles_n;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int li = lane & 15, lk = lane >> 4;
+
+  for (int b = blockIdx.y; b < p.batch; b += gridDim.y) {
+    const T* A = p.A + (long)b * p.sA;
+    const T* B = p.B + (long)b * p.sB;
+    acc_t acc_re[TM][TN], acc_im[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        acc_re[i][j] = acc_t{0, 0, 0, 0};
+        acc_im[i][j] = acc_t{0, 0, 0, 0};
+      }
+    T ra[LA], rb[LB];
+    auto fetch = [&](int k0) {
+#pragma unroll
+      for (int i = 0; i < LA; ++i) {
+        int e = tid + 256 * i;
+        int r = e >> 4, c = e & 15;
+        int gr = m0 + r, gc = k0 + c;
+        ra[i] = (gr < p.M && gc < p.K) ? A[(long)gr * p.lda + gc] : s_zero<T>();
+      }
+#pragma unroll
+      for (int i = 0; i < LB; ++i) {
+        int e = tid + 256 * i;
+        int r = e / BN, c = e % BN;
+        int gr = k0 + r, gc = n0 + c;
+        rb[i] = (gr < p.K && gc < p.N) ? B[(long)gr * p.ldb + gc] : s_zero<T>();
+      }
+    };
+    fetch(0);
+    for (int k0 = 0; k0 < p.K; k0 += GBK) {
+#pragma unroll
+      for (int i = 0; i < LA; ++i) {
+        int e = tid + 256 * i;
+        int r = e >> 4, c = e & 15;
+        As_re[c][r] = s_re(ra[i]);
+        if constexpr (CPLX) As_im[c][r] = s_im(ra[i]);
+      }
+#pragma unroll
+      for (int i = 0; i < LB; ++i) {
+        int e = tid + 256 * i;
+        int r = e / BN, c = e % BN;
+        Bs_re[r][c] = s_re(rb[i]);
+        if constexpr (CPLX) Bs_im[r][c] = s_im(rb[i]);
+      }
+      __syncthreads();
+      if (k0 + GBK < p.K) fetch(k0 + GBK);
+#pragma unroll
+      for (int kk = 0; kk < GBK; kk += 4) {
+        R a_re[TM], b_re[TN], a_im[TM], b_im[TN];
+#pragma unroll
+        for (int t = 0; t < TM; ++t) {
+          a_re[t] = As_re[kk + lk][wr * (BM / 2) + t * 16 + li];
+          if constexpr (CPLX) a_im[t] = As_im[kk + lk][wr * (BM / 2) + t * 16 + li];
+        }
+#pragma unroll
+        for (int t = 0; t < TN; ++t) {
+          b_re[t] = Bs_re[kk + lk][wc * (BN / 2) + t * 16 + li];
+          if constexpr (CPLX) b_im[t] = Bs_im[kk + lk][wc * (BN / 2) + t * 16 + li];
+        }
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            acc_re[i][j] = M::mma(a_re[i], b_re[j], acc_re[i][j]);
+            if constexpr (CPLX) {
+              acc_re[i][j] = M::mma(-a_im[i], b_im[j], acc_re[i][j]);
+              acc_im[i][j] = M::mma(a_re[i], b_im[j], acc_im[i][j]);
+              acc_im[i][j] = M::mma(a_im[i], b_re[j], acc_im[i][j]);
+            }
+          }
+      }
+      __syncthreads();
+    }
+
+    const T* C = p.C ? p.C + (long)b * p.sC : nullptr;
+    T* D = p.D + (long)b * p.sD;
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          int row = m0 + wr * (BM / 2) + i * 16 + M::out_row(lk, r);
+          int col = n0 + wc * (BN / 2) + j * 16 + li;
+          if (row < p.M && col < p.N) {
+            T v = make_val<T>(acc_re[i][j][r], CPLX ? acc_im[i][j][r] : R(0));
+            v = s_scale(v, p.alpha);
+            if (p.beta != 0.0) v = s_add(v, s_scale(C[(long)row * p.ldc + col], p.beta));
+            D[(long)row * p.ldd + col] = v;
+          }
+        }
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ R wave_sum(R v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <typename T, int NMAX>
+__global__ __launch_bounds__(256) void gemm_skinny_kernel(GemmArgs<T> p) {
+  using R = typename real_of<T>::type;
+  constexpr bool CPLX = is_cplx<T>::value;
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (row >= p.M) return;
+  for (int b = blockIdx.y; b < p.batch; b += gridDim.y) {
+    const T* A = p.A + (long)b * p.sA + (long)row * p.lda;
+    const T* B = p.B + (long)b * p.sB;
+    R acc_re[NMAX], acc_im[NMAX];
+#pragma unroll
+    for (int n = 0; n < NMAX; ++n) {
+      acc_re[n] = R(0);
+      acc_im[n] = R(0);
+    }
+    for (int k = lane; k < p.K; k += 64) {
+      T a = A[k];
+      const T* Bk = B + (long)k * p.ldb;
+#pragma unroll
+      for (int n = 0; n < NMAX; ++n) {
+        if (n < p.N) {
+          T bv = Bk[n];
+          if constexpr (CPLX) {
+            acc_re[n] += s_re(a) * s_re(bv) - s_im(a) * s_im(bv);
+            acc_im[n] += s_re(a) * s_im(bv) + s_im(a) * s_re(bv);
+          } else {
+            acc_re[n] += s_re(a) * s_re(bv);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int n = 0; n < NMAX; ++n) {
+      if (n < p.N) {
+        acc_re[n] = wave_sum(acc_re[n]);
+        if constexpr (CPLX) acc_im[n] = wave_sum(acc_im[n]);
+      }
+    }
+    T* D = p.D + (long)b * p.sD + (long)row * p.ldd;
+    const T* C = p.C ? p.C + (long)b * p.sC + (long)row * p.ldc : nullptr;
+#pragma unroll
+    for (int n = 0; n < NMAX; ++n) {
+      if (n < p.N && lane == n) {
+        T v = make_val<T>(acc_re[n], acc_im[n]);
+        v = s_scale(v, p.alpha);
+        if (p.beta != 0.0) v = s_add(v, s_scale(C[n], p.beta));
+        D[n] = v;
+      }
+    }
+  }
+}
+
+
+}  // namespace hps

@@ -253,11 +253,24 @@ __global__ __launch_bounds__(THREADS) void getrf_panel_lds_kernel(int n, T* A, l
 // swap costs a register exchange and two LDS writes; the panel is put into
 // physical order once at the end (only the <= 2*nb rows the swaps touched
 // move).
+// Sub-panel width and left-looking prefetch depth keep the per-thread
+// register arrays near 64 (x) + 32 (prefetch) VGPRs.
+template <typename T, int RPT>
+constexpr int reg_panel_sw() {
+  constexpr int w = 64 / (RPT * (int)(sizeof(T) / 4));
+  return w >= 4 ? 4 : (w >= 2 ? 2 : 1);
+}
+template <typename T, int RPT>
+constexpr int reg_panel_jc() {
+  constexpr int w = 32 / (RPT * (int)(sizeof(T) / 4));
+  return w >= 4 ? 4 : (w >= 2 ? 2 : 1);
+}
+
 template <typename T, int RPT, int THREADS>
 __global__ __launch_bounds__(THREADS) void getrf_panel_reg_kernel(int n, T* A, long lda, long sA, int k, int nb,
                                                                   int* ipiv, int* info, T* dinv, int* swtab) {
   using R = typename real_of<T>::type;
-  constexpr int SW = 4;
+  constexpr int SW = reg_panel_sw<T, RPT>();
   __shared__ R red_v[THREADS / 64];
   __shared__ int red_i[THREADS / 64];
   __shared__ int s_piv;
@@ -282,49 +295,55 @@ __global__ __launch_bounds__(THREADS) void getrf_panel_reg_kernel(int n, T* A, l
   T x[RPT][SW];
   for (int c0 = 0; c0 < nb; c0 += SW) {
     const int sw = nb - c0 < SW ? nb - c0 : SW;
-    // U rows above: forward substitution with the unit-lower top block
+    // U rows above: the sub-panel's top c0 rows are staged in LDS in one
+    // parallel load, solved with the unit-lower top block, written back
+    for (int idx = tid; idx < c0 * sw; idx += THREADS) {
+      int j = idx / sw, q = idx - j * sw;
+      Ut[j][q] = P[(long)rowmap[j] * lda + c0 + q];
+    }
+    __syncthreads();
     if (tid < sw) {
-      const int c = c0 + tid;
       for (int j = 0; j < c0; ++j) {
-        T* Pj = P + (long)rowmap[j] * lda;
-        T acc = Pj[c];
+        T acc = Ut[j][tid];
         for (int i = 0; i < j; ++i) acc = s_fnms(acc, Lt[j][i], Ut[i][tid]);
         Ut[j][tid] = acc;
-        Pj[c] = acc;
       }
     }
     __syncthreads();
-    // load my rows of the sub-panel and apply the left-looking update
+    for (int idx = tid; idx < c0 * sw; idx += THREADS) {
+      int j = idx / sw, q = idx - j * sw;
+      P[(long)rowmap[j] * lda + c0 + q] = Ut[j][q];
+    }
+    // load my rows of the sub-panel and apply the left-looking update; the
+    // j-chunk loop is outside the row loop so RPT*4 loads are in flight
+    const T* Prow[RPT];
 #pragma unroll
     for (int i = 0; i < RPT; ++i) {
       const int r = tid + THREADS * i;
+      const bool live = r >= c0 && r < m;
+      Prow[i] = live ? P + (long)rowmap[r] * lda : nullptr;
 #pragma unroll
-      for (int q = 0; q < SW; ++q) x[i][q] = s_zero<T>();
-      if (r >= c0 && r < m) {
-        const T* Pr = P + (long)rowmap[r] * lda;
-#pragma unroll
-        for (int q = 0; q < SW; ++q)
-          if (q < sw) x[i][q] = Pr[c0 + q];
-        int j = 0;
-        for (; j + 4 <= c0; j += 4) {
-          T l0 = Pr[j], l1 = Pr[j + 1], l2 = Pr[j + 2], l3 = Pr[j + 3];
-#pragma unroll
-          for (int q = 0; q < SW; ++q) {
-            x[i][q] = s_fnms(x[i][q], l0, Ut[j][q]);
-            x[i][q] = s_fnms(x[i][q], l1, Ut[j + 1][q]);
-            x[i][q] = s_fnms(x[i][q], l2, Ut[j + 2][q]);
-            x[i][q] = s_fnms(x[i][q], l3, Ut[j + 3][q]);
-          }
-        }
-        for (; j < c0; ++j) {
-          T l = Pr[j];
-#pragma unroll
-          for (int q = 0; q < SW; ++q) x[i][q] = s_fnms(x[i][q], l, Ut[j][q]);
-        }
-      }
+      for (int q = 0; q < SW; ++q) x[i][q] = (live && q < sw) ? Prow[i][c0 + q] : s_zero<T>();
     }
-    // factor the sub-panel columns
-    for (int q = 0; q < sw; ++q) {
+    constexpr int JC = reg_panel_jc<T, RPT>();
+    for (int j = 0; j < c0; j += JC) {
+      T l[RPT][JC];
+#pragma unroll
+      for (int i = 0; i < RPT; ++i)
+#pragma unroll
+        for (int t = 0; t < JC; ++t) l[i][t] = (Prow[i] && j + t < c0) ? Prow[i][j + t] : s_zero<T>();
+#pragma unroll
+      for (int i = 0; i < RPT; ++i)
+#pragma unroll
+        for (int t = 0; t < JC; ++t)
+          if (j + t < c0)
+#pragma unroll
+            for (int q = 0; q < SW; ++q) x[i][q] = s_fnms(x[i][q], l[i][t], Ut[j + t][q]);
+    }
+    // factor the sub-panel columns (unrolled: x[i][q] needs compile-time q)
+#pragma unroll
+    for (int q = 0; q < SW; ++q) {
+      if (q >= sw) break;
       const int j = c0 + q;
       R bv = R(-1);
       int bi = 0x7fffffff;
@@ -432,6 +451,89 @@ __global__ __launch_bounds__(THREADS) void getrf_panel_reg_kernel(int n, T* A, l
   finish_panel<T>(&Lt[0][0], LU_NB + 1, &Dout[0][0], LU_NB + 1, k, nb, n, b, ipivb, dinv, swtab);
 }
 
+
+// Leaf-sized panels (m <= 256): thread t owns panel row t in registers; the
+// pivot row is broadcast through LDS.  No panel storage in LDS, so several
+// workgroups share a CU.
+template <typename T>
+__global__ __launch_bounds__(256) void getrf_panel_rows_kernel(int n, T* A, long lda, long sA, int k, int nb,
+                                                               int* ipiv, int* info, T* dinv, int* swtab) {
+  using R = typename real_of<T>::type;
+  constexpr int THREADS = 256;
+  __shared__ R red_v[THREADS / 64];
+  __shared__ int red_i[THREADS / 64];
+  __shared__ int s_piv;
+  __shared__ R s_val;
+  __shared__ T xa[LU_NB], xb[LU_NB], prow[LU_NB];
+  __shared__ T Dblk[LU_NB][LU_NB + 1];
+  __shared__ T Dout[LU_NB][LU_NB + 1];
+  const int b = blockIdx.x;
+  const int t = threadIdx.x;
+  const int m = n - k;
+  T* P = A + (long)b * sA + (long)k * lda + k;
+  int* ipivb = ipiv + (long)b * n;
+  const bool live = t < m;
+  // x holds this thread's row; every access uses a compile-time index (the
+  // column loop is a runtime loop and selects column j by predication), so
+  // the row stays in registers
+  T x[LU_NB];
+#pragma unroll
+  for (int c = 0; c < LU_NB; ++c) x[c] = (live && c < nb) ? P[(long)t * lda + c] : s_zero<T>();
+  for (int j = 0; j < nb; ++j) {
+    T xj = s_zero<T>();
+#pragma unroll
+    for (int c = 0; c < LU_NB; ++c)
+      if (c == j) xj = x[c];
+    R bv = R(-1);
+    int bi = 0x7fffffff;
+    if (live && t >= j) argmax_merge(bv, bi, s_abs1(xj), t);
+    int pr = block_argmax<R, THREADS>(bv, bi, red_v, red_i, &s_piv, &s_val);
+    if (pr < j || pr >= m) pr = j;
+    if (t == 0) record_pivot(k, j, pr, s_val == R(0), ipivb, info, b);
+    if (pr != j) {
+      if (t == j)
+#pragma unroll
+        for (int c = 0; c < LU_NB; ++c) xa[c] = x[c];
+      if (t == pr)
+#pragma unroll
+        for (int c = 0; c < LU_NB; ++c) xb[c] = x[c];
+      __syncthreads();
+      if (t == j)
+#pragma unroll
+        for (int c = 0; c < LU_NB; ++c) x[c] = xb[c];
+      if (t == pr)
+#pragma unroll
+        for (int c = 0; c < LU_NB; ++c) x[c] = xa[c];
+    }
+    if (t == j)
+#pragma unroll
+      for (int c = 0; c < LU_NB; ++c) prow[c] = x[c];
+    __syncthreads();
+    const T piv = prow[j];
+    if (live && t > j && !s_is_zero(piv)) {
+      T xs = s_zero<T>();
+#pragma unroll
+      for (int c = 0; c < LU_NB; ++c)
+        if (c == j) xs = x[c];
+      const T l = s_mul(xs, s_inv(piv));
+#pragma unroll
+      for (int c = 0; c < LU_NB; ++c) {
+        if (c == j) x[c] = l;
+        if (c > j && c < nb) x[c] = s_fnms(x[c], l, prow[c]);
+      }
+    }
+  }
+  if (live)
+#pragma unroll
+    for (int c = 0; c < LU_NB; ++c)
+      if (c < nb) P[(long)t * lda + c] = x[c];
+  if (t < nb)
+#pragma unroll
+    for (int c = 0; c < LU_NB; ++c)
+      if (c < nb) Dblk[t][c] = x[c];
+  finish_panel<T>(&Dblk[0][0], LU_NB + 1, &Dout[0][0], LU_NB + 1, k, nb, n, b, ipivb, dinv, swtab);
+}
+
 // Apply the panel's row permutation table to columns outside the panel and,
 // for the trailing columns, U12 = inv(L11) A12.  256 threads = 32 rows x 8
 // columns per workgroup.
@@ -490,10 +592,16 @@ static hipError_t launch_panel(int n, T* A, long lda, long sA, int k, int nb, in
                                int* swtab, int batch, hipStream_t s) {
   const int m = n - k;
   const size_t lds = (size_t)(m + LU_NB) * LU_NB * sizeof(T);
-  static const std::string nm = std::string("getrf_panel<") + tname<T>() + ">";
+  static const std::string nm_rows = std::string("getrf_panel_rows<") + tname<T>() + ">";
+  static const std::string nm_lds = std::string("getrf_panel_lds<") + tname<T>() + ">";
+  static const std::string nm_reg = std::string("getrf_panel_reg<") + tname<T>() + ">";
   const double pf = flop_mult<T>() * batch * ((double)m * nb * nb - (double)nb * nb * nb / 3.0);
-  prof::Scope ps(s, nm.c_str(), pf, 2.0 * sizeof(T) * (double)m * nb * batch);
-  if (lds <= LU_PANEL_LDS_MAX) {
+  prof::Scope ps(s, (m <= 256 ? nm_rows : lds <= LU_PANEL_LDS_MAX ? nm_lds : nm_reg).c_str(), pf,
+                 2.0 * sizeof(T) * (double)m * nb * batch);
+  if (m <= 256) {
+    hipLaunchKernelGGL((getrf_panel_rows_kernel<T>), dim3(batch), dim3(256), 0, s, n, A, lda, sA, k, nb, ipiv, info,
+                       dinv, swtab);
+  } else if (lds <= LU_PANEL_LDS_MAX) {
     static bool attr_set = false;
     if (!attr_set) {
       hipFuncSetAttribute((const void*)getrf_panel_lds_kernel<T, 256>, hipFuncAttributeMaxDynamicSharedMemorySize,

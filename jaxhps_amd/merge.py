@@ -16,9 +16,11 @@ from ._stages import merge_level
 from ._tables import quad_merge_tables
 
 
-def _levels(T_arr, h_arr, l, kind, nc, table_fn, keep_B=False):
+def _levels(T_arr, h_arr, l, kind, nc, table_fn, keep_B=False, root_T=True):
     """Run max(l-1, 0) batched levels plus the root merge, as the reference's
-    merge stages do (``merge/_uniform_2D_DtN.py:90-203``)."""
+    merge stages do (``merge/_uniform_2D_DtN.py:90-203``).  root_T=False skips
+    forming the root's T and h (only needed when the caller asks for them)."""
+    from . import profiling
     nec = T_arr.shape[-1]
     T = T_arr.reshape(-1, nec, nec)
     multi = h_arr is not None and h_arr.ndim == T_arr.ndim
@@ -34,7 +36,10 @@ def _levels(T_arr, h_arr, l, kind, nc, table_fn, keep_B=False):
             h = h[:nc] if h is not None else None
         nec = T.shape[-1]
         tb = table_fn(nec)
-        res = merge_level(T.contiguous(), h.contiguous() if h is not None else None, tb, keep_factors=keep_B)
+        want_T = root_T or lvl < n_levels - 1
+        with profiling.stage(f"merge_level_{lvl + 1}"):
+            res = merge_level(T.contiguous(), h.contiguous() if h is not None else None, tb, keep_factors=keep_B,
+                              want_T=want_T)
         out.append(res)
         T, h = res.T, res.h
     return out, multi
@@ -61,7 +66,8 @@ def merge_stage_uniform_2D_DtN(
     Returns S_lst (level j: (4^(l-1-j), 4m, 8m); root (1, 4m, 8m)),
     g_tilde_lst and optionally the top T.  subtree_recomp=True returns
     (T_top[None], h_top[None]) like the reference."""
-    levels, multi = _levels(T_arr, h_arr, l, "dtn", 4, lambda nec: quad_merge_tables(nec // 4, "dtn"))
+    levels, multi = _levels(T_arr, h_arr, l, "dtn", 4, lambda nec: quad_merge_tables(nec // 4, "dtn"),
+                            root_T=return_T or subtree_recomp)
     if subtree_recomp:
         top = levels[-1]
         hh = top.h if multi else top.h[..., 0]
@@ -81,7 +87,8 @@ def merge_stage_uniform_2D_ItI(
 ):
     """ItI 4:1 merges (``merge/_uniform_2D_ItI.py:19-179``).  S rows and g_tilde
     come out in the reference's order a5, b5, b6, c6, c7, d7, d8, a8."""
-    levels, multi = _levels(T_arr, h_arr, l, "iti", 4, lambda nec: quad_merge_tables(nec // 4, "iti"))
+    levels, multi = _levels(T_arr, h_arr, l, "iti", 4, lambda nec: quad_merge_tables(nec // 4, "iti"),
+                            root_T=return_T or return_h or subtree_recomp)
     if subtree_recomp:
         top = levels[-1]
         hh = top.h if multi else top.h[..., 0]

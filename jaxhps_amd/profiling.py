@@ -19,6 +19,30 @@ PEAK_TFLOPS = {"f64": 78.6, "c128": 78.6, "f32": 157.3, "c64": 157.3, "bf16": 25
 PEAK_HBM_GBS = 8000.0
 
 
+_stage_events = None
+
+
+class stage:
+    """Context manager recording a named stage with CUDA events while
+    instrumented_step is active (a no-op otherwise)."""
+
+    def __init__(self, name):
+        self.name = name
+
+    def __enter__(self):
+        if _stage_events is not None:
+            self.e0 = torch.cuda.Event(enable_timing=True)
+            self.e0.record()
+        return self
+
+    def __exit__(self, *exc):
+        if _stage_events is not None:
+            e1 = torch.cuda.Event(enable_timing=True)
+            e1.record()
+            _stage_events.append((self.name, self.e0, e1))
+        return False
+
+
 def kernel_table():
     lib = _lib.load_library()
     out = []
@@ -58,21 +82,28 @@ def instrumented_step(pde, g_dev, build=None, solve=None):
 
     build = build or jh.build_solver
     solve = solve or jh.solve
+    global _stage_events
     lib = _lib.load_library()
     torch.cuda.synchronize()
     lib.hps_prof_reset()
     lib.hps_prof_enable(1)
+    _stage_events = []
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-    ev[0].record()
-    build(pde)
-    ev[1].record()
-    solve(pde, g_dev)
-    ev[2].record()
-    torch.cuda.synchronize()
-    lib.hps_prof_enable(0)
+    try:
+        ev[0].record()
+        build(pde)
+        ev[1].record()
+        solve(pde, g_dev)
+        ev[2].record()
+        torch.cuda.synchronize()
+    finally:
+        lib.hps_prof_enable(0)
+        recorded, _stage_events = _stage_events, None
     table = kernel_table()
     lib.hps_prof_reset()
     stages = {"build_ms": ev[0].elapsed_time(ev[1]), "solve_ms": ev[1].elapsed_time(ev[2])}
+    for name, e0, e1 in recorded:
+        stages[name] = stages.get(name, 0.0) + e0.elapsed_time(e1)
     dominant = table[0] if table else None
     return {
         "stages_ms": stages,
