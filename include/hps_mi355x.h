@@ -139,6 +139,21 @@ int hps_merge_level(hps_handle_t h, int dtype, int kind, int nm, int nc, int nec
                     void* T_out, void* h_out, void* gt_out, void* F_lu, int* perm, void* F_dinv, int* info,
                     void* D12_out, void* D21_out, void* B_out, void* ws, size_t ws_bytes);
 
+/* ---- up pass (one tree level) --------------------------------------------
+ * New outgoing data through the factors a merge level kept (hps_merge_level
+ * with nsrc = 0 and F_lu/perm/F_dinv, D12/D21 (ItI) and B_out requested):
+ *   h_int = gather(h_in, int_rows) + inj,  gt_out = -D^{-1} h_int (final
+ *   order),  h_out = h_ext + B gt_out (h_out may be NULL at the root).
+ * inj (nm, ni, nsrc), internal equation order, or NULL: extra interface data
+ * (the interface residual of iterative refinement).
+ * Replaces up_pass_uniform_2D_ItI / _DtN with assemble_boundary_data
+ * (up_pass/_uniform_2D_ItI.py:8-219, up_pass/_uniform_2D_DtN.py:8-173). */
+size_t hps_up_workspace(int dtype, int kind, int nm, int ni, int nsrc);
+int hps_up_level(hps_handle_t h, int dtype, int kind, int nm, int nc, int nec, int ni, int ne, int nsrc,
+                 const void* h_in, const void* inj, const int* int_rows, const int* ext, const int* final_of_internal,
+                 int n_upd, const int* upd, const void* F_lu, const int* perm, const void* F_dinv, const void* D12,
+                 const void* D21, const void* B, void* gt_out, void* h_out, void* ws, size_t ws_bytes);
+
 /* ---- down pass (one tree level) -------------------------------------------
  * g_int = S g_in + gt; children's boundary data g_out (nm*nc, nec, nsrc) from
  * child_src (nc*nec ints): >= 0 parent exterior index, < 0 interface index

@@ -1,16 +1,34 @@
-"""``build_solver`` (``_build_solver.py:38-174``), uniform trees, device-resident."""
+"""``build_solver`` (``_build_solver.py:38-174``), uniform trees, device-resident.
+
+Besides the reference's state (Y, v, S_lst, g_tilde_lst) the build keeps the
+LU factors of every leaf and merge matrix plus each level's B (and D12, D21
+for ItI) in ``pde_problem._factors``; they serve solve(..., source=...),
+iterative refinement and the adjoint without refactorising.
+"""
 
 from __future__ import annotations
 
-import torch
+from ._lib import HPS_KIND_DTN, HPS_KIND_ITI, raise_on_info
+from .local_solve import _leaf
+from .merge import _levels
+from ._tables import quad_merge_tables
 
-from ._lib import raise_on_info
-from .local_solve import (
-    _leaf,
-    local_solve_stage_uniform_2D_DtN,
-)
-from ._lib import HPS_KIND_DTN, HPS_KIND_ITI
-from .merge import merge_stage_uniform_2D_DtN, merge_stage_uniform_2D_ItI
+
+def _merge_levels(pde_problem, T, h, return_top_T):
+    dom = pde_problem.domain
+    kind = "iti" if pde_problem.use_ItI else "dtn"
+    if dom.bool_2D:
+        table_fn = lambda nec: quad_merge_tables(nec // 4, kind)  # noqa: E731
+        nc = 4
+    else:
+        from ._tables3d import oct_merge_tables
+
+        table_fn = lambda nec: oct_merge_tables(nec // 6)  # noqa: E731
+        nc = 8
+    levels, multi = _levels(T, h, dom.L, kind, nc, table_fn, keep_B=True, root_T=return_top_T)
+    for r in levels:
+        raise_on_info(r.info, "merge stage")
+    return levels, multi
 
 
 def build_solver(pde_problem, return_top_T: bool = False, compute_device=None, host_device=None):
@@ -19,25 +37,52 @@ def build_solver(pde_problem, return_top_T: bool = False, compute_device=None, h
     ``return_top_T``.  ``compute_device`` may name the GPU (torch device);
     ``host_device`` is accepted for signature compatibility -- the solver
     state stays in HBM."""
+    from . import profiling
+
     dom = pde_problem.domain
     if not dom.bool_uniform:
         raise NotImplementedError("adaptive trees are outside this package's scope")
     if pde_problem.source is None:
-        raise NotImplementedError("the no-source build (solve(..., source=...)) is not implemented yet")
-    if not dom.bool_2D:
-        raise NotImplementedError("3D build is not implemented yet")
-    from . import profiling
-
+        if not dom.bool_2D:
+            raise ValueError(
+                "Build stage for problems without source terms is only implemented for 2D uniform ItI problems."
+            )
+        return _nosource_build_solver(pde_problem, return_top_T, compute_device)
     kind = HPS_KIND_ITI if pde_problem.use_ItI else HPS_KIND_DTN
     with profiling.stage("leaf"):
         res, v, h = _leaf(pde_problem, kind, compute_device)
     raise_on_info(res.info, "leaf stage")
+    levels, multi = _merge_levels(pde_problem, res.T, h, return_top_T)
     pde_problem.Y = res.Y
     pde_problem.v = v
-    merge_fn = merge_stage_uniform_2D_ItI if pde_problem.use_ItI else merge_stage_uniform_2D_DtN
-    out = merge_fn(res.T, h, l=dom.L, return_T=return_top_T)
-    pde_problem.S_lst = out[0]
-    pde_problem.g_tilde_lst = out[1]
+    pde_problem.S_lst = [r.S for r in levels]
+    pde_problem.g_tilde_lst = [r.gt if multi else r.gt[..., 0] for r in levels]
+    pde_problem._factors = {"leaf": res, "levels": levels}
     if return_top_T:
-        return out[2]
+        return levels[-1].T[0]
+    return None
+
+
+def _nosource_build_solver(pde_problem, return_top_T, compute_device):
+    """No-source build (``_build_solver.py:261-331``): leaf factors, S_lst and
+    the merge factors; sources are supplied later to solve(..., source=...)."""
+    from . import profiling
+    from ._stages import leaf_solve
+    from .local_solve import _compute_dtype, _resolve
+
+    dom = pde_problem.domain
+    kind = HPS_KIND_ITI if pde_problem.use_ItI else HPS_KIND_DTN
+    device = _resolve(compute_device)
+    dtype = _compute_dtype(pde_problem, force_complex=(kind == HPS_KIND_ITI))
+    with profiling.stage("leaf"):
+        res = leaf_solve(pde_problem, kind, dtype, device, nsrc_zero=True)
+    raise_on_info(res.info, "leaf stage")
+    levels, _ = _merge_levels(pde_problem, res.T, None, return_top_T)
+    pde_problem.Y = res.Y
+    pde_problem.v = None
+    pde_problem.S_lst = [r.S for r in levels]
+    pde_problem.g_tilde_lst = []
+    pde_problem._factors = {"leaf": res, "levels": levels}
+    if return_top_T:
+        return levels[-1].T[0]
     return None

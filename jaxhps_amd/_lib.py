@@ -53,6 +53,11 @@ SIGNATURES = {
         [_P, _I, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P,
          _P, _P, _P, _P, _S],
     ),
+    "hps_up_workspace": (_S, [_I, _I, _I, _I, _I]),
+    "hps_up_level": (
+        _I,
+        [_P, _I, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _S],
+    ),
     "hps_down_workspace": (_S, [_I, _I, _I, _I]),
     "hps_down_level": (_I, [_P, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _S]),
     "hps_leaf_apply": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P]),

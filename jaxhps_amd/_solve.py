@@ -3,6 +3,15 @@
 from __future__ import annotations
 
 from .down_pass import down_pass_uniform_2D_DtN, down_pass_uniform_2D_ItI
+from .up_pass import up_pass_uniform_2D_DtN, up_pass_uniform_2D_ItI
+
+
+def _up_then_down_pass(pde_problem, boundary_data, source):
+    """Up pass for the new source, then the down pass (``_solve.py:115-151``)."""
+    up = up_pass_uniform_2D_ItI if pde_problem.use_ItI else up_pass_uniform_2D_DtN
+    down = down_pass_uniform_2D_ItI if pde_problem.use_ItI else down_pass_uniform_2D_DtN
+    v, g_tilde_lst = up(source, pde_problem)
+    return down(boundary_data, pde_problem.S_lst, g_tilde_lst, pde_problem.Y, v)
 
 
 def solve(pde_problem, boundary_data, source=None, compute_device=None, host_device=None):
@@ -15,7 +24,7 @@ def solve(pde_problem, boundary_data, source=None, compute_device=None, host_dev
                 "Source can only be specified for 2D uniform ItI problems. For other problems, "
                 "the source must be specified at the time the solver is built."
             )
-        raise NotImplementedError("solve(..., source=...) is not implemented yet")
+        return _up_then_down_pass(pde_problem, boundary_data, source)
     if not dom.bool_uniform:
         raise NotImplementedError("adaptive trees are outside this package's scope")
     if not dom.bool_2D:

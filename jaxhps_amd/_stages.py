@@ -203,3 +203,64 @@ def leaf_apply(Y, g, v):
     rc = lib.hps_leaf_apply(hd, dtype_code(Y.dtype), nl, n, nq, nsrc, ptr(Y), ptr(g), ptr(v), ptr(u))
     check(rc, hd, "hps_leaf_apply")
     return u
+
+
+def getrs_into(n, lu, lu_offset, lda, sA, perm, dinv, R, X):
+    """X = A^{-1} R from factors stored in `lu` (element offset lu_offset,
+    row stride lda, batch stride sA).  R, X: (batch, n, nrhs) contiguous."""
+    lib = load_library()
+    hd = _lib.handle()
+    b = R.shape[0]
+    nrhs = R.shape[2]
+    es = lu.element_size()
+    rc = lib.hps_getrs_batched(
+        hd, dtype_code(lu.dtype), n, nrhs, ctypes.c_void_p(lu.data_ptr() + lu_offset * es), lda, sA, ptr(perm),
+        ptr(dinv), ptr(R), nrhs, n * nrhs, ptr(X), nrhs, n * nrhs, b,
+    )
+    check(rc, hd, "hps_getrs_batched")
+    return X
+
+
+def leaf_apply_inverse(leaf: LeafResult, rhs):
+    """v = (leaf operator)^{-1} rhs for a full collocation right-hand side
+    rhs (n_leaves, n, nsrc): DtN solves the interior rows with A_ii (v is zero
+    on the boundary); ItI solves with B = [G; A_int]."""
+    nl, nrows, n = leaf.lu.shape
+    nsrc = rhs.shape[-1]
+    if leaf.kind == HPS_KIND_DTN:
+        nb = n - nrows
+        R = rhs[:, nb:, :].contiguous()
+        X = torch.empty_like(R)
+        getrs_into(nrows, leaf.lu, nb, n, nrows * n, leaf.perm, leaf.dinv, R, X)
+        v = torch.zeros((nl, n, nsrc), dtype=rhs.dtype, device=rhs.device)
+        v[:, nb:, :] = X
+        return v
+    R = rhs.contiguous()
+    X = torch.empty_like(R)
+    getrs_into(n, leaf.lu, 0, n, n * n, leaf.perm, leaf.dinv, R, X)
+    return X
+
+
+def up_level(h_in, lvl: MergeLevel, inj=None, want_h=True):
+    """New outgoing data through the factors kept by a merge level.
+    h_in (nm*nc, nec, nsrc) -> (g_tilde (nm, ni, nsrc) final order, h_out (nm, ne, nsrc) or None)."""
+    lib = load_library()
+    hd = _lib.handle()
+    tb = lvl.tables
+    dev, dt = h_in.device, h_in.dtype
+    kind = HPS_KIND_ITI if tb.kind == "iti" else HPS_KIND_DTN
+    nm = lvl.F_lu.shape[0]
+    nsrc = h_in.shape[-1]
+    rows, _, ext, _, _, final_of_internal = tb.device(dev)
+    gt = torch.empty((nm, tb.ni, nsrc), dtype=dt, device=dev)
+    ho = torch.empty((nm, tb.ne, nsrc), dtype=dt, device=dev) if want_h else None
+    ws = workspace(lib.hps_up_workspace(dtype_code(dt), kind, nm, tb.ni, nsrc), dev)
+    upd = np.ascontiguousarray(tb.upd, dtype=np.int32)
+    rc = lib.hps_up_level(
+        hd, dtype_code(dt), kind, nm, tb.nc, tb.nec, tb.ni, tb.ne, nsrc, ptr(h_in.contiguous()),
+        ptr(inj.contiguous()) if inj is not None else None, ptr(rows), ptr(ext), ptr(final_of_internal),
+        upd.shape[0], upd.ctypes.data_as(ctypes.c_void_p), ptr(lvl.F_lu), ptr(lvl.perm), ptr(lvl.dinv),
+        ptr(lvl.D12), ptr(lvl.D21), ptr(lvl.B), ptr(gt), ptr(ho), ptr(ws), ws.numel(),
+    )
+    check(rc, hd, "hps_up_level")
+    return gt, ho

@@ -57,6 +57,34 @@ hipError_t copy2d(int rows, int cols, double alpha, const T* src, long lds, long
   return hipGetLastError();
 }
 
+// dst[b][r][c] += alpha * src[b][r][c]
+template <typename T>
+__global__ void add2d_kernel(int rows, int cols, double alpha, const T* src, long lds, long ss, T* dst, long ldd,
+                             long sd, int batch) {
+  long total = (long)rows * cols;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    int r = (int)(idx / cols), c = (int)(idx - (long)r * cols);
+    for (int b = blockIdx.y; b < batch; b += gridDim.y) {
+      T* d = dst + (long)b * sd + (long)r * ldd + c;
+      *d = s_add(*d, s_scale(src[(long)b * ss + (long)r * lds + c], alpha));
+    }
+  }
+}
+
+template <typename T>
+hipError_t add2d(int rows, int cols, double alpha, const T* src, long lds, long ss, T* dst, long ldd, long sd,
+                 int batch, hipStream_t s) {
+  if (rows <= 0 || cols <= 0 || batch <= 0) return hipSuccess;
+  long total = (long)rows * cols;
+  unsigned gx = ceil_div(total, 256);
+  if (gx > 4096) gx = 4096;
+  unsigned gy = batch < 65535 ? batch : 65535;
+  HPS_PROF("add2d", T, 0.0, 3.0 * sizeof(T) * (double)total * batch);
+  hipLaunchKernelGGL((add2d_kernel<T>), dim3(gx, gy), dim3(256), 0, s, rows, cols, alpha, src, lds, ss, dst, ldd, sd,
+                     batch);
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // leaf operator assembly: out[l][r][c] = sum_i coef_i[l][row0+r] * D_{op_i}[row0+r][col0+c]
 // op index < 0 means the identity operator.
@@ -708,6 +736,97 @@ int hps_merge_level(hps_handle_t h, int dtype, int kind, int nm, int nc, int nec
   DISPATCH(dtype, merge_t, h, kind, nm, nc, nec, ni, ne, nsrc, T_in, h_in, int_rows, int_cols, ext, cols_final,
            final_of_internal, n_upd, upd, S_out, T_out, h_out, gt_out, F_lu, perm, F_dinv, info, D12_out, D21_out,
            B_out, ws);
+}
+
+// ---------------------------------------------------------------------------
+// up pass level (new data through stored merge factors)
+// ---------------------------------------------------------------------------
+size_t hps_up_workspace(int dtype, int kind, int nm, int ni, int nsrc) {
+  size_t es = dtype_size(dtype);
+  const int nF = kind == HPS_KIND_ITI ? ni / 2 : ni;
+  return align_up((size_t)nm * ni * nsrc * es) + align_up((size_t)nm * nF * nsrc * es);
+}
+
+template <typename T>
+static int up_t(hps_handle_t h, int kind, int nm, int nc, int nec, int ni, int ne, int nsrc, const void* h_in,
+                const void* inj, const int* int_rows, const int* ext, const int* final_of_internal, int n_upd,
+                const int* upd, const void* F_lu, const int* perm, const void* F_dinv, const void* D12v,
+                const void* D21v, const void* Bv, void* gt_out, void* h_out, void* ws) {
+  hipStream_t s = h->stream;
+  const int4* rows = (const int4*)int_rows;
+  const int4* ex = (const int4*)ext;
+  const bool iti = kind == HPS_KIND_ITI;
+  const int nF = iti ? ni / 2 : ni;
+  const int w = nsrc;
+  const size_t es = sizeof(T);
+  char* wp = (char*)ws;
+  T* Yh = (T*)wp;
+  wp += align_up((size_t)nm * ni * w * es);
+  T* Zh = (T*)wp;
+  const T* Fl = (const T*)F_lu;
+  const T* dinv = (const T*)F_dinv;
+  const long sNF2 = (long)nF * nF;
+  const long sY = (long)ni * w;
+  {
+    static const std::string nmh = std::string("merge_gather_h<") + tname<T>() + ">";
+    prof::Scope ps(s, nmh.c_str(), 0.0, 2.0 * es * (double)ni * w * nm);
+    hipLaunchKernelGGL((vec_gather_kernel<T>), grid_for((long)ni * w, nm), dim3(256), 0, s, (const T*)h_in, nc, nec, w,
+                       rows, ni, -1.0, Yh, nm);
+    HPS_CHECK_HIP(h, hipGetLastError());
+  }
+  // extra interface data (internal equation order) enters with h_int
+  if (inj) HPS_CHECK_HIP(h, add2d<T>(ni, w, -1.0, (const T*)inj, w, sY, Yh, w, sY, nm, s));
+  T* out = (T*)gt_out;
+  if (!iti) {
+    HPS_CHECK_HIP(h, getrs_perm<T>(ni, w, Fl, ni, (long)ni * ni, perm, dinv, Yh, w, sY, out, w, sY, nm, s));
+  } else {
+    const long sZ = (long)nF * w;
+    HPS_CHECK_HIP(h, gemm_call<T>(s, nF, w, nF, -1.0, (const T*)D12v, nF, sNF2, Yh + (long)nF * w, w, sY, 1.0, Yh, w,
+                                  sY, Zh, w, sZ, nm));
+    HPS_CHECK_HIP(h, getrs_perm<T>(nF, w, Fl, nF, sNF2, perm, dinv, Zh, w, sZ, Yh, w, sY, nm, s));
+    HPS_CHECK_HIP(h, gemm_call<T>(s, nF, w, nF, -1.0, (const T*)D21v, nF, sNF2, Yh, w, sY, 1.0, Yh + (long)nF * w, w,
+                                  sY, Yh + (long)nF * w, w, sY, nm));
+    hipLaunchKernelGGL((row_scatter_kernel<T>), grid_for((long)ni * w, nm), dim3(256), 0, s, Yh, out,
+                       final_of_internal, ni, w, nm);
+    HPS_CHECK_HIP(h, hipGetLastError());
+  }
+  if (h_out) {
+    T* ho = (T*)h_out;
+    const long sO = (long)ne * w;
+    hipLaunchKernelGGL((vec_gather_kernel<T>), grid_for((long)ne * w, nm), dim3(256), 0, s, (const T*)h_in, nc, nec, w,
+                       ex, ne, 1.0, ho, nm);
+    HPS_CHECK_HIP(h, hipGetLastError());
+    const T* Bm = (const T*)Bv;
+    if (n_upd <= 0) {
+      HPS_CHECK_HIP(h, gemm_call<T>(s, ne, w, ni, 1.0, Bm, ni, (long)ne * ni, out, w, sY, 1.0, ho, w, sO, ho, w, sO, nm));
+    } else {
+      for (int u = 0; u < n_upd; ++u) {
+        const int r0 = upd[4 * u], nr = upd[4 * u + 1], k0 = upd[4 * u + 2], nk = upd[4 * u + 3];
+        HPS_CHECK_HIP(h, gemm_call<T>(s, nr, w, nk, 1.0, Bm + (long)r0 * ni + k0, ni, (long)ne * ni, out + (long)k0 * w,
+                                      w, sY, 1.0, ho + (long)r0 * w, w, sO, ho + (long)r0 * w, w, sO, nm));
+      }
+    }
+  }
+  return HPS_OK;
+}
+
+int hps_up_level(hps_handle_t h, int dtype, int kind, int nm, int nc, int nec, int ni, int ne, int nsrc,
+                 const void* h_in, const void* inj, const int* int_rows, const int* ext, const int* final_of_internal,
+                 int n_upd, const int* upd, const void* F_lu, const int* perm, const void* F_dinv, const void* D12,
+                 const void* D21, const void* B, void* gt_out, void* h_out, void* ws, size_t ws_bytes) {
+  if (!h) return HPS_ERR_ARG;
+  HPS_ARG(h, kind == HPS_KIND_DTN || kind == HPS_KIND_ITI, "kind must be DtN or ItI");
+  HPS_ARG(h, nm > 0 && nc > 0 && nec > 0 && ni > 0 && ne > 0 && nsrc > 0, "bad up-pass sizes");
+  HPS_ARG(h, kind == HPS_KIND_DTN || (ni % 2 == 0 && D12 && D21 && final_of_internal), "ItI needs D12, D21, row map");
+  HPS_ARG(h, h_in && int_rows && ext && F_lu && perm && F_dinv && gt_out, "null pointer");
+  HPS_ARG(h, !h_out || B, "h_out needs B");
+  HPS_ARG(h, n_upd >= 0 && (n_upd == 0 || upd), "bad update block list");
+  if (ws_bytes < hps_up_workspace(dtype, kind, nm, ni, nsrc)) {
+    h->err = "up workspace too small";
+    return HPS_ERR_WORKSPACE;
+  }
+  DISPATCH(dtype, up_t, h, kind, nm, nc, nec, ni, ne, nsrc, h_in, inj, int_rows, ext, final_of_internal, n_upd, upd,
+           F_lu, perm, F_dinv, D12, D21, B, gt_out, h_out, ws);
 }
 
 // ---------------------------------------------------------------------------

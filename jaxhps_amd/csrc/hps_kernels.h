@@ -47,6 +47,10 @@ hipError_t getrs_perm(int n, int nrhs, const T* LU, long lda, long sA, const int
 // dst[b][r][c] = alpha * src[b][r][c]; src == nullptr writes zeros; a batch
 // stride of 0 on src broadcasts.
 template <typename T>
+hipError_t add2d(int rows, int cols, double alpha, const T* src, long lds, long ss, T* dst, long ldd, long sd,
+                 int batch, hipStream_t s);
+
+template <typename T>
 hipError_t copy2d(int rows, int cols, double alpha, const T* src, long lds, long ss, T* dst, long ldd, long sd,
                   int batch, hipStream_t s);
 

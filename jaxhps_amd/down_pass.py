@@ -28,12 +28,14 @@ def _down(boundary_data, S_lst, g_tilde_lst, Y_arr, v_arr, table_fn, name):
         multi = True
     for lvl in range(len(S_lst) - 1, -1, -1):
         S = S_lst[lvl].contiguous()
-        gt = g_tilde_lst[lvl]
-        gt = gt if gt.ndim == 3 else gt.unsqueeze(-1)
-        if gt.shape[-1] != g.shape[-1]:
-            gt = gt.expand(-1, -1, g.shape[-1])
+        gt = g_tilde_lst[lvl] if g_tilde_lst else None  # empty after a no-source build
+        if gt is not None:
+            gt = gt if gt.ndim == 3 else gt.unsqueeze(-1)
+            if gt.shape[-1] != g.shape[-1]:
+                gt = gt.expand(-1, -1, g.shape[-1])
+            gt = gt.contiguous()
         tb = table_fn(S.shape[-1])
-        g = down_level(S, gt.contiguous(), g.contiguous(), tb)
+        g = down_level(S, gt, g.contiguous(), tb)
     if Y_arr is None:
         return g if multi else g[..., 0]
     v = v_arr

@@ -23,6 +23,7 @@ def _levels(T_arr, h_arr, l, kind, nc, table_fn, keep_B=False, root_T=True):
     from . import profiling
     nec = T_arr.shape[-1]
     T = T_arr.reshape(-1, nec, nec)
+    # the reference's multi-source test (merge/_uniform_2D_DtN.py:73)
     multi = h_arr is not None and h_arr.ndim == T_arr.ndim
     if h_arr is None:
         h = None

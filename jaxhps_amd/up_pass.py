@@ -1,0 +1,68 @@
+"""Up pass for a new source with a prebuilt solver, named like the
+reference's ``up_pass`` package (``up_pass/_uniform_2D_ItI.py:8-219``,
+``up_pass/_uniform_2D_DtN.py:8-173``).
+
+The reference stores explicit inverses (Phi, D^{-1}, B D^{-1}); this package
+keeps the LU factors of every leaf and merge matrix (and B, D12, D21) from
+the build and applies them: v = (leaf operator)^{-1} f, h = Q v, then per
+level g_tilde = -D^{-1} h_int and h_ext' = h_ext + B g_tilde.
+"""
+
+from __future__ import annotations
+
+import torch
+
+from ._stages import leaf_apply_inverse, to_device, up_level
+
+
+def _factors(pde_problem):
+    f = getattr(pde_problem, "_factors", None)
+    if f is None:
+        raise ValueError("the solver was built without factors; call build_solver first")
+    return f
+
+
+def _up(source, pde_problem, return_h_last=False, inj_lst=None):
+    fac = _factors(pde_problem)
+    leaf = fac["leaf"]
+    dev, dt = leaf.lu.device, leaf.lu.dtype
+    src = to_device(source, dev, dt)
+    multi = src.ndim == 3
+    if not multi:
+        src = src.unsqueeze(-1)
+    nb = pde_problem.P.shape[0]
+    rhs = src.clone()
+    if pde_problem.use_ItI:
+        rhs[:, :nb, :] = 0  # ItI: the boundary rows carry the incoming data, zero here
+    v = leaf_apply_inverse(leaf, rhs)
+    Q = pde_problem.device_ops(dev, dt)["Q"]
+    h = _q_apply(Q, v)
+    g_lst = []
+    levels = fac["levels"]
+    for i, lvl in enumerate(levels):
+        last = i == len(levels) - 1
+        nc = lvl.tables.nc
+        inj = inj_lst[i] if inj_lst is not None else None
+        gt, h = up_level(h.reshape(-1, lvl.tables.nec, h.shape[-1]), lvl, inj=inj, want_h=(not last) or return_h_last)
+        g_lst.append(gt if multi else gt[..., 0])
+    v_out = v if multi else v[..., 0]
+    if return_h_last:
+        return v_out, g_lst, (h[0] if multi else h[0, :, 0])
+    return v_out, g_lst
+
+
+def _q_apply(Q, v):
+    from . import _lib
+
+    return _lib.gemm(Q, v)
+
+
+def up_pass_uniform_2D_ItI(source, pde_problem, device=None, host_device=None, return_h_last: bool = False):
+    """v and g_tilde_lst for a new source (n_leaves, p^2[, nsrc]) with an ItI
+    solver built by build_solver (``up_pass/_uniform_2D_ItI.py:8-136``)."""
+    return _up(source, pde_problem, return_h_last)
+
+
+def up_pass_uniform_2D_DtN(source, pde_problem, device=None, host_device=None, return_h_last: bool = False):
+    """DtN version (``up_pass/_uniform_2D_DtN.py:8-107``)."""
+    return _up(source, pde_problem, return_h_last)

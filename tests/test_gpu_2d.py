@@ -146,3 +146,70 @@ def test_multi_source_and_oracle_solution(gpu, kind):
     assert _rel(u, u_ref) < 1e-10
     with pytest.raises(ValueError):
         jh.solve(pde, g[:, 0])
+
+
+@pytest.mark.parametrize("kind", ["dtn", "iti"])
+@pytest.mark.parametrize("nsrc", [None, 2])
+def test_up_pass_new_source_matches_rebuild(gpu, kind, nsrc):
+    """solve(pde, g, source=f2) on a solver built for f1 equals a fresh build for f2."""
+    import jaxhps_amd as jh
+
+    print(f"up pass kind={kind} nsrc={nsrc}", flush=True)
+    dom, pde1 = _problem(8, 6, 3, kind, nsrc=nsrc, seed=0)
+    _, pde2 = _problem(8, 6, 3, kind, nsrc=nsrc, seed=0)
+    rng = np.random.default_rng(11)
+    f2 = rng.standard_normal(pde1.source.shape)
+    pde2.source = f2 if kind == "dtn" else f2 + 0.5j * rng.standard_normal(f2.shape)
+    jh.build_solver(pde1)
+    jh.build_solver(pde2)
+    nb = dom.boundary_points.shape[0]
+    g = rng.standard_normal((nb,) + ((nsrc,) if nsrc else ()))
+    if kind == "iti":
+        g = g + 1j * rng.standard_normal(g.shape)
+    u_new = _np(jh.solve(pde1, g, source=pde2.source))
+    u_ref = _np(jh.solve(pde2, g))
+    assert _rel(u_new, u_ref) < 1e-11
+    from jaxhps_amd.up_pass import up_pass_uniform_2D_DtN, up_pass_uniform_2D_ItI
+
+    fn = up_pass_uniform_2D_ItI if kind == "iti" else up_pass_uniform_2D_DtN
+    v, g_lst = fn(pde2.source, pde1)
+    assert _rel(_np(v), _np(pde2.v)) < 1e-11
+    for a, b in zip(g_lst, pde2.g_tilde_lst):
+        assert _rel(_np(a), _np(b)) < 1e-10
+
+
+@pytest.mark.parametrize("L", [1, 3])
+def test_nosource_iti_build_then_source_solve(gpu, L):
+    """No-source ItI build + solve(source=f) against the oracle's explicit-inverse
+    up pass (Phi, D^{-1}, B D^{-1}) and against the with-source build."""
+    import jaxhps_amd as jh
+
+    print(f"nosource iti L={L}", flush=True)
+    dom, pde_src = _problem(8, 6, L, "iti", nsrc=2)
+    f = pde_src.source
+    ones = np.ones(dom.interior_points.shape[:2])
+    pde = jh.PDEProblem(dom, D_xx_coefficients=ones, D_yy_coefficients=ones, I_coefficients=cs.CPLX * ones,
+                        use_ItI=True, eta=cs.ETA)
+    Ttop = jh.build_solver(pde, return_top_T=True)
+    jh.build_solver(pde_src)
+    rng = np.random.default_rng(5)
+    g = rng.standard_normal((dom.boundary_points.shape[0], 2)) + 1j * rng.standard_normal((dom.boundary_points.shape[0], 2))
+    u = _np(jh.solve(pde, g, source=f))
+    assert _rel(u, _np(jh.solve(pde_src, g))) < 1e-11
+    # oracle: explicit inverses
+    half = (dom.root.xmax - dom.root.xmin) / 2 ** (L + 1)
+    ops = osp.leaf_ops_2D(dom.p, dom.q, half, eta=cs.ETA)
+    coeffs = {"D_xx": ones, "D_yy": ones, "I": cs.CPLX * ones}
+    A = hps2d.assemble_diff_operator(coeffs, ops)
+    T0, Y0, Phi = hps2d.leaf_iti_nosource(A, ops["P_iti"], ops["QH"], ops["G"])
+    S0, Di, BDi, Tt0 = hps2d.merge_stage_iti_nosource(T0, L)
+    assert _rel(_np(Ttop), Tt0) < 1e-10
+    for a, b in zip(pde.S_lst, S0):
+        assert _rel(_np(a), b) < 1e-10
+    v0, g0 = hps2d.up_pass_iti(f, Phi, ops["QH"], Di, BDi, ops["P_iti"].shape[0])
+    from jaxhps_amd.up_pass import up_pass_uniform_2D_ItI
+
+    v, g_lst = up_pass_uniform_2D_ItI(f, pde)
+    assert _rel(_np(v), v0) < 1e-11
+    for a, b in zip(g_lst, g0):
+        assert _rel(_np(a), b) < 1e-10
