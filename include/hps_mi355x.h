@@ -169,6 +169,40 @@ int hps_down_level(hps_handle_t h, int dtype, int nm, int nc, int nec, int ni, i
 int hps_leaf_apply(hps_handle_t h, int dtype, int nl, int n, int nq, int nsrc, const void* Y, const void* g,
                    const void* v, void* u);
 
+/* ---- iterative refinement --------------------------------------------------
+ * The reference has no mixed-precision path; these entry points serve the
+ * FP32/complex64 build with FP64 refinement.  Each evaluates, in FP64, the
+ * residual of the global system the uniform build+solve solves exactly:
+ * the leaf rows of get_DtN / get_ItI (local_solve/_uniform_2D_DtN.py:228-280,
+ * local_solve/_uniform_2D_ItI.py:120-193) and the interface conditions the
+ * merges eliminate (merge/_schur_complement.py:183-237 for DtN flux balance,
+ * merge/_uniform_2D_ItI.py:183-375 for ItI impedance exchange).
+ *
+ * R (nl, nrow_R, nsrc): [op_0 u | ... | op_{nops-1} u | extra], one GEMM of
+ * the stacked operators with the iterate u (nl, n, nsrc); extra = [G u | QH u]
+ * (ItI, nrow_R = nops*n + nb + nq) or [Q u] (DtN, nrow_R = nops*n + nq).
+ * coeffs (ncoef, nl, n) with coef_ops[k] in [-1, nops) (-1 = identity).
+ * nbr (nl, nq): neighbour slot l'*nq + j' (>= 0) or -(root index + 1).
+ * out_dtype is dtype or its FP32 partner (the correction's precision);
+ * part (nl) receives per-leaf sums of |residual|^2. */
+int hps_residual_iti(hps_handle_t h, int dtype, int out_dtype, int nl, int n, int nb, int nq, int nsrc, int nops,
+                     int ncoef, const int* coef_ops, const void* coeffs, const void* R, int nrow_R, const void* u,
+                     const void* f, const void* P, const void* g_root, const int* nbr, void* res, double* part);
+/* DtN: u's boundary rows must equal P g_leaf (g_leaf (nl, nq, nsrc), the
+ * leaves' Gauss boundary data).  res (nl, n, nsrc): interior rows f - A u,
+ * boundary rows 0; hinj (nl, nq, nsrc): interface flux mismatch on one slot
+ * of each interior Gauss node; groot_res (nroot, nsrc): g_root - g_leaf. */
+int hps_residual_dtn(hps_handle_t h, int dtype, int out_dtype, int nl, int n, int nb, int nq, int nsrc, int nops,
+                     int ncoef, const int* coef_ops, const void* coeffs, const void* R, int nrow_R, const void* u,
+                     const void* f, const void* g_leaf, const void* g_root, const int* nbr, void* res, void* hinj,
+                     void* groot_res, double* part);
+/* x += dx over n elements (dx in dtype or its FP32 partner); nparts blocks,
+ * part (2*nparts): per-block sums of |dx|^2 and |x|^2. */
+int hps_refine_update(hps_handle_t h, int dtype, int delta_dtype, long n, void* x, const void* dx, int nparts,
+                      double* part);
+/* dst = src converted elementwise (complex -> real keeps the real part). */
+int hps_convert(hps_handle_t h, int src_dtype, int dst_dtype, long n, const void* src, void* dst);
+
 /* ---- optional kernel timing ----------------------------------------------
  * When enabled, every kernel launch of the library is bracketed by HIP events
  * on its stream and attributed to a kernel class ("gemm_mfma<c128>", ...)

@@ -14,7 +14,26 @@ from .merge import _levels
 from ._tables import quad_merge_tables
 
 
-def _merge_levels(pde_problem, T, h, return_top_T):
+def _level_dtypes(n_levels, leaf_dtype, precision, fp64_top_levels):
+    if precision == "fp64":
+        return None
+    from ._refine import HIGH
+
+    hi = HIGH[leaf_dtype]
+    return [hi if lvl >= n_levels - fp64_top_levels else leaf_dtype for lvl in range(n_levels)]
+
+
+def _compute_dtypes(pde_problem, precision):
+    from .local_solve import _compute_dtype
+    from ._refine import LOW
+
+    if precision not in ("fp64", "mixed"):
+        raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
+    dt = _compute_dtype(pde_problem, force_complex=pde_problem.use_ItI)
+    return dt if precision == "fp64" else LOW[dt]
+
+
+def _merge_levels(pde_problem, T, h, return_top_T, dtypes=None):
     dom = pde_problem.domain
     kind = "iti" if pde_problem.use_ItI else "dtn"
     if dom.bool_2D:
@@ -25,34 +44,44 @@ def _merge_levels(pde_problem, T, h, return_top_T):
 
         table_fn = lambda nec: oct_merge_tables(nec // 6)  # noqa: E731
         nc = 8
-    levels, multi = _levels(T, h, dom.L, kind, nc, table_fn, keep_B=True, root_T=return_top_T)
+    levels, multi = _levels(T, h, dom.L, kind, nc, table_fn, keep_B=True, root_T=return_top_T, dtypes=dtypes)
     for r in levels:
         raise_on_info(r.info, "merge stage")
     return levels, multi
 
 
-def build_solver(pde_problem, return_top_T: bool = False, compute_device=None, host_device=None):
+def build_solver(pde_problem, return_top_T: bool = False, compute_device=None, host_device=None,
+                 precision: str = "fp64", fp64_top_levels: int = 2):
     """Leaf solves and merges for the whole tree; stores Y, v, S_lst and
     g_tilde_lst in ``pde_problem``.  Returns the top-level T when
     ``return_top_T``.  ``compute_device`` may name the GPU (torch device);
     ``host_device`` is accepted for signature compatibility -- the solver
-    state stays in HBM."""
+    state stays in HBM.
+
+    precision="mixed" (2D uniform trees) factors the leaves and all but the
+    top ``fp64_top_levels`` merge levels in FP32 (complex64 for ItI); solve()
+    then refines its solution to FP64 accuracy."""
     from . import profiling
 
     dom = pde_problem.domain
     if not dom.bool_uniform:
         raise NotImplementedError("adaptive trees are outside this package's scope")
+    leaf_dtype = _compute_dtypes(pde_problem, precision)
+    if precision == "mixed" and not dom.bool_2D:
+        raise NotImplementedError("the mixed-precision build covers 2D uniform trees")
+    pde_problem._precision = precision
+    dtypes = _level_dtypes(dom.L if dom.L > 0 else 0, leaf_dtype, precision, fp64_top_levels)
     if pde_problem.source is None:
         if not dom.bool_2D:
             raise ValueError(
                 "Build stage for problems without source terms is only implemented for 2D uniform ItI problems."
             )
-        return _nosource_build_solver(pde_problem, return_top_T, compute_device)
+        return _nosource_build_solver(pde_problem, return_top_T, compute_device, leaf_dtype, dtypes)
     kind = HPS_KIND_ITI if pde_problem.use_ItI else HPS_KIND_DTN
     with profiling.stage("leaf"):
-        res, v, h = _leaf(pde_problem, kind, compute_device)
+        res, v, h = _leaf(pde_problem, kind, compute_device, dtype=leaf_dtype)
     raise_on_info(res.info, "leaf stage")
-    levels, multi = _merge_levels(pde_problem, res.T, h, return_top_T)
+    levels, multi = _merge_levels(pde_problem, res.T, h, return_top_T, dtypes)
     pde_problem.Y = res.Y
     pde_problem.v = v
     pde_problem.S_lst = [r.S for r in levels]
@@ -63,7 +92,7 @@ def build_solver(pde_problem, return_top_T: bool = False, compute_device=None, h
     return None
 
 
-def _nosource_build_solver(pde_problem, return_top_T, compute_device):
+def _nosource_build_solver(pde_problem, return_top_T, compute_device, dtype=None, dtypes=None):
     """No-source build (``_build_solver.py:261-331``): leaf factors, S_lst and
     the merge factors; sources are supplied later to solve(..., source=...)."""
     from . import profiling
@@ -73,11 +102,12 @@ def _nosource_build_solver(pde_problem, return_top_T, compute_device):
     dom = pde_problem.domain
     kind = HPS_KIND_ITI if pde_problem.use_ItI else HPS_KIND_DTN
     device = _resolve(compute_device)
-    dtype = _compute_dtype(pde_problem, force_complex=(kind == HPS_KIND_ITI))
+    if dtype is None:
+        dtype = _compute_dtype(pde_problem, force_complex=(kind == HPS_KIND_ITI))
     with profiling.stage("leaf"):
         res = leaf_solve(pde_problem, kind, dtype, device, nsrc_zero=True)
     raise_on_info(res.info, "leaf stage")
-    levels, _ = _merge_levels(pde_problem, res.T, None, return_top_T)
+    levels, _ = _merge_levels(pde_problem, res.T, None, return_top_T, dtypes)
     pde_problem.Y = res.Y
     pde_problem.v = None
     pde_problem.S_lst = [r.S for r in levels]

@@ -156,6 +156,11 @@ class PDEProblem:
         # factorisations kept on the device for the up pass, refinement and
         # the adjoint (LU of every leaf and merge matrix)
         self._factors = None
+        self._precision = "fp64"
+        self.refine_history: List = []
+        # the refinement operators hold the coefficient fields
+        for k in [k for k in self._dev_cache if k[0] == "refine"]:
+            del self._dev_cache[k]
 
     def update_coefficients(
         self,

@@ -1,0 +1,222 @@
+"""Mixed-precision solve with FP64 iterative refinement (uniform 2D trees).
+
+``build_solver(..., precision="mixed")`` factors the leaves and the lower merge
+levels in FP32 (complex64 for ItI) and the top ``fp64_top_levels`` merge
+levels in FP64.  ``solve`` then computes a first solution through those
+factors and refines it: the residual of the global collocation system is
+evaluated in FP64 (``hps_residual_{iti,dtn}``), the correction goes through
+the same stored factors (up pass + down pass), and the FP64 iterate is
+updated until the relative update falls below ``refine_tol``.
+
+The global system is the one the reference's uniform build+solve eliminates
+exactly (leaf rows of ``local_solve/_uniform_2D_ItI.py:120-193`` /
+``_uniform_2D_DtN.py:228-280``; interface coupling of
+``merge/_uniform_2D_ItI.py:183-375`` / ``merge/_schur_complement.py:183-237``),
+so the refined solution converges to the FP64 HPS solution.
+
+ItI state: u on every leaf's Chebyshev grid.  Residual rows: [P t_in - G u;
+f_int - A_int u] with t_in the root data or minus the neighbour's outgoing
+impedance.  The correction is an up pass whose leaf right-hand side has both
+row blocks, and a down pass from zero root data.
+
+DtN state: the leaves' Gauss boundary data g_leaf (shared values agree on both
+sides of an interface) and u, whose boundary rows are P g_leaf.  Residuals:
+interior rows f - A u, the flux mismatch Q u_a + Q u_b on interior Gauss nodes
+(added to one leaf's h in the up pass) and g_root - g_leaf on the root
+boundary (the down pass's root data).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import warnings
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import HPS_KIND_DTN, check, dtype_code, load_library, ptr
+from ._neighbors import gauss_neighbors_2D
+from ._stages import cast, leaf_apply_inverse, to_device
+
+LOW = {torch.float64: torch.float32, torch.complex128: torch.complex64}
+HIGH = {torch.float32: torch.float64, torch.complex64: torch.complex128,
+        torch.float64: torch.float64, torch.complex128: torch.complex128}
+_NPARTS = 1024
+
+
+class RefineOps:
+    """FP64 operators of the global system for one PDEProblem on one device."""
+
+    def __init__(self, pde, device, dtype):
+        dom = pde.domain
+        names = pde.operator_names()
+        clist = pde.coefficient_list()
+        used = sorted({names.index(nm) for nm, _ in clist if nm != "I"})
+        self.coef_ops = np.array([-1 if nm == "I" else used.index(names.index(nm)) for nm, _ in clist],
+                                 dtype=np.int32)
+        self.nops = len(used)
+        mats = [np.asarray(getattr(pde, names[i])) for i in used]
+        mats += [np.asarray(pde.G), np.asarray(pde.QH)] if pde.use_ItI else [np.asarray(pde.Q)]
+        M = np.concatenate(mats, axis=0)
+        self.nrow_R = M.shape[0]
+
+        def dev(x):
+            return torch.as_tensor(np.ascontiguousarray(x)).to(device=device, dtype=dtype)
+
+        self.M = dev(M)
+        self.MT = dev(M.T)
+        self.P = dev(pde.P)
+        self.coeffs = torch.stack([to_device(a, device, dtype) for _, a in clist]).contiguous()
+        self.nbr = torch.as_tensor(gauss_neighbors_2D(dom.L, dom.q)).to(device)
+        self.nb, self.nq = pde.P.shape
+        self.n = dom.p**2
+        self.nl = dom.n_leaves
+        self.dtype = dtype
+        self.device = device
+
+    def products(self, u):
+        """R (nl, nrow_R, w) = [op_k u | extra] for the iterate u (nl, n, w)."""
+        nl, n, w = u.shape
+        if w == 1:
+            return _lib.gemm(u.view(nl, n), self.MT).view(nl, self.nrow_R, 1)
+        return _lib.gemm(self.M, u)
+
+
+def refine_ops(pde, device, dtype):
+    key = ("refine", str(device), dtype)
+    if key not in pde._dev_cache:
+        pde._dev_cache[key] = RefineOps(pde, device, dtype)
+    return pde._dev_cache[key]
+
+
+def residual_iti(ops: RefineOps, u, f, g_root, out_dtype):
+    """FP64 residual (nl, n, w) in out_dtype of the ItI system; returns (res, sum |res|^2 as a device scalar)."""
+    lib = load_library()
+    hd = _lib.handle()
+    nl, n, w = u.shape
+    R = ops.products(u)
+    res = torch.empty((nl, n, w), dtype=out_dtype, device=u.device)
+    part = torch.empty((nl,), dtype=torch.float64, device=u.device)
+    rc = lib.hps_residual_iti(
+        hd, dtype_code(ops.dtype), dtype_code(out_dtype), nl, n, ops.nb, ops.nq, w, ops.nops, len(ops.coef_ops),
+        ops.coef_ops.ctypes.data_as(ctypes.c_void_p), ptr(ops.coeffs), ptr(R), ops.nrow_R, ptr(u), ptr(f),
+        ptr(ops.P), ptr(g_root), ptr(ops.nbr), ptr(res), ptr(part),
+    )
+    check(rc, hd, "hps_residual_iti")
+    return res, part.sum()
+
+
+def residual_dtn(ops: RefineOps, u, g_leaf, f, g_root, out_dtype):
+    """FP64 DtN residuals: (res (nl, n, w), hinj (nl, nq, w), groot (nroot, w), sum of squares)."""
+    lib = load_library()
+    hd = _lib.handle()
+    nl, n, w = u.shape
+    R = ops.products(u)
+    res = torch.empty((nl, n, w), dtype=out_dtype, device=u.device)
+    hinj = torch.empty((nl, ops.nq, w), dtype=out_dtype, device=u.device)
+    groot = torch.empty((g_root.shape[0], w), dtype=out_dtype, device=u.device)
+    part = torch.empty((nl,), dtype=torch.float64, device=u.device)
+    rc = lib.hps_residual_dtn(
+        hd, dtype_code(ops.dtype), dtype_code(out_dtype), nl, n, ops.nb, ops.nq, w, ops.nops, len(ops.coef_ops),
+        ops.coef_ops.ctypes.data_as(ctypes.c_void_p), ptr(ops.coeffs), ptr(R), ops.nrow_R, ptr(u), ptr(f),
+        ptr(g_leaf), ptr(g_root), ptr(ops.nbr), ptr(res), ptr(hinj), ptr(groot), ptr(part),
+    )
+    check(rc, hd, "hps_residual_dtn")
+    return res, hinj, groot, part.sum()
+
+
+def refine_update(x, dx):
+    """x += dx in x's precision; returns (sum |dx|^2, sum |x|^2) as device scalars."""
+    lib = load_library()
+    hd = _lib.handle()
+    part = torch.empty((2 * _NPARTS,), dtype=torch.float64, device=x.device)
+    rc = lib.hps_refine_update(hd, dtype_code(x.dtype), dtype_code(dx.dtype), x.numel(), ptr(x), ptr(dx.contiguous()),
+                               _NPARTS, ptr(part))
+    check(rc, hd, "hps_refine_update")
+    p = part.view(_NPARTS, 2).sum(0)
+    return p[0], p[1]
+
+
+def set_boundary_rows(ops: RefineOps, u, g_leaf):
+    """u[:, :nb, :] = P g_leaf in u's precision (DtN iterate consistency)."""
+    lib = load_library()
+    hd = _lib.handle()
+    nl, n, w = u.shape
+    rc = lib.hps_gemm_batched(
+        hd, dtype_code(u.dtype), ops.nb, w, ops.nq, 1.0, ptr(ops.P), ops.nq, 0, ptr(g_leaf), w, ops.nq * w, 0.0,
+        None, w, 0, ptr(u), w, n * w, nl,
+    )
+    check(rc, hd, "hps_gemm_batched")
+
+
+def _as3(x):
+    return x if x.ndim == 3 else x.unsqueeze(-1)
+
+
+def _root_data(bd, device, dtype, w):
+    if isinstance(bd, (list, tuple)):
+        bd = torch.cat([to_device(b, device, dtype) for b in bd])
+    t = to_device(bd, device, dtype)
+    if t.ndim == 1:
+        t = t.unsqueeze(-1)
+    if t.shape[-1] != w:
+        t = t.expand(-1, w)
+    return t.contiguous()
+
+
+def refine_solve(pde, boundary_data, u0, source, g_leaf0=None, tol=1e-11, max_iter=10):
+    """Refine the mixed-precision solution u0 (nl, n[, w]) to FP64.
+
+    source: the FP64 source the solution is for (None = zero).  g_leaf0: the
+    DtN leaves' Gauss data of the first solve.  Returns u in FP64 (c128 for
+    complex problems) shaped like u0, and records the iteration history in
+    ``pde.refine_history``."""
+    from .down_pass import down_levels
+    from .up_pass import _up
+
+    squeeze = u0.ndim == 2
+    u0 = _as3(u0)
+    dev = u0.device
+    hi = HIGH[u0.dtype]
+    lo = LOW[hi]
+    ops = refine_ops(pde, dev, hi)
+    nl, n, w = u0.shape
+    u = cast(u0, hi).contiguous()
+    f = None
+    if source is not None:
+        f = _as3(to_device(source, dev, hi))
+        if f.shape[-1] != w:
+            f = f.expand(-1, -1, w)
+        f = f.contiguous()
+    g_root = _root_data(boundary_data, dev, hi, w)
+    iti = pde.use_ItI
+    g_leaf = None
+    if not iti:
+        g_leaf = cast(_as3(g_leaf0), hi).contiguous()
+        set_boundary_rows(ops, u, g_leaf)
+    hist = []
+    converged = False
+    for it in range(max_iter):
+        if iti:
+            res, rn2 = residual_iti(ops, u, f, g_root, lo)
+            v, g_lst = _up(res, pde, rhs_full=True)
+            du, _ = down_levels(torch.zeros_like(g_root), pde.S_lst, g_lst, pde.Y, v)
+            dn2, un2 = refine_update(u, du)
+        else:
+            res, hinj, groot, rn2 = residual_dtn(ops, u, g_leaf, f, g_root, lo)
+            v, g_lst = _up(res, pde, rhs_full=True, hinj=hinj)
+            du, dg = down_levels(groot, pde.S_lst, g_lst, pde.Y, v)
+            dn2, un2 = refine_update(u, du)
+            refine_update(g_leaf, dg)
+            set_boundary_rows(ops, u, g_leaf)
+        dn, un, rn = (float(x) for x in torch.stack([dn2, un2, rn2]).sqrt().cpu())
+        hist.append({"iter": it + 1, "residual": rn, "rel_update": dn / max(un, 1e-300)})
+        if dn <= tol * un:
+            converged = True
+            break
+    pde.refine_history = hist
+    if not converged:
+        warnings.warn(f"iterative refinement stopped after {max_iter} iterations at relative update "
+                      f"{hist[-1]['rel_update']:.2e} (tol {tol:.1e})", RuntimeWarning, stacklevel=3)
+    return u[..., 0] if squeeze else u

@@ -14,10 +14,38 @@ def _up_then_down_pass(pde_problem, boundary_data, source):
     return down(boundary_data, pde_problem.S_lst, g_tilde_lst, pde_problem.Y, v)
 
 
-def solve(pde_problem, boundary_data, source=None, compute_device=None, host_device=None):
+def _solve_mixed(pde_problem, boundary_data, source, refine_tol, max_refine):
+    """First solve through the mixed-precision factors, then FP64 refinement."""
+    from ._refine import refine_solve
+    from .down_pass import _down
+
+    if source is not None:
+        up = up_pass_uniform_2D_ItI if pde_problem.use_ItI else up_pass_uniform_2D_DtN
+        v, g_lst = up(source, pde_problem)
+        f = source
+    else:
+        v, g_lst = pde_problem.v, pde_problem.g_tilde_lst
+        f = pde_problem.source
+    u0, g_leaf = _down(boundary_data, pde_problem.S_lst, g_lst, pde_problem.Y, v, return_leaf_g=True)
+    return refine_solve(pde_problem, boundary_data, u0, f, g_leaf0=g_leaf, tol=refine_tol, max_iter=max_refine)
+
+
+def solve(pde_problem, boundary_data, source=None, compute_device=None, host_device=None,
+          refine_tol: float = 1e-11, max_refine: int = 10):
     """Propagate root boundary data down the tree; returns u (n_leaves, p^d[, nsrc])
-    as a device tensor."""
+    as a device tensor.
+
+    After build_solver(..., precision="mixed") the solution is refined in FP64
+    until the relative update is below ``refine_tol`` (at most ``max_refine``
+    iterations; the history is left in ``pde_problem.refine_history``)."""
     dom = pde_problem.domain
+    if getattr(pde_problem, "_precision", "fp64") == "mixed":
+        if source is not None and (not dom.bool_2D or not dom.bool_uniform):
+            raise ValueError(
+                "Source can only be specified for 2D uniform ItI problems. For other problems, "
+                "the source must be specified at the time the solver is built."
+            )
+        return _solve_mixed(pde_problem, boundary_data, source, refine_tol, max_refine)
     if source is not None:
         if not dom.bool_2D or not dom.bool_uniform:
             raise ValueError(

@@ -264,3 +264,16 @@ def up_level(h_in, lvl: MergeLevel, inj=None, want_h=True):
     )
     check(rc, hd, "hps_up_level")
     return gt, ho
+
+
+def cast(t, dtype):
+    """t converted to `dtype` on the device by hps_convert (t itself if it already has it)."""
+    if t is None or t.dtype == dtype:
+        return t
+    lib = load_library()
+    hd = _lib.handle()
+    src = t.contiguous()
+    out = torch.empty(src.shape, dtype=dtype, device=src.device)
+    rc = lib.hps_convert(hd, dtype_code(src.dtype), dtype_code(dtype), src.numel(), ptr(src), ptr(out))
+    check(rc, hd, "hps_convert")
+    return out

@@ -15,15 +15,7 @@
 #include <cstring>
 #include <string>
 
-#include "hps_kernels.h"
-#include "hps_prof.h"
-#include "../../include/hps_mi355x.h"
-
-// Open a timing scope named "<label><dtype>" for the next launch (no-op unless
-// profiling is enabled).
-#define HPS_PROF(label, T, flops, bytes)                                                 \
-  static const std::string _pn_##__LINE__ = std::string(label "<") + tname<T>() + ">"; \
-  prof::Scope _ps_##__LINE__(s, _pn_##__LINE__.c_str(), (flops), (bytes))
+#include "hps_capi_internal.h"
 
 namespace hps {
 
@@ -89,10 +81,6 @@ hipError_t add2d(int rows, int cols, double alpha, const T* src, long lds, long 
 // leaf operator assembly: out[l][r][c] = sum_i coef_i[l][row0+r] * D_{op_i}[row0+r][col0+c]
 // op index < 0 means the identity operator.
 // ---------------------------------------------------------------------------
-struct CoefOps {
-  int op[16];
-};
-
 template <typename T>
 __global__ __launch_bounds__(256) void assemble_kernel(int n, int nl, int ncoef, CoefOps ops_idx, const T* coeffs,
                                                        const T* diffops, int row0, int nrows, int col0, int ncols,
@@ -185,45 +173,11 @@ __global__ __launch_bounds__(256) void down_scatter_kernel(int nc, int nec, int 
   }
 }
 
-static dim3 grid_for(long total, int batch) {
-  unsigned gx = ceil_div(total, 256);
-  if (gx > 2048) gx = 2048;
-  if (gx == 0) gx = 1;
-  unsigned gy = batch < 65535 ? batch : 65535;
-  return dim3(gx, gy);
-}
-
 }  // namespace hps
 
 // ===========================================================================
 // C ABI
 // ===========================================================================
-struct hps_handle_s {
-  hipStream_t stream = nullptr;
-  std::string err;
-};
-
-using namespace hps;
-
-#define HPS_CHECK_HIP(h, expr)                                                      \
-  do {                                                                              \
-    hipError_t _e = (expr);                                                         \
-    if (_e != hipSuccess) {                                                         \
-      (h)->err = std::string(#expr) + ": " + hipGetErrorString(_e);                 \
-      return HPS_ERR_HIP;                                                           \
-    }                                                                               \
-  } while (0)
-
-#define HPS_ARG(h, cond, msg)   \
-  do {                          \
-    if (!(cond)) {              \
-      (h)->err = (msg);         \
-      return HPS_ERR_ARG;       \
-    }                           \
-  } while (0)
-
-static size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
-
 // The entry points below get C linkage from their declarations in
 // hps_mi355x.h; the template helpers between them stay C++.
 
@@ -279,15 +233,6 @@ static int gemm_t(hps_handle_t h, int M, int N, int K, double alpha, const void*
   HPS_CHECK_HIP(h, gemm<T>(g, h->stream));
   return HPS_OK;
 }
-
-#define DISPATCH(dtype, FN, ...)                                  \
-  switch (dtype) {                                                \
-    case HPS_F64: return FN<double>(__VA_ARGS__);                 \
-    case HPS_F32: return FN<float>(__VA_ARGS__);                  \
-    case HPS_C128: return FN<c128>(__VA_ARGS__);                  \
-    case HPS_C64: return FN<c64>(__VA_ARGS__);                    \
-    default: h->err = "bad dtype"; return HPS_ERR_ARG;            \
-  }
 
 int hps_gemm_batched(hps_handle_t h, int dtype, int M, int N, int K, double alpha, const void* A, long lda, long sA,
                      const void* B, long ldb, long sB, double beta, const void* C, long ldc, long sC, void* D,

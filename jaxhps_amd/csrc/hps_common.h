@@ -110,6 +110,28 @@ __host__ __device__ inline bool s_is_zero(float a) { return a == 0.0f; }
 __host__ __device__ inline bool s_is_zero(c128 a) { return a.re == 0.0 && a.im == 0.0; }
 __host__ __device__ inline bool s_is_zero(c64 a) { return a.re == 0.0f && a.im == 0.0f; }
 
+// squared magnitude, accumulated in double
+__host__ __device__ inline double s_norm2(double a) { return a * a; }
+__host__ __device__ inline double s_norm2(float a) { return (double)a * a; }
+__host__ __device__ inline double s_norm2(c128 a) { return a.re * a.re + a.im * a.im; }
+__host__ __device__ inline double s_norm2(c64 a) { return (double)a.re * a.re + (double)a.im * a.im; }
+
+// conversion between the four scalar types (complex -> real keeps the real part)
+template <typename To> struct cvt_to;
+template <> struct cvt_to<double> {
+  template <typename F> __host__ __device__ static double from(F x) { return (double)s_re(x); }
+};
+template <> struct cvt_to<float> {
+  template <typename F> __host__ __device__ static float from(F x) { return (float)s_re(x); }
+};
+template <> struct cvt_to<c128> {
+  template <typename F> __host__ __device__ static c128 from(F x) { return c128{(double)s_re(x), (double)s_im(x)}; }
+};
+template <> struct cvt_to<c64> {
+  template <typename F> __host__ __device__ static c64 from(F x) { return c64{(float)s_re(x), (float)s_im(x)}; }
+};
+template <typename To, typename F> __host__ __device__ inline To s_cvt(F x) { return cvt_to<To>::from(x); }
+
 // Precision codes shared with the C ABI (include/hps_mi355x.h).
 enum Dtype : int { HPS_F64 = 0, HPS_F32 = 1, HPS_C128 = 2, HPS_C64 = 3 };
 

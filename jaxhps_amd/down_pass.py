@@ -10,55 +10,71 @@ from typing import List
 
 import torch
 
-from ._stages import down_level, leaf_apply, to_device
+from ._stages import cast, down_level, leaf_apply, to_device
 from ._tables import quad_merge_tables
 
 
-def _down(boundary_data, S_lst, g_tilde_lst, Y_arr, v_arr, table_fn, name):
+def down_levels(g_root, S_lst, g_tilde_lst, Y_arr, v_arr, return_leaf_g=False):
+    """Root data (ne_root, nsrc) -> (u (nl, n, nsrc) or None, leaf boundary data
+    (nl, nec, nsrc)).  Each level runs in the precision of its S; the leaf
+    apply in the precision of Y.  g_tilde_lst may be empty (no-source build)."""
+    g = g_root.reshape(1, g_root.shape[0], -1)
+    for lvl in range(len(S_lst) - 1, -1, -1):
+        S = S_lst[lvl].contiguous()
+        g = cast(g, S.dtype)
+        gt = g_tilde_lst[lvl] if g_tilde_lst else None
+        if gt is not None:
+            gt = gt if gt.ndim == 3 else gt.unsqueeze(-1)
+            if gt.shape[-1] != g.shape[-1]:
+                gt = gt.expand(-1, -1, g.shape[-1])
+            gt = cast(gt.contiguous(), S.dtype)
+        tb = _tables_for(S)
+        g = down_level(S, gt, g.contiguous(), tb)
+    if Y_arr is None:
+        return None, g
+    g = cast(g, Y_arr.dtype)
+    v = v_arr
+    if v is not None:
+        v = v if v.ndim == 3 else v.unsqueeze(-1)
+        if v.shape[-1] != g.shape[-1]:
+            v = v.expand(-1, -1, g.shape[-1])
+        v = cast(v.contiguous(), Y_arr.dtype)
+    return leaf_apply(Y_arr.contiguous(), g, v), g
+
+
+def _tables_for(S):
+    ni, ne = S.shape[-2], S.shape[-1]
+    kind = "iti" if ni == ne else "dtn"  # ItI S is 8m x 8m, DtN S is 4m x 8m
+    return quad_merge_tables(ne // 8, kind)
+
+
+def _down(boundary_data, S_lst, g_tilde_lst, Y_arr, v_arr, return_leaf_g=False):
     if isinstance(boundary_data, (list, tuple)):
-        boundary_data = torch.cat([to_device(b, S_lst[0].device, S_lst[0].dtype) for b in boundary_data])
+        boundary_data = torch.cat([to_device(b, S_lst[0].device, S_lst[-1].dtype) for b in boundary_data])
     S0 = S_lst[-1]
     dev, dt = S0.device, S0.dtype
     bd = to_device(boundary_data, dev, dt)
     multi = len(g_tilde_lst) > 0 and g_tilde_lst[0].ndim == 3
     if multi and bd.ndim == 1:
         raise ValueError("For multi-source downward pass, need to specify boundary data for each source.")
-    g = bd.reshape(1, bd.shape[0], -1) if bd.ndim == 2 else bd.reshape(1, -1, 1)
+    g = bd.reshape(bd.shape[0], -1)
     if not multi and g.shape[-1] != 1:
         multi = True
-    for lvl in range(len(S_lst) - 1, -1, -1):
-        S = S_lst[lvl].contiguous()
-        gt = g_tilde_lst[lvl] if g_tilde_lst else None  # empty after a no-source build
-        if gt is not None:
-            gt = gt if gt.ndim == 3 else gt.unsqueeze(-1)
-            if gt.shape[-1] != g.shape[-1]:
-                gt = gt.expand(-1, -1, g.shape[-1])
-            gt = gt.contiguous()
-        tb = table_fn(S.shape[-1])
-        g = down_level(S, gt, g.contiguous(), tb)
-    if Y_arr is None:
-        return g if multi else g[..., 0]
-    v = v_arr
-    if v is not None:
-        v = v if v.ndim == 3 else v.unsqueeze(-1)
-        if v.shape[-1] != g.shape[-1]:
-            v = v.expand(-1, -1, g.shape[-1])
-        v = v.contiguous()
-    u = leaf_apply(Y_arr.contiguous(), g, v)
-    return u if multi else u[..., 0]
+    u, g_leaf = down_levels(g, S_lst, g_tilde_lst, Y_arr, v_arr)
+    out = g_leaf if u is None else u
+    out = out if multi else out[..., 0]
+    if return_leaf_g:
+        return out, g_leaf
+    return out
 
 
 def down_pass_uniform_2D_DtN(boundary_data, S_lst, g_tilde_lst, Y_arr, v_arr, device=None, host_device=None):
     """Dirichlet data on the root boundary -> leaf solutions
     (``down_pass/_uniform_2D_DtN.py:7-122``)."""
-    return _down(
-        boundary_data, S_lst, g_tilde_lst, Y_arr, v_arr, lambda ne: quad_merge_tables(ne // 8, "dtn"), "dtn"
-    )
+    return _down(boundary_data, S_lst, g_tilde_lst, Y_arr, v_arr)
 
 
 def down_pass_uniform_2D_ItI(boundary_data, S_lst, g_tilde_lst, Y_arr, v_arr, device=None, host_device=None):
     """Incoming impedance data on the root boundary -> leaf solutions
     (``down_pass/_uniform_2D_ItI.py:8-118``)."""
-    return _down(
-        boundary_data, S_lst, g_tilde_lst, Y_arr, v_arr, lambda ne: quad_merge_tables(ne // 8, "iti"), "iti"
-    )
+    return _down(boundary_data, S_lst, g_tilde_lst, Y_arr, v_arr)

@@ -12,14 +12,16 @@ from typing import List
 import torch
 
 from ._lib import raise_on_info
-from ._stages import merge_level
+from ._stages import cast, merge_level
 from ._tables import quad_merge_tables
 
 
-def _levels(T_arr, h_arr, l, kind, nc, table_fn, keep_B=False, root_T=True):
+def _levels(T_arr, h_arr, l, kind, nc, table_fn, keep_B=False, root_T=True, dtypes=None):
     """Run max(l-1, 0) batched levels plus the root merge, as the reference's
     merge stages do (``merge/_uniform_2D_DtN.py:90-203``).  root_T=False skips
-    forming the root's T and h (only needed when the caller asks for them)."""
+    forming the root's T and h (only needed when the caller asks for them).
+    dtypes[i], when given, is the precision level i merges in (the children's
+    T and h are converted on entry)."""
     from . import profiling
     nec = T_arr.shape[-1]
     T = T_arr.reshape(-1, nec, nec)
@@ -35,6 +37,9 @@ def _levels(T_arr, h_arr, l, kind, nc, table_fn, keep_B=False, root_T=True):
         if lvl == n_levels - 1:
             T = T[:nc]
             h = h[:nc] if h is not None else None
+        if dtypes is not None:
+            T = cast(T, dtypes[lvl])
+            h = cast(h, dtypes[lvl])
         nec = T.shape[-1]
         tb = table_fn(nec)
         want_T = root_T or lvl < n_levels - 1
