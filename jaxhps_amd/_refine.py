@@ -72,8 +72,19 @@ class RefineOps:
         self.nb, self.nq = pde.P.shape
         self.n = dom.p**2
         self.nl = dom.n_leaves
+        self.nroot = 4 * dom.q * 2**dom.L
         self.dtype = dtype
         self.device = device
+
+    def check(self, u, f, g_root, g_leaf=None):
+        """Shapes the residual kernels index with (they do no bounds checks)."""
+        nl, n, w = u.shape
+        ok = (nl, n) == (self.nl, self.n) and tuple(g_root.shape) == (self.nroot, w)
+        ok = ok and (f is None or tuple(f.shape) == (nl, n, w))
+        ok = ok and (g_leaf is None or tuple(g_leaf.shape) == (nl, self.nq, w))
+        ok = ok and all(t is None or (t.dtype == self.dtype and t.is_contiguous()) for t in (u, f, g_root, g_leaf))
+        if not ok:
+            raise ValueError("refinement operands do not match the problem's shapes / dtype")
 
     def products(self, u):
         """R (nl, nrow_R, w) = [op_k u | extra] for the iterate u (nl, n, w)."""
@@ -94,6 +105,7 @@ def residual_iti(ops: RefineOps, u, f, g_root, out_dtype):
     """FP64 residual (nl, n, w) in out_dtype of the ItI system; returns (res, sum |res|^2 as a device scalar)."""
     lib = load_library()
     hd = _lib.handle()
+    ops.check(u, f, g_root)
     nl, n, w = u.shape
     R = ops.products(u)
     res = torch.empty((nl, n, w), dtype=out_dtype, device=u.device)
@@ -111,6 +123,7 @@ def residual_dtn(ops: RefineOps, u, g_leaf, f, g_root, out_dtype):
     """FP64 DtN residuals: (res (nl, n, w), hinj (nl, nq, w), groot (nroot, w), sum of squares)."""
     lib = load_library()
     hd = _lib.handle()
+    ops.check(u, f, g_root, g_leaf)
     nl, n, w = u.shape
     R = ops.products(u)
     res = torch.empty((nl, n, w), dtype=out_dtype, device=u.device)
@@ -190,6 +203,9 @@ def refine_solve(pde, boundary_data, u0, source, g_leaf0=None, tol=1e-11, max_it
             f = f.expand(-1, -1, w)
         f = f.contiguous()
     g_root = _root_data(boundary_data, dev, hi, w)
+    nroot = 4 * pde.domain.q * 2**pde.domain.L
+    if g_root.shape[0] != nroot:
+        raise ValueError(f"boundary data has {g_root.shape[0]} points, the root boundary has {nroot}")
     iti = pde.use_ItI
     g_leaf = None
     if not iti:

@@ -195,6 +195,20 @@ def test_library_exports_every_declared_symbol():
     assert lib.hps_version() == 1
 
 
+def test_ctypes_signatures_match_header_arity():
+    """Every ctypes signature has as many arguments as the header declaration."""
+    from jaxhps_amd import _lib
+
+    header = (ROOT / "include" / "hps_mi355x.h").read_text()
+    header = re.sub(r"/\*.*?\*/", "", header, flags=re.S)
+    decls = re.findall(r"^\s*(?:int|size_t|const char\*)\s+(hps_\w+)\s*\(([^)]*)\)\s*;", header, re.M)
+    assert len(decls) >= 15
+    for name, params in decls:
+        params = params.strip()
+        n = 0 if params in ("", "void") else len(params.split(","))
+        assert len(_lib.SIGNATURES[name][1]) == n, (name, n, len(_lib.SIGNATURES[name][1]))
+
+
 def test_library_argument_errors_without_gpu():
     """Argument checks run before any device work, so they are testable on the CPU."""
     from jaxhps_amd import _lib
