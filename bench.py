@@ -5,9 +5,15 @@ A step is one build_solver() + solve() of the workload (all right-hand
 sides), with coefficients, source and boundary data already in HBM.  DOF =
 n_leaves * p^d.  Prints ONE JSON line (rank 0).
 
+Two paths are timed on the same inputs: the mixed-precision build
+(complex64 / FP32 factors, FP64 top two merge levels) whose solve is refined
+to FP64 accuracy -- the reported value -- and the plain FP64 path; the line
+carries the relative error of the mixed result against the FP64 one and the
+speed-up.
+
 Default workload: 2D Helmholtz, ItI merges, k = 100, p = 16, L = 7 (16384
-leaves, 4,194,304 DOF), complex128 ("fp64" path).  --config B selects the 2D
-variable-coefficient DtN workload (p = 16, L = 6, 16 RHS).
+leaves, 4,194,304 DOF).  --config B selects the 2D variable-coefficient DtN
+workload (p = 16, L = 6, 16 RHS).
 """
 
 from __future__ import annotations
@@ -31,6 +37,10 @@ def parse():
     ap.add_argument("--L", type=int, default=None, help="override tree depth (testing only)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-only", action="store_true", help="skip the CPU baseline and extra passes")
+    ap.add_argument("--precision", default="both", choices=["both", "mixed", "fp64"],
+                    help="which path(s) to time; the value is the mixed path's unless only fp64 runs")
+    ap.add_argument("--fp64-top-levels", type=int, default=2)
+    ap.add_argument("--refine-tol", type=float, default=1e-11)
     return ap.parse_args()
 
 
@@ -127,26 +137,39 @@ def main():
             dkw[k] = torch.as_tensor(v).to(dev)
         else:
             dkw[k] = v
-    pde = jh.PDEProblem(dom, **dkw)
     g_dev = torch.as_tensor(g).to(dev)
-
-    def step():
-        jh.build_solver(pde)
-        return jh.solve(pde, g_dev)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        u = step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms = (t1 - t0) / args.steps * 1e3
     dof = dom.n_leaves * dom.p ** (2 if dom.bool_2D else 3)
+    paths = ["mixed", "fp64"] if args.precision == "both" else [args.precision]
 
+    def timed(precision):
+        pde = jh.PDEProblem(dom, **dkw)
+
+        def step():
+            if precision == "mixed":
+                jh.build_solver(pde, precision="mixed", fp64_top_levels=args.fp64_top_levels)
+                return jh.solve(pde, g_dev, refine_tol=args.refine_tol)
+            jh.build_solver(pde)
+            return jh.solve(pde, g_dev)
+
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            u = step()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / args.steps * 1e3
+        return pde, u, ms
+
+    res = {}
+    for prec in paths:
+        pde, u, ms = timed(prec)
+        res[prec] = dict(pde=pde, u=u, ms=ms, hist=list(getattr(pde, "refine_history", [])))
+    head = res[paths[0]]
+    ms = head["ms"]
+    is_c = args.config == "C"
     out = {
-        "metric": "build+solve DOF/s (FP64 GPU path; mixed-precision path not yet built)",
+        "metric": "build+solve DOF/s, mixed vs FP64 GPU at <=1e-8 rel err; adjoint grads/s",
         "value": dof / (ms / 1e3),
         "unit": "DOF/s",
         "n_gpus": 1,
@@ -156,16 +179,36 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c128" if args.config == "C" else "f64",
+        "dtype": ("c64+c128" if is_c else "f32+f64") if paths[0] == "mixed" else ("c128" if is_c else "f64"),
         "data": "synthetic (seeded coefficient fields and boundary data of the config)",
-        "config": desc,
+        "config": dict(desc, path=paths[0], fp64_top_levels=args.fp64_top_levels if paths[0] == "mixed" else None),
     }
+    if "mixed" in res:
+        out["refine"] = {"tol": args.refine_tol, "iterations": len(res["mixed"]["hist"]),
+                         "history": res["mixed"]["hist"]}
+    if "fp64" in res and "mixed" in res:
+        um, u64 = res["mixed"]["u"], res["fp64"]["u"]
+        rel = float(torch.linalg.vector_norm(um - u64) / torch.linalg.vector_norm(u64))
+        out["fp64_path"] = {"ms_per_step": res["fp64"]["ms"], "value": dof / (res["fp64"]["ms"] / 1e3)}
+        out["speedup_vs_fp64"] = res["fp64"]["ms"] / res["mixed"]["ms"]
+        out["rel_err_vs_fp64"] = rel
     if not args.profile_only:
         # per-stage breakdown and dominant-kernel roofline from one instrumented step
-        prof = profiling.instrumented_step(pde, g_dev)
+        pde = head["pde"]
+        if paths[0] == "mixed":
+            prof = profiling.instrumented_step(
+                pde, g_dev,
+                build=lambda p: jh.build_solver(p, precision="mixed", fp64_top_levels=args.fp64_top_levels),
+                solve=lambda p, gg: jh.solve(p, gg, refine_tol=args.refine_tol))
+        else:
+            prof = profiling.instrumented_step(pde, g_dev)
         out["stages_ms"] = prof["stages_ms"]
         out["roofline"] = prof["roofline"]
         out["kernels"] = prof["kernels"]
+        if "fp64" in res and paths[0] != "fp64":
+            p64 = profiling.instrumented_step(res["fp64"]["pde"], g_dev)
+            out["fp64_path"]["stages_ms"] = p64["stages_ms"]
+            out["fp64_path"]["roofline"] = p64["roofline"]
     if not args.no_cpu_baseline and not args.profile_only:
         out["cpu_baseline"] = cpu_baseline(args.config)
     print(json.dumps(out), flush=True)

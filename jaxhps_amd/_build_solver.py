@@ -9,28 +9,41 @@ iterative refinement and the adjoint without refactorising.
 from __future__ import annotations
 
 from ._lib import HPS_KIND_DTN, HPS_KIND_ITI, raise_on_info
+from ._stages import cast
 from .local_solve import _leaf
 from .merge import _levels
 from ._tables import quad_merge_tables
 
 
-def _level_dtypes(n_levels, leaf_dtype, precision, fp64_top_levels):
+def _level_dtypes(n_levels, precision, hi, lo, fp64_top_levels):
     if precision == "fp64":
         return None
-    from ._refine import HIGH
-
-    hi = HIGH[leaf_dtype]
-    return [hi if lvl >= n_levels - fp64_top_levels else leaf_dtype for lvl in range(n_levels)]
+    return [hi if lvl >= n_levels - fp64_top_levels else lo for lvl in range(n_levels)]
 
 
-def _compute_dtypes(pde_problem, precision):
+def _compute_dtypes(pde_problem, precision, low_leaf=None):
+    """(leaf factor dtype, leaf output storage dtype, FP64 dtype, low dtype).
+
+    The mixed build factors ItI leaves in complex128 by default: a complex64
+    copy of the leaf operator cannot represent the k^2 term next to the
+    O((p^2/h)^2) differentiation entries (a relative error of about
+    eps_32 * (p^2/h)^2 / k^2 in the local wavenumber, which the global
+    Helmholtz problem amplifies by its phase sensitivity), and refinement
+    then fails to contract on fine trees.  Merges have no such cancellation
+    and run in complex64.  DtN leaves (elliptic, no oscillation) are
+    factored in FP32.  low_leaf overrides the choice."""
     from .local_solve import _compute_dtype
     from ._refine import LOW
 
     if precision not in ("fp64", "mixed"):
         raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
-    dt = _compute_dtype(pde_problem, force_complex=pde_problem.use_ItI)
-    return dt if precision == "fp64" else LOW[dt]
+    hi = _compute_dtype(pde_problem, force_complex=pde_problem.use_ItI)
+    if precision == "fp64":
+        return hi, hi, hi, hi
+    lo = LOW[hi]
+    if low_leaf is None:
+        low_leaf = not pde_problem.use_ItI
+    return (lo if low_leaf else hi), lo, hi, lo
 
 
 def _merge_levels(pde_problem, T, h, return_top_T, dtypes=None):
@@ -51,39 +64,41 @@ def _merge_levels(pde_problem, T, h, return_top_T, dtypes=None):
 
 
 def build_solver(pde_problem, return_top_T: bool = False, compute_device=None, host_device=None,
-                 precision: str = "fp64", fp64_top_levels: int = 2):
+                 precision: str = "fp64", fp64_top_levels: int = 2, low_precision_leaves=None):
     """Leaf solves and merges for the whole tree; stores Y, v, S_lst and
     g_tilde_lst in ``pde_problem``.  Returns the top-level T when
     ``return_top_T``.  ``compute_device`` may name the GPU (torch device);
     ``host_device`` is accepted for signature compatibility -- the solver
     state stays in HBM.
 
-    precision="mixed" (2D uniform trees) factors the leaves and all but the
-    top ``fp64_top_levels`` merge levels in FP32 (complex64 for ItI); solve()
-    then refines its solution to FP64 accuracy."""
+    precision="mixed" (2D uniform trees) runs all but the top
+    ``fp64_top_levels`` merge levels in FP32 (complex64 for ItI) and stores
+    Y, v in that precision; DtN leaves are factored in FP32, ItI leaves in
+    complex128 unless ``low_precision_leaves`` says otherwise.  solve() then
+    refines its solution to FP64 accuracy."""
     from . import profiling
 
     dom = pde_problem.domain
     if not dom.bool_uniform:
         raise NotImplementedError("adaptive trees are outside this package's scope")
-    leaf_dtype = _compute_dtypes(pde_problem, precision)
+    leaf_dtype, store_dtype, hi, lo = _compute_dtypes(pde_problem, precision, low_precision_leaves)
     if precision == "mixed" and not dom.bool_2D:
         raise NotImplementedError("the mixed-precision build covers 2D uniform trees")
     pde_problem._precision = precision
-    dtypes = _level_dtypes(dom.L if dom.L > 0 else 0, leaf_dtype, precision, fp64_top_levels)
+    dtypes = _level_dtypes(dom.L, precision, hi, lo, fp64_top_levels)
     if pde_problem.source is None:
         if not dom.bool_2D:
             raise ValueError(
                 "Build stage for problems without source terms is only implemented for 2D uniform ItI problems."
             )
-        return _nosource_build_solver(pde_problem, return_top_T, compute_device, leaf_dtype, dtypes)
+        return _nosource_build_solver(pde_problem, return_top_T, compute_device, leaf_dtype, dtypes, store_dtype)
     kind = HPS_KIND_ITI if pde_problem.use_ItI else HPS_KIND_DTN
     with profiling.stage("leaf"):
         res, v, h = _leaf(pde_problem, kind, compute_device, dtype=leaf_dtype)
     raise_on_info(res.info, "leaf stage")
     levels, multi = _merge_levels(pde_problem, res.T, h, return_top_T, dtypes)
-    pde_problem.Y = res.Y
-    pde_problem.v = v
+    pde_problem.Y = cast(res.Y, store_dtype)
+    pde_problem.v = cast(v, store_dtype)
     pde_problem.S_lst = [r.S for r in levels]
     pde_problem.g_tilde_lst = [r.gt if multi else r.gt[..., 0] for r in levels]
     pde_problem._factors = {"leaf": res, "levels": levels}
@@ -92,7 +107,7 @@ def build_solver(pde_problem, return_top_T: bool = False, compute_device=None, h
     return None
 
 
-def _nosource_build_solver(pde_problem, return_top_T, compute_device, dtype=None, dtypes=None):
+def _nosource_build_solver(pde_problem, return_top_T, compute_device, dtype=None, dtypes=None, store_dtype=None):
     """No-source build (``_build_solver.py:261-331``): leaf factors, S_lst and
     the merge factors; sources are supplied later to solve(..., source=...)."""
     from . import profiling
@@ -108,7 +123,7 @@ def _nosource_build_solver(pde_problem, return_top_T, compute_device, dtype=None
         res = leaf_solve(pde_problem, kind, dtype, device, nsrc_zero=True)
     raise_on_info(res.info, "leaf stage")
     levels, _ = _merge_levels(pde_problem, res.T, None, return_top_T, dtypes)
-    pde_problem.Y = res.Y
+    pde_problem.Y = cast(res.Y, store_dtype or res.Y.dtype)
     pde_problem.v = None
     pde_problem.S_lst = [r.S for r in levels]
     pde_problem.g_tilde_lst = []

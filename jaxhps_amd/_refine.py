@@ -34,7 +34,7 @@ import warnings
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, profiling
 from ._lib import HPS_KIND_DTN, check, dtype_code, load_library, ptr
 from ._neighbors import gauss_neighbors_2D
 from ._stages import cast, leaf_apply_inverse, to_device
@@ -215,17 +215,25 @@ def refine_solve(pde, boundary_data, u0, source, g_leaf0=None, tol=1e-11, max_it
     converged = False
     for it in range(max_iter):
         if iti:
-            res, rn2 = residual_iti(ops, u, f, g_root, lo)
-            v, g_lst = _up(res, pde, rhs_full=True)
-            du, _ = down_levels(torch.zeros_like(g_root), pde.S_lst, g_lst, pde.Y, v)
-            dn2, un2 = refine_update(u, du)
+            with profiling.stage("refine_residual"):
+                res, rn2 = residual_iti(ops, u, f, g_root, lo)
+            with profiling.stage("refine_up"):
+                v, g_lst = _up(res, pde, rhs_full=True)
+            with profiling.stage("refine_down"):
+                du, _ = down_levels(torch.zeros_like(g_root), pde.S_lst, g_lst, pde.Y, v)
+            with profiling.stage("refine_update"):
+                dn2, un2 = refine_update(u, du)
         else:
-            res, hinj, groot, rn2 = residual_dtn(ops, u, g_leaf, f, g_root, lo)
-            v, g_lst = _up(res, pde, rhs_full=True, hinj=hinj)
-            du, dg = down_levels(groot, pde.S_lst, g_lst, pde.Y, v)
-            dn2, un2 = refine_update(u, du)
-            refine_update(g_leaf, dg)
-            set_boundary_rows(ops, u, g_leaf)
+            with profiling.stage("refine_residual"):
+                res, hinj, groot, rn2 = residual_dtn(ops, u, g_leaf, f, g_root, lo)
+            with profiling.stage("refine_up"):
+                v, g_lst = _up(res, pde, rhs_full=True, hinj=hinj)
+            with profiling.stage("refine_down"):
+                du, dg = down_levels(groot, pde.S_lst, g_lst, pde.Y, v)
+            with profiling.stage("refine_update"):
+                dn2, un2 = refine_update(u, du)
+                refine_update(g_leaf, dg)
+                set_boundary_rows(ops, u, g_leaf)
         dn, un, rn = (float(x) for x in torch.stack([dn2, un2, rn2]).sqrt().cpu())
         hist.append({"iter": it + 1, "residual": rn, "rel_update": dn / max(un, 1e-300)})
         if dn <= tol * un:

@@ -4,7 +4,9 @@
 //   2x2 arrangement, each wave owning a 32x32 sub-tile as 2x2 MFMA 16x16x4
 //   blocks (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32).  Complex types
 //   run 4 real MFMAs per k-step on planar (re/im split) LDS tiles.
-// * gemm_skinny_kernel: N <= 16 (down-pass / leaf-apply GEMV shapes).  One
+// * N <= 16 (several right-hand sides): the same kernel with the four waves
+//   stacked along M, one 16-wide MFMA column block.
+// * gemm_skinny_kernel: N <= 2 (down-pass / leaf-apply GEMV shapes).  One
 //   wave per output row, lanes stride the K dimension with coalesced loads
 //   of A's row, then a wave reduction.  These shapes are HBM-bound on A.
 #include <string>
@@ -21,16 +23,21 @@ hipError_t gemm(const GemmArgs<T>& a, hipStream_t s) {
   const double flops = 2.0 * a.M * a.N * (double)a.K * a.batch * flop_mult<T>();
   const double bytes = sizeof(T) * ((double)a.M * a.K * (a.sA ? a.batch : 1) + (double)a.K * a.N * (a.sB ? a.batch : 1) +
                                     (double)a.M * a.N * a.batch * (a.beta != 0.0 ? 2 : 1));
-  if (a.N <= 16) {
+  if (a.N <= 2) {
     static const std::string nm = std::string("gemm_skinny<") + tname<T>() + ">";
     prof::Scope ps(s, nm.c_str(), flops, bytes);
     dim3 grid(ceil_div(a.M, 4), gy);
     if (a.N <= 1)
       hipLaunchKernelGGL((gemm_skinny_kernel<T, 1>), grid, dim3(256), 0, s, a);
-    else if (a.N <= 4)
-      hipLaunchKernelGGL((gemm_skinny_kernel<T, 4>), grid, dim3(256), 0, s, a);
     else
-      hipLaunchKernelGGL((gemm_skinny_kernel<T, 16>), grid, dim3(256), 0, s, a);
+      hipLaunchKernelGGL((gemm_skinny_kernel<T, 2>), grid, dim3(256), 0, s, a);
+  } else if (a.N <= 16) {
+    // one 16-wide MFMA column block; the four waves stack along M (64-row tiles)
+    static const std::string nm = std::string("gemm_mfma_n16<") + tname<T>() + ">";
+    prof::Scope ps(s, nm.c_str(), flops, bytes);
+    int tiles_m = ceil_div(a.M, 64);
+    dim3 grid((unsigned)tiles_m, gy);
+    hipLaunchKernelGGL((gemm_mfma_kernel<T, 1, 1, 4, 4>), grid, dim3(256), 0, s, a, 1);
   } else {
     static const std::string nm = std::string("gemm_mfma<") + tname<T>() + ">";
     prof::Scope ps(s, nm.c_str(), flops, bytes);

@@ -36,16 +36,21 @@ template <> __device__ __forceinline__ c64 make_val<c64>(float re, float im) { r
 
 constexpr int GBK = 16;
 
-// Block tile (2*TM*16) x (2*TN*16) x 16; 4 waves in a 2x2 arrangement, each
-// owning TM x TN MFMA 16x16 blocks.  The next K-tile is fetched from global
+// Block tile (WR*TM*16) x (WC*TN*16) x 16 with WC = 4 / WR; 4 waves in a
+// WR x WC arrangement, each owning TM x TN MFMA 16x16 blocks.  WR = 2 is the
+// square layout; WR = 4 stacks the waves along M for N <= 16 (the
+// multi-right-hand-side products).  The next K-tile is fetched from global
 // memory into registers while the current one is multiplied out of LDS.
-template <typename T, int TM, int TN, int MINB = 1>
+template <typename T, int TM, int TN, int MINB = 1, int WR = 2>
 __global__ __launch_bounds__(256, MINB) void gemm_mfma_kernel(GemmArgs<T> p, int tiles_n) {
   using R = typename real_of<T>::type;
   using M = Mfma<R>;
   using acc_t = typename M::acc_t;
   constexpr bool CPLX = is_cplx<T>::value;
-  constexpr int BM = 2 * TM * 16, BN = 2 * TN * 16;
+  constexpr int WC = 4 / WR;
+  static_assert(WR * WC == 4, "four waves per block");
+  constexpr int BM = WR * TM * 16, BN = WC * TN * 16;
+  static_assert((BM * GBK) % 256 == 0 && (GBK * BN) % 256 == 0, "tile loads must cover the block evenly");
   constexpr int LA = BM * GBK / 256, LB = GBK * BN / 256;
   __shared__ R As_re[GBK][BM + 1];
   __shared__ R Bs_re[GBK][BN + 1];
@@ -55,7 +60,7 @@ __global__ __launch_bounds__(256, MINB) void gemm_mfma_kernel(GemmArgs<T> p, int
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
-  const int wr = wave >> 1, wc = wave & 1;
+  const int wr = wave / WC, wc = wave % WC;
   const int tm = blockIdx.x / tiles_n, tn = blockIdx.x % tiles_n;
   const int m0 = tm * BM, n0 = tn * BN;
   const int li = lane & 15, lk = lane >> 4;
@@ -111,13 +116,13 @@ __global__ __launch_bounds__(256, MINB) void gemm_mfma_kernel(GemmArgs<T> p, int
         R a_re[TM], b_re[TN], a_im[TM], b_im[TN];
 #pragma unroll
         for (int t = 0; t < TM; ++t) {
-          a_re[t] = As_re[kk + lk][wr * (BM / 2) + t * 16 + li];
-          if constexpr (CPLX) a_im[t] = As_im[kk + lk][wr * (BM / 2) + t * 16 + li];
+          a_re[t] = As_re[kk + lk][wr * (BM / WR) + t * 16 + li];
+          if constexpr (CPLX) a_im[t] = As_im[kk + lk][wr * (BM / WR) + t * 16 + li];
         }
 #pragma unroll
         for (int t = 0; t < TN; ++t) {
-          b_re[t] = Bs_re[kk + lk][wc * (BN / 2) + t * 16 + li];
-          if constexpr (CPLX) b_im[t] = Bs_im[kk + lk][wc * (BN / 2) + t * 16 + li];
+          b_re[t] = Bs_re[kk + lk][wc * (BN / WC) + t * 16 + li];
+          if constexpr (CPLX) b_im[t] = Bs_im[kk + lk][wc * (BN / WC) + t * 16 + li];
         }
 #pragma unroll
         for (int i = 0; i < TM; ++i)
@@ -142,8 +147,8 @@ __global__ __launch_bounds__(256, MINB) void gemm_mfma_kernel(GemmArgs<T> p, int
       for (int j = 0; j < TN; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          int row = m0 + wr * (BM / 2) + i * 16 + M::out_row(lk, r);
-          int col = n0 + wc * (BN / 2) + j * 16 + li;
+          int row = m0 + wr * (BM / WR) + i * 16 + M::out_row(lk, r);
+          int col = n0 + wc * (BN / WC) + j * 16 + li;
           if (row < p.M && col < p.N) {
             T v = make_val<T>(acc_re[i][j][r], CPLX ? acc_im[i][j][r] : R(0));
             v = s_scale(v, p.alpha);

@@ -26,7 +26,7 @@ def _ref64(t):
 @pytest.mark.parametrize(
     "batch,M,N,K",
     [(1, 16, 16, 4), (3, 64, 64, 16), (5, 56, 56, 256), (2, 130, 70, 33), (7, 196, 72, 60), (2, 100, 1, 300),
-     (3, 77, 16, 45), (4, 33, 3, 7), (1, 300, 257, 129)],
+     (3, 77, 16, 45), (4, 33, 3, 7), (1, 300, 257, 129), (6, 200, 2, 70), (3, 150, 9, 256)],
 )
 def test_gemm_matches_reference(gpu, dtype, batch, M, N, K):
     from jaxhps_amd import _lib
