@@ -41,6 +41,9 @@ def parse():
                     help="which path(s) to time; the value is the mixed path's unless only fp64 runs")
     ap.add_argument("--fp64-top-levels", type=int, default=2)
     ap.add_argument("--refine-tol", type=float, default=1e-11)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of known size (hps_convert, complex128 -> complex128, "
+                         "256 MiB, 10 times): the reference for scaling HBM byte counters")
     return ap.parse_args()
 
 
@@ -126,6 +129,19 @@ def main():
 
     import jaxhps_amd as jh
     from jaxhps_amd import _lib, profiling
+
+    if args.copy_calibration:
+        x = torch.zeros(2**24, dtype=torch.complex128, device="cuda:0")  # 256 MiB
+        lib = _lib.load_library()
+        y = torch.empty_like(x)
+        for _ in range(10):
+            rc = lib.hps_convert(_lib.handle(), 2, 2, x.numel(), _lib.ptr(x), _lib.ptr(y))
+            _lib.check(rc, _lib.handle(), "hps_convert")
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration": {"kernel": "convert_kernel<c128,c128>", "launches": 10,
+                                               "read_bytes_per_launch": x.numel() * 16,
+                                               "write_bytes_per_launch": x.numel() * 16}}), flush=True)
+        return
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)

@@ -33,18 +33,20 @@ hipError_t gemm(const GemmArgs<T>& a, hipStream_t s) {
       hipLaunchKernelGGL((gemm_skinny_kernel<T, 2>), grid, dim3(256), 0, s, a);
   } else if (a.N <= 16) {
     // one 16-wide MFMA column block; the four waves stack along M (64-row tiles)
-    static const std::string nm = std::string("gemm_mfma_n16<") + tname<T>() + ">";
+    static const std::string nm = std::string("gemm_mfma_64x16<") + tname<T>() + ">";
     prof::Scope ps(s, nm.c_str(), flops, bytes);
     int tiles_m = ceil_div(a.M, 64);
     dim3 grid((unsigned)tiles_m, gy);
     hipLaunchKernelGGL((gemm_mfma_kernel<T, 1, 1, 4, 4>), grid, dim3(256), 0, s, a, 1);
   } else {
-    static const std::string nm = std::string("gemm_mfma<") + tname<T>() + ">";
-    prof::Scope ps(s, nm.c_str(), flops, bytes);
+    static const std::string nm64 = std::string("gemm_mfma_64x64<") + tname<T>() + ">";
+    static const std::string nm32 = std::string("gemm_mfma_64x32<") + tname<T>() + ">";
+    const bool short_k = a.K <= 64 && a.N <= 256;
+    prof::Scope ps(s, (short_k ? nm32 : nm64).c_str(), flops, bytes);
     // Tile choice measured on MI355X over the solver's shapes: 64x64 tiles
     // with 3 workgroups per CU for general shapes; 64x32 tiles with 2 per CU
     // for the short-K batched updates of the leaf LU and solves.
-    if (a.K <= 64 && a.N <= 256) {
+    if (short_k) {
       int tiles_n = ceil_div(a.N, 32), tiles_m = ceil_div(a.M, 64);
       dim3 grid((unsigned)(tiles_m * tiles_n), gy);
       hipLaunchKernelGGL((gemm_mfma_kernel<T, 2, 1, 2>), grid, dim3(256), 0, s, a, tiles_n);
