@@ -53,7 +53,7 @@ def _merge_levels(pde_problem, T, h, return_top_T, dtypes=None):
         table_fn = lambda nec: quad_merge_tables(nec // 4, kind)  # noqa: E731
         nc = 4
     else:
-        from ._tables3d import oct_merge_tables
+        from ._tables import oct_merge_tables
 
         table_fn = lambda nec: oct_merge_tables(nec // 6)  # noqa: E731
         nc = 8
@@ -101,6 +101,10 @@ def build_solver(pde_problem, return_top_T: bool = False, compute_device=None, h
     pde_problem.v = cast(v, store_dtype)
     pde_problem.S_lst = [r.S for r in levels]
     pde_problem.g_tilde_lst = [r.gt if multi else r.gt[..., 0] for r in levels]
+    if not dom.bool_2D:
+        # the 3D root level is stored without its batch axis (merge/_uniform_3D_DtN.py:115-117)
+        pde_problem.S_lst[-1] = pde_problem.S_lst[-1][0]
+        pde_problem.g_tilde_lst[-1] = pde_problem.g_tilde_lst[-1][0]
     pde_problem._factors = {"leaf": res, "levels": levels}
     if return_top_T:
         return levels[-1].T[0]

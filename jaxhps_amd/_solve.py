@@ -2,7 +2,7 @@
 
 from __future__ import annotations
 
-from .down_pass import down_pass_uniform_2D_DtN, down_pass_uniform_2D_ItI
+from .down_pass import down_pass_uniform_2D_DtN, down_pass_uniform_2D_ItI, down_pass_uniform_3D_DtN
 from .up_pass import up_pass_uniform_2D_DtN, up_pass_uniform_2D_ItI
 
 
@@ -56,6 +56,7 @@ def solve(pde_problem, boundary_data, source=None, compute_device=None, host_dev
     if not dom.bool_uniform:
         raise NotImplementedError("adaptive trees are outside this package's scope")
     if not dom.bool_2D:
-        raise NotImplementedError("3D solve is not implemented yet")
-    fn = down_pass_uniform_2D_ItI if pde_problem.use_ItI else down_pass_uniform_2D_DtN
+        fn = down_pass_uniform_3D_DtN
+    else:
+        fn = down_pass_uniform_2D_ItI if pde_problem.use_ItI else down_pass_uniform_2D_DtN
     return fn(boundary_data, pde_problem.S_lst, pde_problem.g_tilde_lst, pde_problem.Y, pde_problem.v)

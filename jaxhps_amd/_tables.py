@@ -193,3 +193,36 @@ def quad_merge_tables(m: int, kind: str) -> MergeTables:
                            final_of_internal=final_of_internal, block=m)
     else:
         raise ValueError(kind)
+
+
+# ---------------------------------------------------------------------------
+# 3D 8:1 merges (merge/_uniform_3D_DtN.py:128-541, merge/_schur_complement.py:294-774)
+# ---------------------------------------------------------------------------
+# children a..h = 0..7 as in the oct subdivision (_grid_creation_3D.py:29-78):
+# a..d upper z (x-y quadrants SW, SE, NE, NW), e..h lower z.
+# faces 0..5 of a box: x-, x+, y-, y+, z-, z+ (the reference's faces 1..6).
+# Interfaces 9..20 in the reference's unknown order: (child, face) on both sides.
+_OCT_IFACES = (
+    (0, 1, 1, 0), (1, 3, 2, 2), (2, 0, 3, 1), (3, 2, 0, 3),  # 9 a|b, 10 b|c, 11 c|d, 12 d|a
+    (4, 1, 5, 0), (5, 3, 6, 2), (6, 0, 7, 1), (7, 2, 4, 3),  # 13 e|f, 14 f|g, 15 g|h, 16 h|e
+    (0, 4, 4, 5), (1, 4, 5, 5), (2, 4, 6, 5), (3, 4, 7, 5),  # 17 a|e, 18 b|f, 19 c|g, 20 d|h
+)
+# merged face k = the children's face-k blocks in this order (the reference's
+# get_rearrange_indices, :443-541): the quad order of the face's two coordinates
+_OCT_EXT = ((4, 7, 3, 0), (5, 6, 2, 1), (4, 5, 1, 0), (7, 6, 2, 3), (4, 5, 6, 7), (0, 1, 2, 3))
+
+
+@functools.lru_cache(maxsize=None)
+def oct_merge_tables(n: int) -> MergeTables:
+    """Tables for a 3D 8:1 DtN merge of children with n Gauss nodes per face.
+
+    Both children see a shared face in the same (first, second coordinate)
+    order, so interface node j is node j of each child's face block -- no
+    flips in 3D."""
+    nec = 6 * n
+    nodes = [((c1, f1 * n + j), (c2, f2 * n + j)) for (c1, f1, c2, f2) in _OCT_IFACES for j in range(n)]
+    rows = _pack(nodes)
+    ext = [(c, k * n + j) for k, kids in enumerate(_OCT_EXT) for c in kids for j in range(n)]
+    ext_arr = _pack([[e] for e in ext])
+    src = _child_src(8, nec, ext, [list(p) for p in nodes])
+    return MergeTables("dtn", 8, nec, rows, rows.copy(), ext_arr, src, block=n)

@@ -11,13 +11,14 @@ from typing import List
 import torch
 
 from ._stages import cast, down_level, leaf_apply, to_device
-from ._tables import quad_merge_tables
+from ._tables import oct_merge_tables, quad_merge_tables
 
 
-def down_levels(g_root, S_lst, g_tilde_lst, Y_arr, v_arr, return_leaf_g=False):
+def down_levels(g_root, S_lst, g_tilde_lst, Y_arr, v_arr, table_fn=None):
     """Root data (ne_root, nsrc) -> (u (nl, n, nsrc) or None, leaf boundary data
     (nl, nec, nsrc)).  Each level runs in the precision of its S; the leaf
-    apply in the precision of Y.  g_tilde_lst may be empty (no-source build)."""
+    apply in the precision of Y.  g_tilde_lst may be empty (no-source build).
+    table_fn(ne) gives a level's merge tables (2D quad tables by default)."""
     g = g_root.reshape(1, g_root.shape[0], -1)
     for lvl in range(len(S_lst) - 1, -1, -1):
         S = S_lst[lvl].contiguous()
@@ -28,7 +29,7 @@ def down_levels(g_root, S_lst, g_tilde_lst, Y_arr, v_arr, return_leaf_g=False):
             if gt.shape[-1] != g.shape[-1]:
                 gt = gt.expand(-1, -1, g.shape[-1])
             gt = cast(gt.contiguous(), S.dtype)
-        tb = _tables_for(S)
+        tb = table_fn(S.shape[-1]) if table_fn is not None else _tables_for(S)
         g = down_level(S, gt, g.contiguous(), tb)
     if Y_arr is None:
         return None, g
@@ -48,7 +49,7 @@ def _tables_for(S):
     return quad_merge_tables(ne // 8, kind)
 
 
-def _down(boundary_data, S_lst, g_tilde_lst, Y_arr, v_arr, return_leaf_g=False):
+def _down(boundary_data, S_lst, g_tilde_lst, Y_arr, v_arr, return_leaf_g=False, table_fn=None):
     if isinstance(boundary_data, (list, tuple)):
         boundary_data = torch.cat([to_device(b, S_lst[0].device, S_lst[-1].dtype) for b in boundary_data])
     S0 = S_lst[-1]
@@ -60,7 +61,7 @@ def _down(boundary_data, S_lst, g_tilde_lst, Y_arr, v_arr, return_leaf_g=False):
     g = bd.reshape(bd.shape[0], -1)
     if not multi and g.shape[-1] != 1:
         multi = True
-    u, g_leaf = down_levels(g, S_lst, g_tilde_lst, Y_arr, v_arr)
+    u, g_leaf = down_levels(g, S_lst, g_tilde_lst, Y_arr, v_arr, table_fn=table_fn)
     out = g_leaf if u is None else u
     out = out if multi else out[..., 0]
     if return_leaf_g:
@@ -78,3 +79,16 @@ def down_pass_uniform_2D_ItI(boundary_data, S_lst, g_tilde_lst, Y_arr, v_arr, de
     """Incoming impedance data on the root boundary -> leaf solutions
     (``down_pass/_uniform_2D_ItI.py:8-118``)."""
     return _down(boundary_data, S_lst, g_tilde_lst, Y_arr, v_arr)
+
+
+def down_pass_uniform_3D_DtN(boundary_data, S_lst, g_tilde_lst, Y_arr, v_arr, device=None, host_device=None):
+    """Dirichlet data on the root boundary (faces 1..6) -> leaf solutions
+    (``down_pass/_uniform_3D_DtN.py:8-113``).  The root S and g_tilde come
+    without a batch axis, as merge_stage_uniform_3D_DtN stores them."""
+    S_lst = list(S_lst)
+    g_tilde_lst = list(g_tilde_lst)
+    if S_lst and S_lst[-1].ndim == 2:
+        S_lst[-1] = S_lst[-1].unsqueeze(0)
+        if g_tilde_lst:
+            g_tilde_lst[-1] = g_tilde_lst[-1].unsqueeze(0)
+    return _down(boundary_data, S_lst, g_tilde_lst, Y_arr, v_arr, table_fn=lambda ne: oct_merge_tables(ne // 24))

@@ -13,7 +13,7 @@ import torch
 
 from ._lib import raise_on_info
 from ._stages import cast, merge_level
-from ._tables import quad_merge_tables
+from ._tables import oct_merge_tables, quad_merge_tables
 
 
 def _levels(T_arr, h_arr, l, kind, nc, table_fn, keep_B=False, root_T=True, dtypes=None):
@@ -104,3 +104,20 @@ def merge_stage_uniform_2D_ItI(
         top = levels[-1]
         out = tuple(out) + ((top.h[0] if multi else top.h[0, :, 0]),)
     return out
+
+
+def merge_stage_uniform_3D_DtN(T_arr, h_arr, l: int, device=None, host_device=None, return_T: bool = False):
+    """3D DtN 8:1 merges (``merge/_uniform_3D_DtN.py:12-124``).
+
+    Returns S_lst, g_tilde_lst (and the top T with return_T); interface
+    unknowns in the reference's order 9..20, exterior faces 1..6 with the
+    children's face blocks in the reference's order.  The root level is
+    stored without a batch axis, as the reference does (:115-117)."""
+    levels, multi = _levels(T_arr, h_arr, l, "dtn", 8, lambda nec: oct_merge_tables(nec // 6), root_T=return_T)
+    out = _pack_outputs(levels, multi, return_T, "merge_stage_uniform_3D_DtN")
+    S_lst, g_lst = list(out[0]), list(out[1])
+    S_lst[-1] = S_lst[-1][0]
+    g_lst[-1] = g_lst[-1][0]
+    if return_T:
+        return S_lst, g_lst, out[2]
+    return S_lst, g_lst

@@ -23,10 +23,11 @@ Pinning (what checks each function beyond re-typing it):
   atol 1e-12 (polynomials) / 1e-8 (Helmholtz); global_system at 1e-11.
 * hps2d up pass / no-source: equality with the with-source path, as the
   reference's ``test_nosource_accuracy.py`` checks.
-* hps3d: manufactured polynomial solutions (exact up to rounding) and the
-  3D global system on tiny inputs.
+* hps3d: leaf operators exact on polynomials (derivatives, P, Q); a
+  manufactured cubic solution reproduced to rounding through the oct merges
+  and down pass; the 3D global system (coordinates only) on tiny inputs.
 """
 
-from . import spectral, hps2d, global_system  # noqa: F401
+from . import spectral, hps2d, hps3d, global_system  # noqa: F401
 
-__all__ = ["spectral", "hps2d", "global_system"]
+__all__ = ["spectral", "hps2d", "hps3d", "global_system"]
