@@ -33,7 +33,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="C", choices=["B", "C"])
+    ap.add_argument("--config", default="C", choices=["B", "C", "D"])
     ap.add_argument("--L", type=int, default=None, help="override tree depth (testing only)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile-only", action="store_true", help="skip the CPU baseline and extra passes")
@@ -53,22 +53,28 @@ def make_workload(cfg, L=None):
     if cfg == "C":
         dom, kw, g = wl.config_C(L=L or 7)
         desc = dict(workload="2D Helmholtz ItI k=100 p=16", L=dom.L, n_leaves=dom.n_leaves, p=16, q=14, nsrc=1)
-    else:
+    elif cfg == "B":
         dom, kw, g = wl.config_B(L=L or 6)
         desc = dict(workload="2D var-coeff elliptic DtN p=16", L=dom.L, n_leaves=dom.n_leaves, p=16, q=14, nsrc=16)
+    else:
+        dom, kw, g = wl.config_D(L=L or 3)
+        desc = dict(workload="3D var-coeff Poisson DtN p=10", L=dom.L, n_leaves=dom.n_leaves, p=10, q=8, nsrc=4)
     return dom, kw, g, desc
 
 
 def cpu_baseline(cfg):
     """The FP64 oracle (numpy/LAPACK on the host cores) on a bounded sample of
-    the same workload: the same p and coefficients at L = 4 (256 leaves)."""
+    the same workload: the same p and coefficients at L = 4 (256 leaves; 3D:
+    L = 2, 64 leaves)."""
     import numpy as np
 
     from jaxhps_amd import workloads as wl
-    from oracle import hps2d
+    from oracle import hps2d, hps3d
     from oracle import spectral as osp
 
     L = 4
+    if cfg == "D":
+        return _cpu_baseline_3d()
     if cfg == "C":
         dom, kw, g = wl.config_C(L=L)
         kind = "iti"
@@ -115,6 +121,44 @@ def cpu_baseline(cfg):
     }
 
 
+def _time_bounded(run, budget_s=20.0):
+    run()
+    times = []
+    t_end = time.time() + budget_s
+    while len(times) < 3 and (time.time() < t_end or not times):
+        t0 = time.perf_counter()
+        run()
+        times.append(time.perf_counter() - t0)
+    return sorted(times)[len(times) // 2], len(times)
+
+
+def _cpu_baseline_3d():
+    from jaxhps_amd import workloads as wl
+    from oracle import hps2d, hps3d
+
+    L = 2
+    dom, kw, g = wl.config_D(L=L)
+    p, q = dom.p, dom.q
+    ops = hps3d.leaf_ops_3D(p, q, 2.0 / 2 ** (L + 1))
+    names = {"D_xx_coefficients": "D_xx", "D_yy_coefficients": "D_yy", "D_zz_coefficients": "D_zz",
+             "D_x_coefficients": "D_x", "D_y_coefficients": "D_y", "D_z_coefficients": "D_z"}
+    coeffs = {names[k]: v for k, v in kw.items() if k in names}
+    order = ("D_xx", "D_xy", "D_yy", "D_xz", "D_yz", "D_zz", "D_x", "D_y", "D_z", "I")
+
+    def run():
+        A = hps2d.assemble_diff_operator(coeffs, ops, order=order)
+        Y, T, v, h = hps2d.leaf_dtn(A, kw["source"], ops["P"], ops["Q"])
+        S, gl, _, _ = hps3d.merge_stage_3d(T, h, L)
+        return hps3d.down_pass_3d(g, S, gl, Y, v)[0]
+
+    t, n = _time_bounded(run, 30.0)
+    dof = dom.n_leaves * p**3
+    cores = len(os.sched_getaffinity(0))
+    return {"value": dof / t, "unit": "DOF/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 numpy oracle, full 3D build+solve at L={L} ({dom.n_leaves} leaves, {dof} DOF, 4 RHS), "
+                      f"median of {n} runs, BLAS threads <= {cores}"}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -156,6 +200,8 @@ def main():
     g_dev = torch.as_tensor(g).to(dev)
     dof = dom.n_leaves * dom.p ** (2 if dom.bool_2D else 3)
     paths = ["mixed", "fp64"] if args.precision == "both" else [args.precision]
+    if args.config == "D":
+        paths = ["fp64"]  # the mixed-precision build covers 2D trees
 
     def timed(precision):
         pde = jh.PDEProblem(dom, **dkw)

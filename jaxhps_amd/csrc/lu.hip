@@ -32,6 +32,8 @@ namespace hps {
 constexpr int LU_NB = 32;
 constexpr size_t LU_PANEL_LDS_MAX = 150 * 1024;
 constexpr int SWTAB = 1 + 4 * LU_NB;  // count, then up to 2*NB (row, src) int pairs
+constexpr int LU_NBO = 256;           // column block of the blocked factorisation
+constexpr int LU_BLOCKED_MIN = 768;   // matrices larger than this are factored in column blocks
 
 __global__ void fill_int_kernel(int* p, long n, int v) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -539,7 +541,7 @@ __global__ __launch_bounds__(256) void getrf_panel_rows_kernel(int n, T* A, long
 // columns per workgroup.
 template <typename T>
 __global__ __launch_bounds__(256) void getrf_swap_u12_kernel(int n, T* A, long lda, long sA, int k, int nb,
-                                                             const T* dinv, const int* swtab) {
+                                                             const T* dinv, const int* swtab, int u12_end) {
   __shared__ T Li[LU_NB][LU_NB + 1];
   __shared__ T vals[2 * LU_NB][8];
   __shared__ T X[LU_NB][8];
@@ -563,16 +565,19 @@ __global__ __launch_bounds__(256) void getrf_swap_u12_kernel(int n, T* A, long l
   for (int a = rr; a < cnt; a += 32)
     if (valid) Ab[(long)tab[1 + 2 * a] * lda + c] = vals[a][cc];
   __syncthreads();
-  if (cbase >= k) {  // trailing slab (slabs never straddle k: k is a multiple of 8)
+  // U12 for the trailing columns up to u12_end (the end of the current column
+  // block); slabs never straddle k or u12_end - nb (multiples of 8)
+  if (cbase >= k && cbase + nb < u12_end) {
     const int nblk = (n + LU_NB - 1) / LU_NB;
     const T* db = dinv + ((long)b * nblk + k / LU_NB) * LU_NB * LU_NB;
     for (int idx = tid; idx < LU_NB * LU_NB; idx += 256) {
       int r = idx / LU_NB, q = idx - r * LU_NB;
       Li[r][q] = q < r ? db[idx] : (q == r ? s_one<T>() : s_zero<T>());
     }
-    X[rr][cc] = (valid && rr < nb) ? Ab[(long)(k + rr) * lda + c] : s_zero<T>();
+    const bool in_u12 = valid && c < u12_end;
+    X[rr][cc] = (in_u12 && rr < nb) ? Ab[(long)(k + rr) * lda + c] : s_zero<T>();
     __syncthreads();
-    if (valid && rr < nb) {
+    if (in_u12 && rr < nb) {
       T acc = s_zero<T>();
       for (int q = 0; q <= rr; ++q) acc = s_add(acc, s_mul(Li[rr][q], X[q][cc]));
       Ab[(long)(k + rr) * lda + c] = acc;
@@ -632,66 +637,6 @@ size_t getrf_scratch_bytes(int n, int batch) {
   return ((size_t)batch * n + (size_t)batch * SWTAB) * sizeof(int);
 }
 
-template <typename T>
-hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, T* dinv, int* scratch, int* info, int batch,
-                    hipStream_t s) {
-  if (n <= 0 || batch <= 0) return hipSuccess;
-  int* ipiv = scratch;
-  int* swtab = scratch + (long)batch * n;
-  hipLaunchKernelGGL(fill_int_kernel, dim3(ceil_div(batch, 256)), dim3(256), 0, s, info, (long)batch, 0);
-  for (int k = 0; k < n; k += LU_NB) {
-    int nb = n - k < LU_NB ? n - k : LU_NB;
-    hipError_t e = launch_panel<T>(n, A, lda, sA, k, nb, ipiv, info, dinv, swtab, batch, s);
-    if (e != hipSuccess) return e;
-    int others = n - nb;
-    if (others > 0) {
-      static const std::string nm = std::string("getrf_swap_u12<") + tname<T>() + ">";
-      prof::Scope ps(s, nm.c_str(), flop_mult<T>() * batch * (double)(n - k - nb) * nb * nb,
-                     3.0 * sizeof(T) * (double)nb * others * batch);
-      dim3 grid(ceil_div(others, 8), batch);
-      hipLaunchKernelGGL((getrf_swap_u12_kernel<T>), grid, dim3(256), 0, s, n, A, lda, sA, k, nb, dinv, swtab);
-    }
-    int rem = n - k - nb;
-    if (rem > 0) {
-      GemmArgs<T> g;
-      g.M = rem;
-      g.N = rem;
-      g.K = nb;
-      g.alpha = -1.0;
-      g.beta = 1.0;
-      g.A = A + (long)(k + nb) * lda + k;
-      g.lda = lda;
-      g.sA = sA;
-      g.B = A + (long)k * lda + k + nb;
-      g.ldb = lda;
-      g.sB = sA;
-      g.C = A + (long)(k + nb) * lda + k + nb;
-      g.ldc = lda;
-      g.sC = sA;
-      g.D = const_cast<T*>(g.C);
-      g.ldd = lda;
-      g.sD = sA;
-      g.batch = batch;
-      e = gemm<T>(g, s);
-      if (e != hipSuccess) return e;
-    }
-  }
-  hipLaunchKernelGGL(perm_from_ipiv_kernel, dim3(batch), dim3(256), (size_t)n * sizeof(int), s, n, ipiv, perm);
-  return hipGetLastError();
-}
-
-// X[b][i][c] = R[b][perm[b][i]][c]
-template <typename T>
-__global__ void gather_rows_kernel(int n, int nrhs, const int* perm, const T* R, long ldr, long sR, T* X, long ldx,
-                                   long sX) {
-  const int b = blockIdx.z;
-  const int i = blockIdx.y;
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nrhs) return;
-  int src = perm[(long)b * n + i];
-  X[(long)b * sX + (long)i * ldx + c] = R[(long)b * sR + (long)src * ldr + c];
-}
-
 // X[k:k+nb, :] = tri(Dinv_k) X[k:k+nb, :] in place; tri = unit lower (forward)
 // or upper (backward).  256 threads = 32 rows x 8 columns per workgroup.
 template <typename T, bool UPPER>
@@ -725,6 +670,97 @@ __global__ __launch_bounds__(256) void trsm_inv_kernel(int n, int nrhs, const T*
 }
 
 template <typename T>
+hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, T* dinv, int* scratch, int* info, int batch,
+                    hipStream_t s) {
+  if (n <= 0 || batch <= 0) return hipSuccess;
+  int* ipiv = scratch;
+  int* swtab = scratch + (long)batch * n;
+  hipLaunchKernelGGL(fill_int_kernel, dim3(ceil_div(batch, 256)), dim3(256), 0, s, info, (long)batch, 0);
+  auto gemm_update = [&](int M, int N, int K, const T* Ap, const T* Bp, T* Cp) -> hipError_t {
+    if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
+    GemmArgs<T> g;
+    g.M = M;
+    g.N = N;
+    g.K = K;
+    g.alpha = -1.0;
+    g.beta = 1.0;
+    g.A = Ap;
+    g.lda = lda;
+    g.sA = sA;
+    g.B = Bp;
+    g.ldb = lda;
+    g.sB = sA;
+    g.C = Cp;
+    g.ldc = lda;
+    g.sC = sA;
+    g.D = Cp;
+    g.ldd = lda;
+    g.sD = sA;
+    g.batch = batch;
+    return gemm<T>(g, s);
+  };
+  // Large matrices are factored in column blocks of LU_NBO: inside a block the
+  // 32-wide panels update only the block's columns; the block row's U12 and the
+  // trailing update then run once per block, the latter as a GEMM with K = LU_NBO.
+  const int nbo = n > LU_BLOCKED_MIN ? LU_NBO : n;
+  for (int k0 = 0; k0 < n; k0 += nbo) {
+    const int bend = n - k0 < nbo ? n : k0 + nbo;
+    for (int k = k0; k < bend; k += LU_NB) {
+      const int nb = bend - k < LU_NB ? bend - k : LU_NB;
+      hipError_t e = launch_panel<T>(n, A, lda, sA, k, nb, ipiv, info, dinv, swtab, batch, s);
+      if (e != hipSuccess) return e;
+      const int others = n - nb;
+      if (others > 0) {
+        static const std::string nm = std::string("getrf_swap_u12<") + tname<T>() + ">";
+        prof::Scope ps(s, nm.c_str(), flop_mult<T>() * batch * (double)(bend - k - nb) * nb * nb,
+                       3.0 * sizeof(T) * (double)nb * others * batch);
+        dim3 grid(ceil_div(others, 8), batch);
+        hipLaunchKernelGGL((getrf_swap_u12_kernel<T>), grid, dim3(256), 0, s, n, A, lda, sA, k, nb, dinv, swtab,
+                           bend);
+      }
+      e = gemm_update(n - k - nb, bend - k - nb, nb, A + (long)(k + nb) * lda + k, A + (long)k * lda + k + nb,
+                      A + (long)(k + nb) * lda + k + nb);
+      if (e != hipSuccess) return e;
+    }
+    if (bend < n) {
+      // U12 = inv(L11) A12 for the block row, block by block with the inverted diagonal blocks
+      const int ncols = n - bend;
+      dim3 tgrid(ceil_div(ncols, 8), batch);
+      for (int k = k0; k < bend; k += LU_NB) {
+        const int nb = bend - k < LU_NB ? bend - k : LU_NB;
+        {
+          static const std::string nm = std::string("getrf_u12<") + tname<T>() + ">";
+          prof::Scope ps(s, nm.c_str(), flop_mult<T>() * batch * (double)nb * nb * ncols,
+                         2.0 * sizeof(T) * (double)nb * ncols * batch);
+          hipLaunchKernelGGL((trsm_inv_kernel<T, false>), tgrid, dim3(256), 0, s, n, ncols, dinv, A + bend, lda, sA,
+                             k, nb);
+        }
+        hipError_t e = gemm_update(bend - k - nb, ncols, nb, A + (long)(k + nb) * lda + k, A + (long)k * lda + bend,
+                                   A + (long)(k + nb) * lda + bend);
+        if (e != hipSuccess) return e;
+      }
+      hipError_t e = gemm_update(n - bend, ncols, bend - k0, A + (long)bend * lda + k0, A + (long)k0 * lda + bend,
+                                 A + (long)bend * lda + bend);
+      if (e != hipSuccess) return e;
+    }
+  }
+  hipLaunchKernelGGL(perm_from_ipiv_kernel, dim3(batch), dim3(256), (size_t)n * sizeof(int), s, n, ipiv, perm);
+  return hipGetLastError();
+}
+
+// X[b][i][c] = R[b][perm[b][i]][c]
+template <typename T>
+__global__ void gather_rows_kernel(int n, int nrhs, const int* perm, const T* R, long ldr, long sR, T* X, long ldx,
+                                   long sX) {
+  const int b = blockIdx.z;
+  const int i = blockIdx.y;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nrhs) return;
+  int src = perm[(long)b * n + i];
+  X[(long)b * sX + (long)i * ldx + c] = R[(long)b * sR + (long)src * ldr + c];
+}
+
+template <typename T>
 hipError_t getrs_perm(int n, int nrhs, const T* LU, long lda, long sA, const int* perm, const T* dinv, const T* R,
                       long ldr, long sR, T* X, long ldx, long sX, int batch, hipStream_t s) {
   if (n <= 0 || nrhs <= 0 || batch <= 0) return hipSuccess;
@@ -736,73 +772,70 @@ hipError_t getrs_perm(int n, int nrhs, const T* LU, long lda, long sA, const int
     hipLaunchKernelGGL((gather_rows_kernel<T>), grid, dim3(64), 0, s, n, nrhs, perm + (long)b0 * n,
                        R + (long)b0 * sR, ldr, sR, X + (long)b0 * sX, ldx, sX);
   }
-  const int nblk = (n + LU_NB - 1) / LU_NB;
   dim3 tgrid(ceil_div(nrhs, 8), batch);
-  for (int k = 0; k < n; k += LU_NB) {
-    int nb = n - k < LU_NB ? n - k : LU_NB;
-    {
-      static const std::string nm = std::string("trsm_lower<") + tname<T>() + ">";
-      prof::Scope ps(s, nm.c_str(), flop_mult<T>() * batch * (double)nb * nb * nrhs,
-                     2.0 * sizeof(T) * (double)nb * nrhs * batch);
-      hipLaunchKernelGGL((trsm_inv_kernel<T, false>), tgrid, dim3(256), 0, s, n, nrhs, dinv, X, ldx, sX, k, nb);
-    }
-    int rem = n - k - nb;
-    if (rem > 0) {
-      GemmArgs<T> g;
-      g.M = rem;
-      g.N = nrhs;
-      g.K = nb;
-      g.alpha = -1.0;
-      g.beta = 1.0;
-      g.A = LU + (long)(k + nb) * lda + k;
-      g.lda = lda;
-      g.sA = sA;
-      g.B = X + (long)k * ldx;
-      g.ldb = ldx;
-      g.sB = sX;
-      g.C = X + (long)(k + nb) * ldx;
-      g.ldc = ldx;
-      g.sC = sX;
-      g.D = X + (long)(k + nb) * ldx;
-      g.ldd = ldx;
-      g.sD = sX;
-      g.batch = batch;
-      hipError_t e = gemm<T>(g, s);
+  auto gemm_update = [&](int M, int K, const T* Ap, const T* Bp, T* Cp) -> hipError_t {
+    if (M <= 0 || K <= 0) return hipSuccess;
+    GemmArgs<T> g;
+    g.M = M;
+    g.N = nrhs;
+    g.K = K;
+    g.alpha = -1.0;
+    g.beta = 1.0;
+    g.A = Ap;
+    g.lda = lda;
+    g.sA = sA;
+    g.B = Bp;
+    g.ldb = ldx;
+    g.sB = sX;
+    g.C = Cp;
+    g.ldc = ldx;
+    g.sC = sX;
+    g.D = Cp;
+    g.ldd = ldx;
+    g.sD = sX;
+    g.batch = batch;
+    return gemm<T>(g, s);
+  };
+  // Column blocks of LU_NBO for large matrices (as in getrf): the 32-row
+  // steps update only the rows of their block; the rest of X is updated once
+  // per block by a GEMM with K = LU_NBO.
+  const int nbo = n > LU_BLOCKED_MIN ? LU_NBO : n;
+  for (int k0 = 0; k0 < n; k0 += nbo) {
+    const int bend = n - k0 < nbo ? n : k0 + nbo;
+    for (int k = k0; k < bend; k += LU_NB) {
+      const int nb = bend - k < LU_NB ? bend - k : LU_NB;
+      {
+        static const std::string nm = std::string("trsm_lower<") + tname<T>() + ">";
+        prof::Scope ps(s, nm.c_str(), flop_mult<T>() * batch * (double)nb * nb * nrhs,
+                       2.0 * sizeof(T) * (double)nb * nrhs * batch);
+        hipLaunchKernelGGL((trsm_inv_kernel<T, false>), tgrid, dim3(256), 0, s, n, nrhs, dinv, X, ldx, sX, k, nb);
+      }
+      hipError_t e = gemm_update(bend - k - nb, nb, LU + (long)(k + nb) * lda + k, X + (long)k * ldx,
+                                 X + (long)(k + nb) * ldx);
       if (e != hipSuccess) return e;
     }
+    hipError_t e = gemm_update(n - bend, bend - k0, LU + (long)bend * lda + k0, X + (long)k0 * ldx,
+                               X + (long)bend * ldx);
+    if (e != hipSuccess) return e;
   }
-  for (int bi = nblk - 1; bi >= 0; --bi) {
-    int k = bi * LU_NB;
-    int nb = n - k < LU_NB ? n - k : LU_NB;
-    {
-      static const std::string nm = std::string("trsm_upper<") + tname<T>() + ">";
-      prof::Scope ps(s, nm.c_str(), flop_mult<T>() * batch * (double)nb * nb * nrhs,
-                     2.0 * sizeof(T) * (double)nb * nrhs * batch);
-      hipLaunchKernelGGL((trsm_inv_kernel<T, true>), tgrid, dim3(256), 0, s, n, nrhs, dinv, X, ldx, sX, k, nb);
-    }
-    if (k > 0) {
-      GemmArgs<T> g;
-      g.M = k;
-      g.N = nrhs;
-      g.K = nb;
-      g.alpha = -1.0;
-      g.beta = 1.0;
-      g.A = LU + k;
-      g.lda = lda;
-      g.sA = sA;
-      g.B = X + (long)k * ldx;
-      g.ldb = ldx;
-      g.sB = sX;
-      g.C = X;
-      g.ldc = ldx;
-      g.sC = sX;
-      g.D = X;
-      g.ldd = ldx;
-      g.sD = sX;
-      g.batch = batch;
-      hipError_t e = gemm<T>(g, s);
+  const int nblo = (n + nbo - 1) / nbo;
+  for (int bo = nblo - 1; bo >= 0; --bo) {
+    const int k0 = bo * nbo;
+    const int bend = n - k0 < nbo ? n : k0 + nbo;
+    const int last = ((bend - 1 - k0) / LU_NB) * LU_NB + k0;
+    for (int k = last; k >= k0; k -= LU_NB) {
+      const int nb = bend - k < LU_NB ? bend - k : LU_NB;
+      {
+        static const std::string nm = std::string("trsm_upper<") + tname<T>() + ">";
+        prof::Scope ps(s, nm.c_str(), flop_mult<T>() * batch * (double)nb * nb * nrhs,
+                       2.0 * sizeof(T) * (double)nb * nrhs * batch);
+        hipLaunchKernelGGL((trsm_inv_kernel<T, true>), tgrid, dim3(256), 0, s, n, nrhs, dinv, X, ldx, sX, k, nb);
+      }
+      hipError_t e = gemm_update(k - k0, nb, LU + (long)k0 * lda + k, X + (long)k * ldx, X + (long)k0 * ldx);
       if (e != hipSuccess) return e;
     }
+    hipError_t e = gemm_update(k0, bend - k0, LU + k0, X + (long)k0 * ldx, X);
+    if (e != hipSuccess) return e;
   }
   return hipGetLastError();
 }
