@@ -203,6 +203,43 @@ int hps_refine_update(hps_handle_t h, int dtype, int delta_dtype, long n, void* 
 /* dst = src converted elementwise (complex -> real keeps the real part). */
 int hps_convert(hps_handle_t h, int src_dtype, int dst_dtype, long n, const void* src, void* dst);
 
+/* ---- adjoint (reverse mode) through the stored factors --------------------
+ * The reference differentiates build+solve with JAX reverse mode
+ * (examples/check_autodiff_vJp.py:154-243, inverse_wave_scattering.py:209-292
+ * use it); these entry points are the hand-written adjoint of the uniform
+ * solve: the adjoint of each down-pass level, of each up-pass (merge) level
+ * and of the leaf solves, so that lambda = K^{-H} r for the global system K
+ * the build factored, and the gradient of the coefficient fields from lambda.
+ *
+ * D = alpha * A^H @ B + beta * C with A stored K x M (lda). */
+int hps_gemm_batched_conjt(hps_handle_t h, int dtype, int M, int N, int K, double alpha, const void* A, long lda,
+                           long sA, const void* B, long ldb, long sB, double beta, const void* C, long ldc, long sC,
+                           void* D, long ldd, long sD, int batch);
+/* X = A^{-H} R from hps_getrf_batched's factors (out of place). */
+size_t hps_getrs_conjt_workspace(int dtype, int n, int nrhs, int batch);
+int hps_getrs_batched_conjt(hps_handle_t h, int dtype, int n, int nrhs, const void* LU, long lda, long sA,
+                            const int* perm, const void* dinv, const void* R, long ldr, long sR, void* X, long ldx,
+                            long sX, int batch, void* ws, size_t ws_bytes);
+/* Adjoint of hps_down_level: children's data gbar_in (nm*nc, nec, nsrc) ->
+ * gtbar_out (nm, ni, nsrc) = sum over each interface unknown's slots
+ * (cols_final table), gbar_ext_out (nm, ne, nsrc) = exterior slots + S^H gtbar. */
+int hps_adjoint_down_level(hps_handle_t h, int dtype, int nm, int nc, int nec, int ni, int ne, int nsrc, const void* S,
+                           const void* gbar_in, const int* ext, const int* cols_final, void* gbar_ext_out,
+                           void* gtbar_out);
+/* Adjoint of hps_up_level: y = gtbar + B^H hbar_out (hbar_out NULL at the
+ * root), hbar_int = -D^{-H} y (internal order via final_of_internal; ItI in
+ * the Schur form of W = I - D12 D21), children's slots from adj_src
+ * (>= 0 exterior index into hbar_out, < 0 internal row -(r+1)). */
+size_t hps_adjoint_up_workspace(int dtype, int kind, int nm, int ni, int nsrc);
+int hps_adjoint_up_level(hps_handle_t h, int dtype, int kind, int nm, int nc, int nec, int ni, int ne, int nsrc,
+                         const void* gtbar, const void* hbar_out, const void* B, const void* F_lu, const int* perm,
+                         const void* F_dinv, const void* D12, const void* D21, const int* final_of_internal,
+                         const int* adj_src, void* hbar_children, void* ws, size_t ws_bytes);
+/* grad (ncoef, nl, n) float64: -Re sum_s conj(lam) * (op_k u) on rows >= row0
+ * (the rows the coefficients enter), 0 below; R as for the residuals. */
+int hps_coef_gradient(hps_handle_t h, int dtype, int nl, int n, int row0, int nsrc, int nops, int ncoef,
+                      const int* coef_ops, const void* R, int nrow_R, const void* u, const void* lam, double* grad);
+
 /* ---- optional kernel timing ----------------------------------------------
  * When enabled, every kernel launch of the library is bracketed by HIP events
  * on its stream and attributed to a kernel class ("gemm_mfma<c128>", ...)

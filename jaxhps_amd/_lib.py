@@ -64,6 +64,13 @@ SIGNATURES = {
     "hps_residual_iti": (_I, [_P] + [_I] * 9 + [_P, _P, _P, _I] + [_P] * 7),
     "hps_residual_dtn": (_I, [_P] + [_I] * 9 + [_P, _P, _P, _I] + [_P] * 9),
     "hps_refine_update": (_I, [_P, _I, _I, _L, _P, _P, _I, _P]),
+    "hps_gemm_batched_conjt": (_I, [_P, _I, _I, _I, _I, _D, _P, _L, _L, _P, _L, _L, _D, _P, _L, _L, _P, _L, _L, _I]),
+    "hps_getrs_conjt_workspace": (_S, [_I, _I, _I, _I]),
+    "hps_getrs_batched_conjt": (_I, [_P, _I, _I, _I, _P, _L, _L, _P, _P, _P, _L, _L, _P, _L, _L, _I, _P, _S]),
+    "hps_adjoint_down_level": (_I, [_P] + [_I] * 7 + [_P] * 6),
+    "hps_adjoint_up_workspace": (_S, [_I, _I, _I, _I, _I]),
+    "hps_adjoint_up_level": (_I, [_P] + [_I] * 8 + [_P] * 12 + [_S]),
+    "hps_coef_gradient": (_I, [_P] + [_I] * 7 + [_P, _P, _I, _P, _P, _P]),
     "hps_convert": (_I, [_P, _I, _I, _L, _P, _P]),
     "hps_prof_enable": (_I, [_I]),
     "hps_prof_reset": (_I, []),

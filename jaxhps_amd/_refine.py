@@ -68,7 +68,8 @@ class RefineOps:
         self.MT = dev(M.T)
         self.P = dev(pde.P)
         self.coeffs = torch.stack([to_device(a, device, dtype) for _, a in clist]).contiguous()
-        self.nbr = torch.as_tensor(gauss_neighbors_2D(dom.L, dom.q)).to(device)
+        # leaf adjacency (2D trees; the refinement residuals need it)
+        self.nbr = torch.as_tensor(gauss_neighbors_2D(dom.L, dom.q)).to(device) if dom.bool_2D else None
         self.nb, self.nq = pde.P.shape
         self.n = dom.p**2
         self.nl = dom.n_leaves

@@ -86,7 +86,15 @@ class MergeTables:
             np.arange(self.ni, dtype=np.int32) if final_of_internal is None else final_of_internal.astype(np.int32)
         )
         self.upd = _sparse_blocks(self.ext, self.cols_final, block) if block else np.zeros((0, 4), np.int32)
+        self.adj_src = _adj_src(nc, nec, int_rows, ext)
         self._dev = {}
+
+    def adj_device(self, dev):
+        """adj_src on the device (the adjoint up pass's child-slot sources)."""
+        key = ("adj", str(dev))
+        if key not in self._dev:
+            self._dev[key] = torch.from_numpy(np.ascontiguousarray(self.adj_src, dtype=np.int32)).to(dev)
+        return self._dev[key]
 
     def device(self, dev):
         key = str(dev)
@@ -139,6 +147,23 @@ def _pack(pairs):
             out[r, 2 * a] = c
             out[r, 2 * a + 1] = l
     return out
+
+
+def _adj_src(nc, nec, int_rows, ext):
+    """For the adjoint of the up pass: each child slot feeds either one
+    merged exterior node (index >= 0) or one interface equation row r of
+    int_rows (internal order, coded -(r+1))."""
+    src = np.full(nc * nec, np.iinfo(np.int32).min, dtype=np.int64)
+    for e, entry in enumerate(ext):
+        src[entry[0] * nec + entry[1]] = e
+    for r, entry in enumerate(int_rows):
+        for a in (0, 2):
+            if entry[a] >= 0:
+                slot = entry[a] * nec + entry[a + 1]
+                assert src[slot] == np.iinfo(np.int32).min, "child slot feeds two equations"
+                src[slot] = -(r + 1)
+    assert (src != np.iinfo(np.int32).min).all(), "child slot feeds nothing"
+    return src.astype(np.int32)
 
 
 def _child_src(nc, nec, ext, unknown_slots):

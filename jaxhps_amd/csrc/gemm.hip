@@ -23,6 +23,31 @@ hipError_t gemm(const GemmArgs<T>& a, hipStream_t s) {
   const double flops = 2.0 * a.M * a.N * (double)a.K * a.batch * flop_mult<T>();
   const double bytes = sizeof(T) * ((double)a.M * a.K * (a.sA ? a.batch : 1) + (double)a.K * a.N * (a.sB ? a.batch : 1) +
                                     (double)a.M * a.N * a.batch * (a.beta != 0.0 ? 2 : 1));
+  if (a.transA) {
+    // op(A) = A^H: the adjoint products (S^H, B^H, Y^H, D12^H ... and the
+    // conjugate-transposed triangular updates)
+    if (a.N <= 2) {
+      static const std::string nm = std::string("gemv_conjt<") + tname<T>() + ">";
+      prof::Scope ps(s, nm.c_str(), flops, bytes);
+      dim3 grid(ceil_div(a.M, 256), gy);
+      if (a.N <= 1)
+        hipLaunchKernelGGL((gemv_conjt_kernel<T, 1>), grid, dim3(256), 0, s, a);
+      else
+        hipLaunchKernelGGL((gemv_conjt_kernel<T, 2>), grid, dim3(256), 0, s, a);
+    } else if (a.N <= 16) {
+      static const std::string nm = std::string("gemm_mfma_64x16_conjt<") + tname<T>() + ">";
+      prof::Scope ps(s, nm.c_str(), flops, bytes);
+      dim3 grid((unsigned)ceil_div(a.M, 64), gy);
+      hipLaunchKernelGGL((gemm_mfma_kernel<T, 1, 1, 4, 4, true>), grid, dim3(256), 0, s, a, 1);
+    } else {
+      static const std::string nm = std::string("gemm_mfma_64x64_conjt<") + tname<T>() + ">";
+      prof::Scope ps(s, nm.c_str(), flops, bytes);
+      int tiles_n = ceil_div(a.N, 64), tiles_m = ceil_div(a.M, 64);
+      dim3 grid((unsigned)(tiles_m * tiles_n), gy);
+      hipLaunchKernelGGL((gemm_mfma_kernel<T, 2, 2, 3, 2, true>), grid, dim3(256), 0, s, a, tiles_n);
+    }
+    return hipGetLastError();
+  }
   if (a.N <= 2) {
     static const std::string nm = std::string("gemm_skinny<") + tname<T>() + ">";
     prof::Scope ps(s, nm.c_str(), flops, bytes);

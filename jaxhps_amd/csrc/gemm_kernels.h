@@ -41,7 +41,13 @@ constexpr int GBK = 16;
 // square layout; WR = 4 stacks the waves along M for N <= 16 (the
 // multi-right-hand-side products).  The next K-tile is fetched from global
 // memory into registers while the current one is multiplied out of LDS.
-template <typename T, int TM, int TN, int MINB = 1, int WR = 2>
+template <typename T> __device__ __forceinline__ T s_conj(T x) { return x; }
+template <> __device__ __forceinline__ c128 s_conj<c128>(c128 x) { return c128{x.re, -x.im}; }
+template <> __device__ __forceinline__ c64 s_conj<c64>(c64 x) { return c64{x.re, -x.im}; }
+
+// TA: op(A) = A^H with A stored K x M; the tile load then walks M fastest so
+// the reads stay coalesced.
+template <typename T, int TM, int TN, int MINB = 1, int WR = 2, bool TA = false>
 __global__ __launch_bounds__(256, MINB) void gemm_mfma_kernel(GemmArgs<T> p, int tiles_n) {
   using R = typename real_of<T>::type;
   using M = Mfma<R>;
@@ -81,9 +87,15 @@ __global__ __launch_bounds__(256, MINB) void gemm_mfma_kernel(GemmArgs<T> p, int
 #pragma unroll
       for (int i = 0; i < LA; ++i) {
         int e = tid + 256 * i;
-        int r = e >> 4, c = e & 15;
-        int gr = m0 + r, gc = k0 + c;
-        ra[i] = (gr < p.M && gc < p.K) ? A[(long)gr * p.lda + gc] : s_zero<T>();
+        if constexpr (TA) {
+          int r = e % BM, c = e / BM;
+          int gr = m0 + r, gc = k0 + c;
+          ra[i] = (gr < p.M && gc < p.K) ? s_conj(A[(long)gc * p.lda + gr]) : s_zero<T>();
+        } else {
+          int r = e >> 4, c = e & 15;
+          int gr = m0 + r, gc = k0 + c;
+          ra[i] = (gr < p.M && gc < p.K) ? A[(long)gr * p.lda + gc] : s_zero<T>();
+        }
       }
 #pragma unroll
       for (int i = 0; i < LB; ++i) {
@@ -98,7 +110,7 @@ __global__ __launch_bounds__(256, MINB) void gemm_mfma_kernel(GemmArgs<T> p, int
 #pragma unroll
       for (int i = 0; i < LA; ++i) {
         int e = tid + 256 * i;
-        int r = e >> 4, c = e & 15;
+        int r = TA ? e % BM : e >> 4, c = TA ? e / BM : e & 15;
         As_re[c][r] = s_re(ra[i]);
         if constexpr (CPLX) As_im[c][r] = s_im(ra[i]);
       }
@@ -164,6 +176,37 @@ __device__ __forceinline__ R wave_sum(R v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
   return v;
+}
+
+// N <= 2 with op(A) = A^H (A stored K x M): thread m accumulates column m of
+// A against B; consecutive threads read consecutive elements of A's rows.
+template <typename T, int NMAX>
+__global__ __launch_bounds__(256) void gemv_conjt_kernel(GemmArgs<T> p) {
+  const int m = blockIdx.x * 256 + threadIdx.x;
+  if (m >= p.M) return;
+  for (int b = blockIdx.y; b < p.batch; b += gridDim.y) {
+    const T* A = p.A + (long)b * p.sA + m;
+    const T* B = p.B + (long)b * p.sB;
+    T acc[NMAX];
+#pragma unroll
+    for (int n = 0; n < NMAX; ++n) acc[n] = s_zero<T>();
+    for (int k = 0; k < p.K; ++k) {
+      const T a = s_conj(A[(long)k * p.lda]);
+#pragma unroll
+      for (int n = 0; n < NMAX; ++n)
+        if (n < p.N) acc[n] = s_add(acc[n], s_mul(a, B[(long)k * p.ldb + n]));
+    }
+    T* D = p.D + (long)b * p.sD + (long)m * p.ldd;
+    const T* C = p.C ? p.C + (long)b * p.sC + (long)m * p.ldc : nullptr;
+#pragma unroll
+    for (int n = 0; n < NMAX; ++n) {
+      if (n < p.N) {
+        T v = s_scale(acc[n], p.alpha);
+        if (p.beta != 0.0) v = s_add(v, s_scale(C[n], p.beta));
+        D[n] = v;
+      }
+    }
+  }
 }
 
 template <typename T, int NMAX>

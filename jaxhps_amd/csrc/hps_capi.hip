@@ -167,7 +167,8 @@ __global__ __launch_bounds__(256) void down_scatter_kernel(int nc, int nec, int 
     int slot = (int)(idx / nsrc), s = (int)(idx - (long)slot * nsrc);
     int src = child_src[slot];
     for (int m = blockIdx.y; m < nm; m += gridDim.y) {
-      T v = src >= 0 ? g_par[((long)m * ne + src) * nsrc + s] : g_int[((long)m * ni + (-src - 1)) * nsrc + s];
+      T v = src >= 0 ? (g_par ? g_par[((long)m * ne + src) * nsrc + s] : s_zero<T>())
+                     : g_int[((long)m * ni + (-src - 1)) * nsrc + s];
       g_out[((long)m * nc * nec + slot) * nsrc + s] = v;
     }
   }
@@ -490,6 +491,41 @@ __global__ void row_scatter_kernel(const T* src, T* dst, const int* map, int row
     int r = (int)(idx / cols), c = (int)(idx - (long)r * cols);
     for (int m = blockIdx.y; m < nm; m += gridDim.y)
       dst[((long)m * rows + map[r]) * cols + c] = src[((long)m * rows + r) * cols + c];
+  }
+}
+
+// dst[m][r][c] = src[m][map[r]][c]
+template <typename T>
+__global__ void row_gather_kernel(const T* src, T* dst, const int* map, int rows, int cols, int nm) {
+  long total = (long)rows * cols;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    int r = (int)(idx / cols), c = (int)(idx - (long)r * cols);
+    for (int m = blockIdx.y; m < nm; m += gridDim.y)
+      dst[((long)m * rows + r) * cols + c] = src[((long)m * rows + map[r]) * cols + c];
+  }
+}
+
+// grad[k][l][i] = -sum_s Re(conj(lam[l][i][s]) * x_k[l][i][s]) on the rows i >= row0 that the
+// coefficients enter (x_k = op_k u from R, or u for the identity), 0 on the rows below
+template <typename T>
+__global__ __launch_bounds__(256) void coef_grad_kernel(int nl, int n, int row0, int w, int ncoef, CoefOps ops,
+                                                        const T* R, int nrow_R, const T* u, const T* lam,
+                                                        double* grad) {
+  long total = (long)nl * n;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
+    const int l = (int)(idx / n), i = (int)(idx - (long)l * n);
+    for (int k = 0; k < ncoef; ++k) {
+      double g = 0.0;
+      if (i >= row0) {
+        const int op = ops.op[k];
+        for (int sidx = 0; sidx < w; ++sidx) {
+          const T x = op < 0 ? u[((long)l * n + i) * w + sidx] : R[((long)l * nrow_R + (long)op * n + i) * w + sidx];
+          const T lv = lam[((long)l * n + i) * w + sidx];
+          g -= (double)s_re(lv) * (double)s_re(x) + (double)s_im(lv) * (double)s_im(x);
+        }
+      }
+      grad[((long)k * nl + l) * n + i] = g;
+    }
   }
 }
 
@@ -862,3 +898,210 @@ int hps_leaf_apply(hps_handle_t h, int dtype, int nl, int n, int nq, int nsrc, c
   DISPATCH(dtype, apply_t, h, nl, n, nq, nsrc, Y, g, v, u);
 }
 
+
+
+// ---------------------------------------------------------------------------
+// adjoint (reverse mode) of the solve through the stored factors
+// ---------------------------------------------------------------------------
+template <typename T>
+static int gemm_ct_call(hps_handle_t h, int M, int N, int K, double alpha, const T* A, long lda, long sA, const T* B,
+                        long ldb, long sB, double beta, const T* C, long ldc, long sC, T* D, long ldd, long sD,
+                        int batch) {
+  GemmArgs<T> g;
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.alpha = alpha;
+  g.beta = beta;
+  g.A = A;
+  g.lda = lda;
+  g.sA = sA;
+  g.transA = 1;
+  g.B = B;
+  g.ldb = ldb;
+  g.sB = sB;
+  g.C = C;
+  g.ldc = ldc;
+  g.sC = sC;
+  g.D = D;
+  g.ldd = ldd;
+  g.sD = sD;
+  g.batch = batch;
+  HPS_CHECK_HIP(h, gemm<T>(g, h->stream));
+  return HPS_OK;
+}
+
+template <typename T>
+static int gemm_conjt_t(hps_handle_t h, int M, int N, int K, double alpha, const void* A, long lda, long sA,
+                        const void* B, long ldb, long sB, double beta, const void* C, long ldc, long sC, void* D,
+                        long ldd, long sD, int batch) {
+  return gemm_ct_call<T>(h, M, N, K, alpha, (const T*)A, lda, sA, (const T*)B, ldb, sB, beta, (const T*)C, ldc, sC,
+                         (T*)D, ldd, sD, batch);
+}
+
+int hps_gemm_batched_conjt(hps_handle_t h, int dtype, int M, int N, int K, double alpha, const void* A, long lda,
+                           long sA, const void* B, long ldb, long sB, double beta, const void* C, long ldc, long sC,
+                           void* D, long ldd, long sD, int batch) {
+  if (!h) return HPS_ERR_ARG;
+  HPS_ARG(h, M >= 0 && N >= 0 && K >= 0 && batch >= 0, "negative size");
+  HPS_ARG(h, D != nullptr && (K == 0 || (A && B)), "null matrix");
+  HPS_ARG(h, beta == 0.0 || C != nullptr, "beta != 0 needs C");
+  DISPATCH(dtype, gemm_conjt_t, h, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C, ldc, sC, D, ldd, sD, batch);
+}
+
+size_t hps_getrs_conjt_workspace(int dtype, int n, int nrhs, int batch) {
+  return align_up((size_t)batch * n * nrhs * dtype_size(dtype));
+}
+
+template <typename T>
+static int getrs_conjt_t(hps_handle_t h, int n, int nrhs, const void* LU, long lda, long sA, const int* perm,
+                         const void* dinv, const void* R, long ldr, long sR, void* X, long ldx, long sX, int batch,
+                         void* ws) {
+  HPS_CHECK_HIP(h, getrs_conjt<T>(n, nrhs, (const T*)LU, lda, sA, perm, (const T*)dinv, (const T*)R, ldr, sR, (T*)X,
+                                  ldx, sX, (T*)ws, batch, h->stream));
+  return HPS_OK;
+}
+
+int hps_getrs_batched_conjt(hps_handle_t h, int dtype, int n, int nrhs, const void* LU, long lda, long sA,
+                            const int* perm, const void* dinv, const void* R, long ldr, long sR, void* X, long ldx,
+                            long sX, int batch, void* ws, size_t ws_bytes) {
+  if (!h) return HPS_ERR_ARG;
+  HPS_ARG(h, n >= 0 && nrhs >= 0 && batch >= 0 && lda >= n, "bad size");
+  HPS_ARG(h, LU && perm && dinv && R && X && ws, "null pointer");
+  HPS_ARG(h, R != X, "the solve is out of place: R and X must differ");
+  if (ws_bytes < hps_getrs_conjt_workspace(dtype, n, nrhs, batch)) {
+    h->err = "conjugate-transposed solve workspace too small";
+    return HPS_ERR_WORKSPACE;
+  }
+  DISPATCH(dtype, getrs_conjt_t, h, n, nrhs, LU, lda, sA, perm, dinv, R, ldr, sR, X, ldx, sX, batch, ws);
+}
+
+template <typename T>
+static int adj_down_t(hps_handle_t h, int nm, int nc, int nec, int ni, int ne, int nsrc, const void* S,
+                      const void* gbar_in, const int* ext, const int* cols_final, void* gbar_ext, void* gtbar) {
+  hipStream_t s = h->stream;
+  const int w = nsrc;
+  {
+    HPS_PROF("adjoint_gather", T, 0.0, sizeof(T) * 2.0 * (double)(ni + ne) * w * nm);
+    hipLaunchKernelGGL((vec_gather_kernel<T>), grid_for((long)ni * w, nm), dim3(256), 0, s, (const T*)gbar_in, nc,
+                       nec, w, (const int4*)cols_final, ni, 1.0, (T*)gtbar, nm);
+    hipLaunchKernelGGL((vec_gather_kernel<T>), grid_for((long)ne * w, nm), dim3(256), 0, s, (const T*)gbar_in, nc,
+                       nec, w, (const int4*)ext, ne, 1.0, (T*)gbar_ext, nm);
+    HPS_CHECK_HIP(h, hipGetLastError());
+  }
+  return gemm_ct_call<T>(h, ne, w, ni, 1.0, (const T*)S, ne, (long)ni * ne, (const T*)gtbar, w, (long)ni * w, 1.0,
+                         (const T*)gbar_ext, w, (long)ne * w, (T*)gbar_ext, w, (long)ne * w, nm);
+}
+
+int hps_adjoint_down_level(hps_handle_t h, int dtype, int nm, int nc, int nec, int ni, int ne, int nsrc, const void* S,
+                           const void* gbar_in, const int* ext, const int* cols_final, void* gbar_ext_out,
+                           void* gtbar_out) {
+  if (!h) return HPS_ERR_ARG;
+  HPS_ARG(h, nm > 0 && nc > 0 && nec > 0 && ni > 0 && ne > 0 && nsrc > 0, "bad adjoint down-pass sizes");
+  HPS_ARG(h, S && gbar_in && ext && cols_final && gbar_ext_out && gtbar_out, "null pointer");
+  DISPATCH(dtype, adj_down_t, h, nm, nc, nec, ni, ne, nsrc, S, gbar_in, ext, cols_final, gbar_ext_out, gtbar_out);
+}
+
+size_t hps_adjoint_up_workspace(int dtype, int kind, int nm, int ni, int nsrc) {
+  const size_t es = dtype_size(dtype);
+  const int nf = kind == HPS_KIND_ITI ? ni / 2 : ni;
+  const size_t v = align_up((size_t)nm * ni * nsrc * es);
+  return 3 * v + hps_getrs_conjt_workspace(dtype, nf, nsrc, nm);
+}
+
+template <typename T>
+static int adj_up_t(hps_handle_t h, int kind, int nm, int nc, int nec, int ni, int ne, int nsrc, const void* gtbar,
+                    const void* hbar_out, const void* Bv, const void* F_lu, const int* perm, const void* F_dinv,
+                    const void* D12v, const void* D21v, const int* final_of_internal, const int* adj_src,
+                    void* hbar_children, void* ws) {
+  hipStream_t s = h->stream;
+  const int w = nsrc;
+  const bool iti = kind == HPS_KIND_ITI;
+  const int nf = iti ? ni / 2 : ni;
+  const long sY = (long)ni * w;
+  char* wp = (char*)ws;
+  const size_t v = align_up((size_t)nm * ni * w * sizeof(T));
+  T* yf = (T*)wp;
+  T* y = (T*)(wp + v);
+  T* x = (T*)(wp + 2 * v);
+  T* W = (T*)(wp + 3 * v);
+  // y (final order) = gtbar + B^H hbar_out
+  HPS_CHECK_HIP(h, copy2d<T>(ni, w, 1.0, (const T*)gtbar, w, sY, yf, w, sY, nm, s));
+  if (hbar_out) {
+    int rc = gemm_ct_call<T>(h, ni, w, ne, 1.0, (const T*)Bv, ni, (long)ne * ni, (const T*)hbar_out, w, (long)ne * w,
+                             1.0, yf, w, sY, yf, w, sY, nm);
+    if (rc != HPS_OK) return rc;
+  }
+  hipLaunchKernelGGL((row_gather_kernel<T>), grid_for((long)ni * w, nm), dim3(256), 0, s, (const T*)yf, y,
+                     final_of_internal, ni, w, nm);
+  HPS_CHECK_HIP(h, hipGetLastError());
+  const T* Fl = (const T*)F_lu;
+  const T* dv = (const T*)F_dinv;
+  const long sF = (long)nf * nf;
+  if (!iti) {
+    HPS_CHECK_HIP(h, getrs_conjt<T>(ni, w, Fl, ni, sF, perm, dv, y, w, sY, x, w, sY, W, nm, s));
+  } else {
+    // D^H = I + [[0, D21^H], [D12^H, 0]]:  x_F = W^{-H} (y_F - D21^H y_G),  x_G = y_G - D12^H x_F
+    T* yF = y;
+    T* yG = y + (long)nf * w;
+    int rc = gemm_ct_call<T>(h, nf, w, nf, -1.0, (const T*)D21v, nf, sF, yG, w, sY, 1.0, yF, w, sY, yF, w, sY, nm);
+    if (rc != HPS_OK) return rc;
+    HPS_CHECK_HIP(h, getrs_conjt<T>(nf, w, Fl, nf, sF, perm, dv, yF, w, sY, x, w, sY, W, nm, s));
+    rc = gemm_ct_call<T>(h, nf, w, nf, -1.0, (const T*)D12v, nf, sF, x, w, sY, 1.0, yG, w, sY, x + (long)nf * w, w, sY,
+                         nm);
+    if (rc != HPS_OK) return rc;
+  }
+  // hbar_int = -x; children's slots take hbar_int (the rows they feed) or hbar_out (exterior)
+  HPS_CHECK_HIP(h, copy2d<T>(ni, w, -1.0, x, w, sY, y, w, sY, nm, s));
+  {
+    HPS_PROF("adjoint_scatter", T, 0.0, 2.0 * sizeof(T) * (double)nc * nec * w * nm);
+    hipLaunchKernelGGL((down_scatter_kernel<T>), grid_for((long)nc * nec * w, nm), dim3(256), 0, s, nc, nec, ne, ni,
+                       w, adj_src, (const T*)hbar_out, (const T*)y, (T*)hbar_children, nm);
+    HPS_CHECK_HIP(h, hipGetLastError());
+  }
+  return HPS_OK;
+}
+
+int hps_adjoint_up_level(hps_handle_t h, int dtype, int kind, int nm, int nc, int nec, int ni, int ne, int nsrc,
+                         const void* gtbar, const void* hbar_out, const void* B, const void* F_lu, const int* perm,
+                         const void* F_dinv, const void* D12, const void* D21, const int* final_of_internal,
+                         const int* adj_src, void* hbar_children, void* ws, size_t ws_bytes) {
+  if (!h) return HPS_ERR_ARG;
+  HPS_ARG(h, kind == HPS_KIND_DTN || kind == HPS_KIND_ITI, "kind must be DtN or ItI");
+  HPS_ARG(h, nm > 0 && nc > 0 && nec > 0 && ni > 0 && ne > 0 && nsrc > 0, "bad adjoint up-pass sizes");
+  HPS_ARG(h, kind == HPS_KIND_DTN || (ni % 2 == 0 && D12 && D21), "ItI needs D12 and D21");
+  HPS_ARG(h, gtbar && F_lu && perm && F_dinv && final_of_internal && adj_src && hbar_children && ws, "null pointer");
+  HPS_ARG(h, !hbar_out || B, "hbar_out needs B");
+  if (ws_bytes < hps_adjoint_up_workspace(dtype, kind, nm, ni, nsrc)) {
+    h->err = "adjoint up workspace too small";
+    return HPS_ERR_WORKSPACE;
+  }
+  DISPATCH(dtype, adj_up_t, h, kind, nm, nc, nec, ni, ne, nsrc, gtbar, hbar_out, B, F_lu, perm, F_dinv, D12, D21,
+           final_of_internal, adj_src, hbar_children, ws);
+}
+
+template <typename T>
+static int coef_grad_t(hps_handle_t h, int nl, int n, int row0, int nsrc, int ncoef, CoefOps co, const void* R,
+                       int nrow_R, const void* u, const void* lam, double* grad) {
+  hipStream_t s = h->stream;
+  HPS_PROF("coef_gradient", T, 0.0, sizeof(T) * (double)nl * n * nsrc * (ncoef + 2) + 8.0 * ncoef * nl * n);
+  hipLaunchKernelGGL((coef_grad_kernel<T>), grid_for((long)nl * n, 1), dim3(256), 0, s, nl, n, row0, nsrc, ncoef, co,
+                     (const T*)R, nrow_R, (const T*)u, (const T*)lam, grad);
+  HPS_CHECK_HIP(h, hipGetLastError());
+  return HPS_OK;
+}
+
+int hps_coef_gradient(hps_handle_t h, int dtype, int nl, int n, int row0, int nsrc, int nops, int ncoef,
+                      const int* coef_ops, const void* R, int nrow_R, const void* u, const void* lam, double* grad) {
+  if (!h) return HPS_ERR_ARG;
+  HPS_ARG(h, nl > 0 && n > 0 && row0 >= 0 && row0 <= n && nsrc > 0 && nops >= 0, "bad gradient sizes");
+  HPS_ARG(h, ncoef >= 1 && ncoef <= 16 && coef_ops, "ncoef must be in [1, 16] with coef_ops given");
+  HPS_ARG(h, nrow_R >= nops * n, "nrow_R too small");
+  HPS_ARG(h, (nops == 0 || R) && u && lam && grad, "null pointer");
+  CoefOps co;
+  for (int k = 0; k < ncoef; ++k) {
+    HPS_ARG(h, coef_ops[k] >= -1 && coef_ops[k] < nops, "coef_ops entry out of range");
+    co.op[k] = coef_ops[k];
+  }
+  DISPATCH(dtype, coef_grad_t, h, nl, n, row0, nsrc, ncoef, co, R, nrow_R, u, lam, grad);
+}

@@ -5,9 +5,10 @@
 
 namespace hps {
 
-// D[b] = alpha * A[b] @ B[b] + beta * C[b]   (row-major, per batch element)
-// A: M x K (lda), B: K x N (ldb), C/D: M x N (ldc/ldd).  s* are batch strides
-// (0 broadcasts).  C may be null when beta == 0.  C and D may alias.
+// D[b] = alpha * op(A[b]) @ B[b] + beta * C[b]   (row-major, per batch element)
+// op(A) = A (M x K, lda) or, with transA, A^H where A is stored K x M (lda);
+// B: K x N (ldb), C/D: M x N (ldc/ldd).  s* are batch strides (0 broadcasts).
+// C may be null when beta == 0.  C and D may alias.
 template <typename T>
 struct GemmArgs {
   int M = 0, N = 0, K = 0;
@@ -21,6 +22,7 @@ struct GemmArgs {
   T* D = nullptr;
   long ldd = 0, sD = 0;
   int batch = 1;
+  int transA = 0;
 };
 
 template <typename T>
@@ -43,6 +45,12 @@ hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, T* dinv, int* scr
 template <typename T>
 hipError_t getrs_perm(int n, int nrhs, const T* LU, long lda, long sA, const int* perm, const T* dinv, const T* R,
                       long ldr, long sR, T* X, long ldx, long sX, int batch, hipStream_t s);
+
+// X = A^{-H} R from getrf_ws's factors (A^H = U^H L^H P).  W is n x nrhs
+// scratch per batch element (row stride nrhs); X, W and R must not alias.
+template <typename T>
+hipError_t getrs_conjt(int n, int nrhs, const T* LU, long lda, long sA, const int* perm, const T* dinv, const T* R,
+                       long ldr, long sR, T* X, long ldx, long sX, T* W, int batch, hipStream_t s);
 
 // dst[b][r][c] = alpha * src[b][r][c]; src == nullptr writes zeros; a batch
 // stride of 0 on src broadcasts.

@@ -29,9 +29,19 @@
 
 namespace hps {
 
+#define HPS_LU_CHECK(expr)              \
+  do {                                  \
+    hipError_t _e = (expr);             \
+    if (_e != hipSuccess) return _e;    \
+  } while (0)
+
 constexpr int LU_NB = 32;
 constexpr size_t LU_PANEL_LDS_MAX = 150 * 1024;
 constexpr int SWTAB = 1 + 4 * LU_NB;  // count, then up to 2*NB (row, src) int pairs
+template <typename T> __device__ __forceinline__ T make_conj(T x) { return x; }
+template <> __device__ __forceinline__ c128 make_conj<c128>(c128 x) { return c128{x.re, -x.im}; }
+template <> __device__ __forceinline__ c64 make_conj<c64>(c64 x) { return c64{x.re, -x.im}; }
+
 constexpr int LU_NBO = 256;           // column block of the blocked factorisation
 constexpr int LU_BLOCKED_MIN = 768;   // matrices larger than this are factored in column blocks
 
@@ -638,8 +648,10 @@ size_t getrf_scratch_bytes(int n, int batch) {
 }
 
 // X[k:k+nb, :] = tri(Dinv_k) X[k:k+nb, :] in place; tri = unit lower (forward)
-// or upper (backward).  256 threads = 32 rows x 8 columns per workgroup.
-template <typename T, bool UPPER>
+// or upper (backward).  With CT the conjugate transpose of the other stored
+// inverse is used: lower = inv(U_kk)^H (the U^H solve), upper = inv(L_kk)^H
+// (the L^H solve).  256 threads = 32 rows x 8 columns per workgroup.
+template <typename T, bool UPPER, bool CT = false>
 __global__ __launch_bounds__(256) void trsm_inv_kernel(int n, int nrhs, const T* dinv, T* X, long ldx, long sX, int k,
                                                        int nb) {
   __shared__ T Di[LU_NB][LU_NB + 1];
@@ -653,10 +665,19 @@ __global__ __launch_bounds__(256) void trsm_inv_kernel(int n, int nrhs, const T*
   for (int idx = tid; idx < LU_NB * LU_NB; idx += 256) {
     int r = idx / LU_NB, q = idx - r * LU_NB;
     T v;
-    if (UPPER)
-      v = q >= r ? db[idx] : s_zero<T>();
-    else
-      v = q < r ? db[idx] : (q == r ? s_one<T>() : s_zero<T>());
+    if constexpr (CT) {
+      const T t = db[q * LU_NB + r];
+      const T ct = make_conj(t);
+      if (UPPER)
+        v = q > r ? ct : (q == r ? s_one<T>() : s_zero<T>());
+      else
+        v = q <= r ? ct : s_zero<T>();
+    } else {
+      if (UPPER)
+        v = q >= r ? db[idx] : s_zero<T>();
+      else
+        v = q < r ? db[idx] : (q == r ? s_one<T>() : s_zero<T>());
+    }
     Di[r][q] = v;
   }
   T* Xb = X + (long)b * sX;
@@ -840,12 +861,108 @@ hipError_t getrs_perm(int n, int nrhs, const T* LU, long lda, long sA, const int
   return hipGetLastError();
 }
 
+// X[b][perm[b][i]][c] = W[b][i][c]  (the inverse of gather_rows_kernel)
+template <typename T>
+__global__ void scatter_rows_kernel(int n, int nrhs, const int* perm, const T* W, long ldw, long sW, T* X, long ldx,
+                                    long sX) {
+  const int b = blockIdx.z;
+  const int i = blockIdx.y;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nrhs) return;
+  int dst = perm[(long)b * n + i];
+  X[(long)b * sX + (long)dst * ldx + c] = W[(long)b * sW + (long)i * ldw + c];
+}
+
+// A^H X = R with P A = L U:  X = P^T L^{-H} U^{-H} R.  U^H is lower
+// triangular (forward, with inv(U_kk)^H), L^H unit upper (backward, with
+// inv(L_kk)^H); the off-diagonal updates are conjugate-transposed GEMMs on
+// the stored factors, blocked like getrs_perm.
+template <typename T>
+hipError_t getrs_conjt(int n, int nrhs, const T* LU, long lda, long sA, const int* perm, const T* dinv, const T* R,
+                       long ldr, long sR, T* X, long ldx, long sX, T* W, int batch, hipStream_t s) {
+  if (n <= 0 || nrhs <= 0 || batch <= 0) return hipSuccess;
+  const long ldw = nrhs, sW = (long)n * nrhs;
+  HPS_LU_CHECK(copy2d<T>(n, nrhs, 1.0, R, ldr, sR, W, ldw, sW, batch, s));
+  dim3 tgrid(ceil_div(nrhs, 8), batch);
+  auto gemm_ct = [&](int M, int K, const T* Ap, const T* Bp, T* Cp) -> hipError_t {
+    if (M <= 0 || K <= 0) return hipSuccess;
+    GemmArgs<T> g;
+    g.M = M;
+    g.N = nrhs;
+    g.K = K;
+    g.alpha = -1.0;
+    g.beta = 1.0;
+    g.A = Ap;
+    g.lda = lda;
+    g.sA = sA;
+    g.transA = 1;
+    g.B = Bp;
+    g.ldb = ldw;
+    g.sB = sW;
+    g.C = Cp;
+    g.ldc = ldw;
+    g.sC = sW;
+    g.D = Cp;
+    g.ldd = ldw;
+    g.sD = sW;
+    g.batch = batch;
+    return gemm<T>(g, s);
+  };
+  const int nbo = n > LU_BLOCKED_MIN ? LU_NBO : n;
+  // forward: U^H Y = R
+  for (int k0 = 0; k0 < n; k0 += nbo) {
+    const int bend = n - k0 < nbo ? n : k0 + nbo;
+    for (int k = k0; k < bend; k += LU_NB) {
+      const int nb = bend - k < LU_NB ? bend - k : LU_NB;
+      {
+        static const std::string nm = std::string("trsm_lower_conjt<") + tname<T>() + ">";
+        prof::Scope ps(s, nm.c_str(), flop_mult<T>() * batch * (double)nb * nb * nrhs,
+                       2.0 * sizeof(T) * (double)nb * nrhs * batch);
+        hipLaunchKernelGGL((trsm_inv_kernel<T, false, true>), tgrid, dim3(256), 0, s, n, nrhs, dinv, W, ldw, sW, k,
+                           nb);
+      }
+      HPS_LU_CHECK(gemm_ct(bend - k - nb, nb, LU + (long)k * lda + k + nb, W + (long)k * ldw, W + (long)(k + nb) * ldw));
+    }
+    HPS_LU_CHECK(gemm_ct(n - bend, bend - k0, LU + (long)k0 * lda + bend, W + (long)k0 * ldw, W + (long)bend * ldw));
+  }
+  // backward: L^H Z = Y
+  const int nblo = (n + nbo - 1) / nbo;
+  for (int bo = nblo - 1; bo >= 0; --bo) {
+    const int k0 = bo * nbo;
+    const int bend = n - k0 < nbo ? n : k0 + nbo;
+    const int last = ((bend - 1 - k0) / LU_NB) * LU_NB + k0;
+    for (int k = last; k >= k0; k -= LU_NB) {
+      const int nb = bend - k < LU_NB ? bend - k : LU_NB;
+      {
+        static const std::string nm = std::string("trsm_upper_conjt<") + tname<T>() + ">";
+        prof::Scope ps(s, nm.c_str(), flop_mult<T>() * batch * (double)nb * nb * nrhs,
+                       2.0 * sizeof(T) * (double)nb * nrhs * batch);
+        hipLaunchKernelGGL((trsm_inv_kernel<T, true, true>), tgrid, dim3(256), 0, s, n, nrhs, dinv, W, ldw, sW, k,
+                           nb);
+      }
+      HPS_LU_CHECK(gemm_ct(k - k0, nb, LU + (long)k * lda + k0, W + (long)k * ldw, W + (long)k0 * ldw));
+    }
+    HPS_LU_CHECK(gemm_ct(k0, bend - k0, LU + (long)k0 * lda, W + (long)k0 * ldw, W));
+  }
+  for (int b0 = 0; b0 < batch; b0 += 65535) {
+    static const std::string nm = std::string("getrs_scatter<") + tname<T>() + ">";
+    prof::Scope ps(s, nm.c_str(), 0.0, 2.0 * sizeof(T) * (double)n * nrhs * batch);
+    int bc = batch - b0 < 65535 ? batch - b0 : 65535;
+    dim3 grid(ceil_div(nrhs, 64), n, bc);
+    hipLaunchKernelGGL((scatter_rows_kernel<T>), grid, dim3(64), 0, s, n, nrhs, perm + (long)b0 * n,
+                       W + (long)b0 * sW, ldw, sW, X + (long)b0 * sX, ldx, sX);
+  }
+  return hipGetLastError();
+}
+
 size_t dinv_elems(int n) { return (size_t)((n + LU_NB - 1) / LU_NB) * LU_NB * LU_NB; }
 
 #define HPS_INST_LU(T)                                                                                          \
   template hipError_t getrf_ws<T>(int, T*, long, long, int*, T*, int*, int*, int, hipStream_t);                 \
   template hipError_t getrs_perm<T>(int, int, const T*, long, long, const int*, const T*, const T*, long, long, \
-                                    T*, long, long, int, hipStream_t);
+                                    T*, long, long, int, hipStream_t);                                          \
+  template hipError_t getrs_conjt<T>(int, int, const T*, long, long, const int*, const T*, const T*, long, long, \
+                                     T*, long, long, T*, int, hipStream_t);
 HPS_INST_LU(double)
 HPS_INST_LU(float)
 HPS_INST_LU(c128)

@@ -154,3 +154,44 @@ def exact_D(dom, nsrc: int = 4):
         X, Y, Z = x[..., 0], x[..., 1], x[..., 2]
         out.append(np.sin(np.pi * X) * np.sin(np.pi * Y) * np.sin(np.pi * Z) * (1 + 0.25 * kk * (X + Y + Z)))
     return np.stack(out, -1)
+
+
+def sine_freqs(k: float, n_terms: int = 25) -> np.ndarray:
+    """The first n_terms frequency pairs (k1, k2), 1 <= k1, k2, |(k1, k2)| <= 2k,
+    in increasing norm (ties in (k1, k2) order) -- get_freqs_up_to_2k of
+    ``examples/sine_transform.py:103-140``."""
+    gamma = int(k * 2)
+    f = np.array([[k1, k2] for k1 in range(1, gamma + 1) for k2 in range(1, gamma + 1)
+                  if np.sqrt(k1**2 + k2**2) <= gamma], dtype=np.float64)
+    order = np.argsort(np.linalg.norm(f, axis=1), kind="stable")
+    return f[order][:n_terms]
+
+
+def sine_basis(freqs: np.ndarray, pts: np.ndarray) -> np.ndarray:
+    """(n_terms, n_leaves, n): sin(pi/2 k1 (x+1)) sin(pi/2 k2 (y+1)), the basis of
+    adjoint_nu_sinetransform (``examples/sine_transform.py:61-100``)."""
+    sx = np.sin(np.pi / 2 * freqs[:, 0, None, None] * (pts[None, ..., 0] + 1.0))
+    sy = np.sin(np.pi / 2 * freqs[:, 1, None, None] * (pts[None, ..., 1] + 1.0))
+    return sx * sy
+
+
+def config_E(L: int = 6, p: int = 16, k: float = 40.0, n_theta: int = 8, n_terms: int = 25, seed: int = 0):
+    """Inverse-scattering gradient workload: Lap u + k^2 (1 - q_theta) u = 0,
+    ItI with eta = k, q_theta = sum_j theta_j basis_j over the first 25 sine
+    frequencies, plane-wave impedance data; thetas ~ N(0, 0.1^2) (n_theta of
+    them) and a target theta_star.  Returns (Domain, PDEProblem kwargs without
+    I_coefficients, boundary data, basis (n_terms, n_leaves, n), thetas,
+    theta_star)."""
+    rng = np.random.default_rng(seed)
+    root = DiscretizationNode2D(-1.0, 1.0, -1.0, 1.0)
+    dom = Domain(p=p, q=p - 2, root=root, L=L)
+    x = dom.interior_points
+    basis = sine_basis(sine_freqs(k, n_terms), x)
+    ones = np.ones(x.shape[:2])
+    kwargs = dict(D_xx_coefficients=ones, D_yy_coefficients=ones, source=np.zeros(x.shape[:2], dtype=np.complex128),
+                  use_ItI=True, eta=k)
+    alpha = rng.uniform(0.0, 2 * np.pi)
+    g = plane_wave_impedance(dom.boundary_points, k, k, alpha)
+    thetas = 0.1 * rng.standard_normal((n_theta, n_terms))
+    theta_star = 0.1 * rng.standard_normal(n_terms)
+    return dom, kwargs, g, basis, thetas, theta_star

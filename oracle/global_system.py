@@ -84,6 +84,17 @@ def solve_dtn_global(A, f, P, Q, bounds, L, q, g_root, root_pts):
 
 def solve_iti_global(A, f, P_iti, QH, G, bounds, L, q, t_root, root_pts):
     """ItI version; t_root is the incoming impedance at root_pts.  Returns u (nl, n)."""
+    M, rhs, per = iti_system(A, f, P_iti, QH, G, bounds, L, q, t_root, root_pts)
+    nl, n, _ = A.shape
+    x = np.linalg.solve(M, rhs)
+    return np.stack([x[l * per : l * per + n] for l in range(nl)])
+
+
+def iti_system(A, f, P_iti, QH, G, bounds, L, q, t_root, root_pts):
+    """The dense ItI system M x = rhs; unknowns per leaf l: u_l (n values) then
+    its incoming impedance t_l (4q values), offset l * per.  Leaf l's interior
+    rows are rows l*per + nb .. l*per + n, holding A[l, nb:, :] in columns
+    l*per .. l*per + n."""
     nl, n, _ = A.shape
     nb = P_iti.shape[0]
     nq = 4 * q
@@ -117,5 +128,4 @@ def solve_iti_global(A, f, P_iti, QH, G, bounds, L, q, t_root, root_pts):
                 assert len(others) == 1
                 lb, b = others[0]
                 M[row, lb * per : lb * per + n] += QH[b]
-    x = np.linalg.solve(M, rhs)
-    return np.stack([x[l * per : l * per + n] for l in range(nl)])
+    return M, rhs, per

@@ -94,3 +94,43 @@ def test_getrf_reports_singular(gpu):
     assert info[1] == 5
     with pytest.raises(_lib.HPSSingularError):
         _lib.raise_on_info(torch.tensor(info), "test")
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("batch,M,N,K", [(3, 70, 1, 45), (2, 33, 2, 300), (4, 100, 9, 64), (2, 130, 70, 33)])
+def test_gemm_conjt_matches_reference(gpu, dtype, batch, M, N, K):
+    """D = alpha A^H B + beta C with A stored K x M."""
+    from jaxhps_amd import _lib
+
+    print(f"gemm_conjt dtype={dtype} shape={(batch, M, N, K)}", flush=True)
+    g = torch.Generator().manual_seed(batch + M + N + K)
+    A = _rand((batch, K, M), dtype, g)
+    B = _rand((batch, K, N), dtype, g)
+    C = _rand((batch, M, N), dtype, g)
+    lib = _lib.load_library()
+    h = _lib.handle()
+    Ad, Bd, Cd = A.to(gpu), B.to(gpu), C.to(gpu)
+    out = torch.empty((batch, M, N), dtype=dtype, device=gpu)
+    rc = lib.hps_gemm_batched_conjt(h, _lib.dtype_code(dtype), M, N, K, 0.5, _lib.ptr(Ad), M, K * M, _lib.ptr(Bd), N,
+                                    K * N, -1.0, _lib.ptr(Cd), N, M * N, _lib.ptr(out), N, M * N, batch)
+    _lib.check(rc, h, "conjt")
+    ref = 0.5 * (_ref64(A).conj().transpose(1, 2) @ _ref64(B)) - _ref64(C)
+    err = (_ref64(out) - ref).abs().max().item() / max(1.0, ref.abs().max().item())
+    assert err < TOL[dtype] * max(1, K) ** 0.5, err
+
+
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("n,nrhs,batch", [(37, 3, 4), (256, 1, 3), (1100, 5, 1)])
+def test_getrs_conjt_solves_adjoint_system(gpu, dtype, n, nrhs, batch):
+    from jaxhps_amd import _lib
+    from jaxhps_amd.adjoint import getrs_conjt
+
+    print(f"getrs_conjt dtype={dtype} n={n}", flush=True)
+    g = torch.Generator().manual_seed(n + nrhs)
+    A = _rand((batch, n, n), dtype, g) + n**0.5 * torch.eye(n, dtype=dtype)
+    R = _rand((batch, n, nrhs), dtype, g)
+    LU = A.to(gpu).contiguous()
+    perm, dinv, info = _lib.getrf(LU)
+    X = getrs_conjt(n, LU, 0, n, n * n, perm, dinv, R.to(gpu).contiguous())
+    res = _ref64(A).conj().transpose(1, 2) @ _ref64(X) - _ref64(R)
+    assert res.abs().max().item() < TOL[dtype] * 50 * n**0.5
