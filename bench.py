@@ -39,8 +39,12 @@ def parse():
     ap.add_argument("--profile-only", action="store_true", help="skip the CPU baseline and extra passes")
     ap.add_argument("--precision", default="both", choices=["both", "mixed", "fp64"],
                     help="which path(s) to time; the value is the mixed path's unless only fp64 runs")
-    ap.add_argument("--fp64-top-levels", type=int, default=2)
+    ap.add_argument("--fp64-top-levels", type=int, default=0,
+                    help="merge levels at the top kept in FP64 by the mixed path (complex64 merges suffice for "
+                         "<= 1e-8 here: the refined error is set by the FP64 residual)")
     ap.add_argument("--refine-tol", type=float, default=1e-11)
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint grads/s measurement")
+    ap.add_argument("--adjoint-L", type=int, default=6, help="tree depth of the adjoint workload (testing only)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (hps_convert, complex128 -> complex128, "
                          "256 MiB, 10 times): the reference for scaling HBM byte counters")
@@ -159,6 +163,57 @@ def _cpu_baseline_3d():
                       f"median of {n} runs, BLAS threads <= {cores}"}
 
 
+def adjoint_bench(L=6, warmup=1, reps=2):
+    """Inverse-scattering gradients: config E's problem (k = 40, p = 16, 25 sine
+    coefficients, 8 seeded thetas); one gradient = build + solve + adjoint of
+    J(theta) = ||u(q_theta) - u_obs||^2 for one theta.  Returns grads/s over the
+    batch of 8, timed after warmup."""
+    import numpy as np
+    import torch
+
+    import jaxhps_amd as jh
+    from jaxhps_amd import workloads as wl
+    from jaxhps_amd.adjoint import differentiable_solve
+
+    k = 40.0
+    dom, kw, g, basis, thetas, theta_star = wl.config_E(L=L, p=16, k=k)
+    dev = torch.device("cuda", 0)
+    basis_t = torch.as_tensor(basis, device=dev)
+    g_dev = torch.as_tensor(g, device=dev)
+    dkw = {key: (torch.as_tensor(v, device=dev) if isinstance(v, np.ndarray) else v) for key, v in kw.items()}
+    ones = torch.ones(dom.interior_points.shape[:2], dtype=torch.float64, device=dev)
+    pde = jh.PDEProblem(dom, I_coefficients=ones, **dkw)
+    with torch.no_grad():
+        I_star = k**2 * (1.0 - torch.einsum("j,jln->ln", torch.as_tensor(theta_star, device=dev), basis_t))
+        u_obs = differentiable_solve(pde, g_dev, I=I_star).clone()
+    thetas_t = torch.as_tensor(thetas, device=dev)
+
+    def batch():
+        grads = []
+        for th in thetas_t:
+            th = th.clone().requires_grad_(True)
+            I = k**2 * (1.0 - torch.einsum("j,jln->ln", th, basis_t))
+            u = differentiable_solve(pde, g_dev, I=I)
+            J = ((u - u_obs).abs() ** 2).sum()
+            J.backward()
+            grads.append(th.grad)
+        return torch.stack(grads)
+
+    for _ in range(warmup):
+        batch()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        grads = batch()
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / reps
+    return {"metric": "adjoint grads/s", "value": thetas.shape[0] / t, "unit": "grads/s",
+            "ms_per_grad": t / thetas.shape[0] * 1e3, "batch": int(thetas.shape[0]),
+            "config": {"workload": "2D Helmholtz inverse scattering ItI k=40 p=16, 25 sine coefficients",
+                       "L": L, "n_leaves": dom.n_leaves},
+            "dtype": "c128", "grad_norm": float(grads.norm())}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -273,6 +328,12 @@ def main():
             out["fp64_path"]["roofline"] = p64["roofline"]
     if not args.no_cpu_baseline and not args.profile_only:
         out["cpu_baseline"] = cpu_baseline(args.config)
+    if args.config == "C" and not args.no_adjoint and not args.profile_only:
+        for prec in res.values():
+            prec.pop("pde", None)
+        res.clear()
+        torch.cuda.empty_cache()
+        out["adjoint"] = adjoint_bench(args.adjoint_L)
     print(json.dumps(out), flush=True)
 
 

@@ -185,7 +185,12 @@ def refine_solve(pde, boundary_data, u0, source, g_leaf0=None, tol=1e-11, max_it
     source: the FP64 source the solution is for (None = zero).  g_leaf0: the
     DtN leaves' Gauss data of the first solve.  Returns u in FP64 (c128 for
     complex problems) shaped like u0, and records the iteration history in
-    ``pde.refine_history``."""
+    ``pde.refine_history``.
+
+    Stopping: after each correction the residual of the new iterate is
+    evaluated; the next update is estimated as the last update times the
+    residual's reduction and the loop ends when that estimate (or an actual
+    update) is below tol relative to |u|."""
     from .down_pass import down_levels
     from .up_pass import _up
 
@@ -214,10 +219,27 @@ def refine_solve(pde, boundary_data, u0, source, g_leaf0=None, tol=1e-11, max_it
         set_boundary_rows(ops, u, g_leaf)
     hist = []
     converged = False
-    for it in range(max_iter):
-        if iti:
-            with profiling.stage("refine_residual"):
+    prev = None  # (residual norm, update norm) of the last correction
+    for it in range(max_iter + 1):
+        with profiling.stage("refine_residual"):
+            if iti:
                 res, rn2 = residual_iti(ops, u, f, g_root, lo)
+            else:
+                res, hinj, groot, rn2 = residual_dtn(ops, u, g_leaf, f, g_root, lo)
+        if prev is not None:
+            # The correction operator is linear, so the next update would be
+            # about the last one scaled by the residual's reduction; stop
+            # without computing it once that estimate is below tol.
+            rn = float(rn2.sqrt())
+            est = prev[1] * rn / max(prev[0], 1e-300)
+            hist[-1]["residual_after"] = rn
+            hist[-1]["next_update_estimate"] = est / max(un, 1e-300)
+            if est <= tol * un:
+                converged = True
+                break
+        if it == max_iter:
+            break
+        if iti:
             with profiling.stage("refine_up"):
                 v, g_lst = _up(res, pde, rhs_full=True)
             with profiling.stage("refine_down"):
@@ -225,8 +247,6 @@ def refine_solve(pde, boundary_data, u0, source, g_leaf0=None, tol=1e-11, max_it
             with profiling.stage("refine_update"):
                 dn2, un2 = refine_update(u, du)
         else:
-            with profiling.stage("refine_residual"):
-                res, hinj, groot, rn2 = residual_dtn(ops, u, g_leaf, f, g_root, lo)
             with profiling.stage("refine_up"):
                 v, g_lst = _up(res, pde, rhs_full=True, hinj=hinj)
             with profiling.stage("refine_down"):
@@ -237,10 +257,12 @@ def refine_solve(pde, boundary_data, u0, source, g_leaf0=None, tol=1e-11, max_it
                 set_boundary_rows(ops, u, g_leaf)
         dn, un, rn = (float(x) for x in torch.stack([dn2, un2, rn2]).sqrt().cpu())
         hist.append({"iter": it + 1, "residual": rn, "rel_update": dn / max(un, 1e-300)})
+        prev = (rn, dn)
         if dn <= tol * un:
             converged = True
             break
     pde.refine_history = hist
+    pde.refine_converged = converged
     if not converged:
         warnings.warn(f"iterative refinement stopped after {max_iter} iterations at relative update "
                       f"{hist[-1]['rel_update']:.2e} (tol {tol:.1e})", RuntimeWarning, stacklevel=3)

@@ -781,10 +781,130 @@ __global__ void gather_rows_kernel(int n, int nrhs, const int* perm, const T* R,
   X[(long)b * sX + (long)i * ldx + c] = R[(long)b * sR + (long)src * ldr + c];
 }
 
+// Small systems with few right-hand sides (the leaf solves of the up pass,
+// refinement and adjoint): one workgroup per matrix keeps X (n x nrhs) in LDS
+// and streams the factors once, 32-row block by block: the diagonal block
+// through its stored inverse, the rows below (above) by per-thread dot
+// products over the block's 32 columns.  CT solves with A^H instead
+// (U^H forward, L^H backward, then X = P^T Z).
+constexpr int LU_SMALL_MAX_N = 1024;
+constexpr int LU_SMALL_MAX_RHS = 16;
+
+template <typename T, bool CT>
+__global__ __launch_bounds__(256) void getrs_small_kernel(int n, int nrhs, const T* LU, long lda, long sA,
+                                                          const int* perm, const T* dinv, const T* R, long ldr,
+                                                          long sR, T* X, long ldx, long sX) {
+  extern __shared__ char smem_raw[];
+  T* xs = (T*)smem_raw;  // (n, nrhs)
+  __shared__ T Di[LU_NB][LU_NB + 1];
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x;
+  const T* A = LU + (long)b * sA;
+  const int* pb = perm + (long)b * n;
+  const int nblk = (n + LU_NB - 1) / LU_NB;
+  const T* db = dinv + (long)b * nblk * LU_NB * LU_NB;
+  const T* Rb = R + (long)b * sR;
+  for (int idx = tid; idx < n * nrhs; idx += 256) {
+    const int i = idx / nrhs, c = idx - i * nrhs;
+    xs[idx] = CT ? Rb[(long)i * ldr + c] : Rb[(long)pb[i] * ldr + c];
+  }
+  __syncthreads();
+  // pass 0: lower (L, or U^H for CT), forward; pass 1: upper (U, or L^H), backward
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int s = 0; s < nblk; ++s) {
+      const int kb = pass == 0 ? s : nblk - 1 - s;
+      const int k = kb * LU_NB;
+      const int nb = n - k < LU_NB ? n - k : LU_NB;
+      const T* dk = db + (long)kb * LU_NB * LU_NB;
+      for (int idx = tid; idx < LU_NB * LU_NB; idx += 256) {
+        const int r = idx / LU_NB, q = idx - r * LU_NB;
+        T v;
+        if (pass == 0) {
+          if (CT)
+            v = q <= r ? make_conj(dk[q * LU_NB + r]) : s_zero<T>();
+          else
+            v = q < r ? dk[idx] : (q == r ? s_one<T>() : s_zero<T>());
+        } else {
+          if (CT)
+            v = q > r ? make_conj(dk[q * LU_NB + r]) : (q == r ? s_one<T>() : s_zero<T>());
+          else
+            v = q >= r ? dk[idx] : s_zero<T>();
+        }
+        Di[r][q] = v;
+      }
+      __syncthreads();
+      // diagonal block: y = Di x_k
+      T y[LU_SMALL_MAX_RHS / 4];
+      const int per = (nb * nrhs + 255) / 256;  // <= 2 for nb <= 32, nrhs <= 16
+      for (int e = 0; e < per; ++e) {
+        const int idx = tid + 256 * e;
+        T acc = s_zero<T>();
+        if (idx < nb * nrhs) {
+          const int r = idx / nrhs, c = idx - r * nrhs;
+          for (int q = 0; q < nb; ++q) acc = s_add(acc, s_mul(Di[r][q], xs[(k + q) * nrhs + c]));
+        }
+        y[e] = acc;
+      }
+      __syncthreads();
+      for (int e = 0; e < per; ++e) {
+        const int idx = tid + 256 * e;
+        if (idx < nb * nrhs) xs[(k + idx / nrhs) * nrhs + idx % nrhs] = y[e];
+      }
+      __syncthreads();
+      // rows outside the block: below (pass 0) or above (pass 1)
+      const int r0 = pass == 0 ? k + nb : 0;
+      const int r1 = pass == 0 ? n : k;
+      for (int i = r0 + tid; i < r1; i += 256) {
+        for (int c = 0; c < nrhs; ++c) {
+          T acc = xs[i * nrhs + c];
+          for (int q = 0; q < nb; ++q) {
+            // L[i][k+q] / U[i][k+q], or for CT conj(U[k+q][i]) / conj(L[k+q][i])
+            const T a = CT ? make_conj(A[(long)(k + q) * lda + i]) : A[(long)i * lda + k + q];
+            acc = s_fnms(acc, a, xs[(k + q) * nrhs + c]);
+          }
+          xs[i * nrhs + c] = acc;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  T* Xb = X + (long)b * sX;
+  for (int idx = tid; idx < n * nrhs; idx += 256) {
+    const int i = idx / nrhs, c = idx - i * nrhs;
+    if (CT)
+      Xb[(long)pb[i] * ldx + c] = xs[idx];
+    else
+      Xb[(long)i * ldx + c] = xs[idx];
+  }
+}
+
+template <typename T>
+static bool getrs_small_ok(int n, int nrhs) {
+  return n <= LU_SMALL_MAX_N && nrhs <= LU_SMALL_MAX_RHS && (size_t)n * nrhs * sizeof(T) <= 64 * 1024;
+}
+
+template <typename T, bool CT>
+static hipError_t getrs_small(int n, int nrhs, const T* LU, long lda, long sA, const int* perm, const T* dinv,
+                              const T* R, long ldr, long sR, T* X, long ldx, long sX, int batch, hipStream_t s) {
+  static const std::string nm = std::string(CT ? "getrs_small_conjt<" : "getrs_small<") + tname<T>() + ">";
+  prof::Scope ps(s, nm.c_str(), flop_mult<T>() * 2.0 * batch * (double)n * n * nrhs,
+                 sizeof(T) * (double)batch * ((double)n * n + 2.0 * n * nrhs));
+  for (int b0 = 0; b0 < batch; b0 += 65535) {
+    const int bc = batch - b0 < 65535 ? batch - b0 : 65535;
+    hipLaunchKernelGGL((getrs_small_kernel<T, CT>), dim3(bc), dim3(256), (size_t)n * nrhs * sizeof(T), s, n, nrhs,
+                       LU + (long)b0 * sA, lda, sA, perm + (long)b0 * n,
+                       dinv + (long)b0 * ((n + LU_NB - 1) / LU_NB) * LU_NB * LU_NB, R + (long)b0 * sR, ldr, sR,
+                       X + (long)b0 * sX, ldx, sX);
+  }
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t getrs_perm(int n, int nrhs, const T* LU, long lda, long sA, const int* perm, const T* dinv, const T* R,
                       long ldr, long sR, T* X, long ldx, long sX, int batch, hipStream_t s) {
   if (n <= 0 || nrhs <= 0 || batch <= 0) return hipSuccess;
+  if (getrs_small_ok<T>(n, nrhs))
+    return getrs_small<T, false>(n, nrhs, LU, lda, sA, perm, dinv, R, ldr, sR, X, ldx, sX, batch, s);
   for (int b0 = 0; b0 < batch; b0 += 65535) {
     static const std::string nm = std::string("getrs_gather<") + tname<T>() + ">";
     prof::Scope ps(s, nm.c_str(), 0.0, 2.0 * sizeof(T) * (double)n * nrhs * batch);
@@ -881,6 +1001,8 @@ template <typename T>
 hipError_t getrs_conjt(int n, int nrhs, const T* LU, long lda, long sA, const int* perm, const T* dinv, const T* R,
                        long ldr, long sR, T* X, long ldx, long sX, T* W, int batch, hipStream_t s) {
   if (n <= 0 || nrhs <= 0 || batch <= 0) return hipSuccess;
+  if (getrs_small_ok<T>(n, nrhs))
+    return getrs_small<T, true>(n, nrhs, LU, lda, sA, perm, dinv, R, ldr, sR, X, ldx, sX, batch, s);
   const long ldw = nrhs, sW = (long)n * nrhs;
   HPS_LU_CHECK(copy2d<T>(n, nrhs, 1.0, R, ldr, sR, W, ldw, sW, batch, s));
   dim3 tgrid(ceil_div(nrhs, 8), batch);
