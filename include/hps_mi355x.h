@@ -240,6 +240,29 @@ int hps_adjoint_up_level(hps_handle_t h, int dtype, int kind, int nm, int nc, in
 int hps_coef_gradient(hps_handle_t h, int dtype, int nl, int n, int row0, int nsrc, int nops, int ncoef,
                       const int* coef_ops, const void* R, int nrow_R, const void* u, const void* lam, double* grad);
 
+/* ---- learned leaf-ItI surrogate (no counterpart in the reference) ---------
+ * A small MLP maps a leaf's coefficient samples to a rank-r complex
+ * correction U V^H of the q = 0 leaf ItI map.  FP32 tensors in memory, BF16
+ * MFMA products with FP32 accumulation.
+ * C = op(A) op(B) [+ beta C] [+ bias[n]] [tanh]; op(A) = A^T (A stored K x M)
+ * when transA, op(B) = B^T (B stored N x K) when transB. */
+int hps_mlp_gemm_bf16(hps_handle_t h, int transA, int transB, int M, int N, int K, const float* A, int lda,
+                      const float* B, int ldb, float beta, float* C, int ldc, const float* bias, int act);
+/* dH *= 1 - H^2 (tanh backward, in place). */
+int hps_mlp_tanh_backward(hps_handle_t h, long n, const float* H, float* dH);
+/* db[n] = sum_m dZ[m][n]. */
+int hps_mlp_bias_grad(hps_handle_t h, int M, int N, const float* dZ, float* db);
+/* Adam step (bias-corrected, step >= 1) on n parameters. */
+int hps_mlp_adam(hps_handle_t h, long n, float* w, const float* g, float* m1, float* m2, float lr, float b1, float b2,
+                 float eps, int step);
+/* Y (nsamp, 4 nq r) = [U re | U im | V re | V im]; dT (nsamp, nq, nq)
+ * complex64: loss[s] = scale |U V^H - dT|_F^2, dY = its gradient. */
+int hps_mlp_lowrank_loss(hps_handle_t h, int nsamp, int nq, int r, const float* Y, const float* dT, float scale,
+                         float* dY, float* loss);
+/* T[s] = T0 + out_scale U V^H, complex128 (nsamp, nq, nq). */
+int hps_mlp_lowrank_apply(hps_handle_t h, int nsamp, int nq, int r, const float* Y, const double* T0,
+                          double out_scale, double* T);
+
 /* ---- optional kernel timing ----------------------------------------------
  * When enabled, every kernel launch of the library is bracketed by HIP events
  * on its stream and attributed to a kernel class ("gemm_mfma<c128>", ...)

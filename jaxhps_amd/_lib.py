@@ -71,6 +71,12 @@ SIGNATURES = {
     "hps_adjoint_up_workspace": (_S, [_I, _I, _I, _I, _I]),
     "hps_adjoint_up_level": (_I, [_P] + [_I] * 8 + [_P] * 12 + [_S]),
     "hps_coef_gradient": (_I, [_P] + [_I] * 7 + [_P, _P, _I, _P, _P, _P]),
+    "hps_mlp_gemm_bf16": (_I, [_P, _I, _I, _I, _I, _I, _P, _I, _P, _I, ctypes.c_float, _P, _I, _P, _I]),
+    "hps_mlp_tanh_backward": (_I, [_P, _L, _P, _P]),
+    "hps_mlp_bias_grad": (_I, [_P, _I, _I, _P, _P]),
+    "hps_mlp_adam": (_I, [_P, _L, _P, _P, _P, _P] + [ctypes.c_float] * 4 + [_I]),
+    "hps_mlp_lowrank_loss": (_I, [_P, _I, _I, _I, _P, _P, ctypes.c_float, _P, _P]),
+    "hps_mlp_lowrank_apply": (_I, [_P, _I, _I, _I, _P, _P, _D, _P]),
     "hps_convert": (_I, [_P, _I, _I, _L, _P, _P]),
     "hps_prof_enable": (_I, [_I]),
     "hps_prof_reset": (_I, []),

@@ -121,7 +121,8 @@ __global__ __launch_bounds__(256) void perm_from_ipiv_kernel(int n, const int* i
 // After the panel: (row, source) table of its net row permutation, from the
 // swap sequence ipiv[k .. k+nb).  Single thread.
 __device__ void build_swap_table(int k, int nb, const int* ipivb, int* tab) {
-  int rows[2 * LU_NB], src[2 * LU_NB];
+  // working lists in LDS: a per-thread array indexed at run time would live in scratch
+  __shared__ int rows[2 * LU_NB], src[2 * LU_NB];
   int cnt = 0;
   for (int j = 0; j < nb; ++j) {
     int a = k + j, bb = ipivb[k + j];
@@ -154,33 +155,27 @@ __device__ void build_swap_table(int k, int nb, const int* ipivb, int* tab) {
 
 // Invert the factored nb x nb diagonal block D (row stride ldd) into the
 // packed form out (row stride ldo): strict lower = inv(L11), upper = inv(U11).
+// Thread c < nb forms column c of inv(L11), thread 64 + c column c of
+// inv(U11); each substitution works in its own column of `out` (LDS), so no
+// per-thread array is indexed at run time (those would live in scratch).
 template <typename T>
 __device__ void invert_diag_block(const T* D, int ldd, T* out, int ldo, int nb) {
   const int t = threadIdx.x;
   if (t < nb) {
-    // column t of inv(L11): unit lower forward substitution
     const int c = t;
-    T y[LU_NB];
-#pragma unroll
-    for (int i = 0; i < LU_NB; ++i) y[i] = s_zero<T>();
-    for (int i = c; i < nb; ++i) {
-      T acc = i == c ? s_one<T>() : s_zero<T>();
-      for (int j = c; j < i; ++j) acc = s_fnms(acc, D[i * ldd + j], y[j]);
-      y[i] = acc;
+    // unit lower forward substitution; the unit diagonal stays implicit
+    for (int i = c + 1; i < nb; ++i) {
+      T acc = s_neg(D[i * ldd + c]);
+      for (int j = c + 1; j < i; ++j) acc = s_fnms(acc, D[i * ldd + j], out[j * ldo + c]);
+      out[i * ldo + c] = acc;
     }
-    for (int i = c + 1; i < nb; ++i) out[i * ldo + c] = y[i];
   } else if (t >= 64 && t < 64 + nb) {
-    // column c of inv(U11): back substitution
     const int c = t - 64;
-    T x[LU_NB];
-#pragma unroll
-    for (int i = 0; i < LU_NB; ++i) x[i] = s_zero<T>();
     for (int i = c; i >= 0; --i) {
       T acc = i == c ? s_one<T>() : s_zero<T>();
-      for (int j = i + 1; j <= c; ++j) acc = s_fnms(acc, D[i * ldd + j], x[j]);
-      x[i] = s_mul(acc, s_inv(D[i * ldd + i]));
+      for (int j = i + 1; j <= c; ++j) acc = s_fnms(acc, D[i * ldd + j], out[j * ldo + c]);
+      out[i * ldo + c] = s_mul(acc, s_inv(D[i * ldd + i]));
     }
-    for (int i = 0; i <= c; ++i) out[i * ldo + c] = x[i];
   }
 }
 
@@ -266,15 +261,16 @@ __global__ __launch_bounds__(THREADS) void getrf_panel_lds_kernel(int n, T* A, l
 // physical order once at the end (only the <= 2*nb rows the swaps touched
 // move).
 // Sub-panel width and left-looking prefetch depth keep the per-thread
-// register arrays near 64 (x) + 32 (prefetch) VGPRs.
+// register arrays near 48 (x) + 16 (prefetch) VGPRs: a 1024-thread workgroup
+// leaves 128 VGPRs per thread, and larger arrays spill to scratch.
 template <typename T, int RPT>
 constexpr int reg_panel_sw() {
-  constexpr int w = 64 / (RPT * (int)(sizeof(T) / 4));
+  constexpr int w = 48 / (RPT * (int)(sizeof(T) / 4));
   return w >= 4 ? 4 : (w >= 2 ? 2 : 1);
 }
 template <typename T, int RPT>
 constexpr int reg_panel_jc() {
-  constexpr int w = 32 / (RPT * (int)(sizeof(T) / 4));
+  constexpr int w = 16 / (RPT * (int)(sizeof(T) / 4));
   return w >= 4 ? 4 : (w >= 2 ? 2 : 1);
 }
 
