@@ -125,6 +125,20 @@ def cpu_baseline(cfg):
     }
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the rocprofv3 PMC summaries under
+    profiles/ (a 'traffic: <kernel> <bytes>' line), or (None, None)."""
+    import glob
+
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(path) as fh:
+            for line in fh:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic:" and parts[1] == kernel:
+                    return float(parts[2]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def _time_bounded(run, budget_s=20.0):
     run()
     times = []
@@ -321,6 +335,10 @@ def main():
             prof = profiling.instrumented_step(pde, g_dev)
         out["stages_ms"] = prof["stages_ms"]
         out["roofline"] = prof["roofline"]
+        if out["roofline"] is not None:
+            traffic, src = measured_traffic(out["roofline"]["kernel"])
+            out["roofline"]["traffic"] = traffic
+            out["roofline"]["traffic_source"] = src
         out["kernels"] = prof["kernels"]
         if "fp64" in res and paths[0] != "fp64":
             p64 = profiling.instrumented_step(res["fp64"]["pde"], g_dev)
