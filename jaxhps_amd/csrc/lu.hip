@@ -44,6 +44,12 @@ template <> __device__ __forceinline__ c64 make_conj<c64>(c64 x) { return c64{x.
 
 constexpr int LU_NBO = 256;           // column block of the blocked factorisation
 constexpr int LU_BLOCKED_MIN = 768;   // matrices larger than this are factored in column blocks
+// tournament pivoting for panels taller than this when batch x rows is small
+// enough that one workgroup per matrix would leave most of the GPU idle
+constexpr int LU_TSLU_MIN_ROWS = 1024;
+constexpr int TS_CHUNK = 256;  // rows per first-round tournament chunk
+static size_t tslu_cand_ints(int n, int batch) { return (size_t)batch * ((n + TS_CHUNK - 1) / TS_CHUNK) * LU_NB; }
+constexpr long LU_TSLU_MAX_WORK = 64L * 1024;
 
 __global__ void fill_int_kernel(int* p, long n, int v) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -591,6 +597,256 @@ __global__ __launch_bounds__(256) void getrf_swap_u12_kernel(int n, T* A, long l
   }
 }
 
+// ---------------------------------------------------------------------------
+// Tall panels of a few matrices (the top merge levels): tournament pivoting
+// (communication-avoiding LU, Grigori, Demmel & Xiang, SIAM J. Matrix Anal.
+// Appl. 32 (2011)).  Every 256-row chunk of the panel picks nb candidate
+// pivot rows by partial pivoting on its own rows (one workgroup per chunk,
+// so the whole GPU works on one panel); groups of 8 candidate sets are
+// reduced the same way until nb winners remain.  The winners are swapped to
+// the top in their selection order, the top block is factored without
+// further pivoting (it reproduces the final round's elimination), and every
+// other row gets its multipliers L21 = A21 inv(U11) independently.
+// ---------------------------------------------------------------------------
+
+// Partial pivoting over the rows held one per thread (x = row, valid if
+// row_id >= 0): the nb pivot rows' ids, in order, to out[0..nb).
+template <typename T>
+__device__ void tslu_select(T (&x)[LU_NB], int row_id, int nb, int* out) {
+  using R = typename real_of<T>::type;
+  constexpr int THREADS = 256;
+  __shared__ R red_v[THREADS / 64];
+  __shared__ int red_i[THREADS / 64];
+  __shared__ int s_piv;
+  __shared__ R s_val;
+  __shared__ T prow[LU_NB];
+  const int t = threadIdx.x;
+  bool chosen = row_id < 0;
+  for (int j = 0; j < nb; ++j) {
+    T xj = s_zero<T>();
+#pragma unroll
+    for (int c = 0; c < LU_NB; ++c)
+      if (c == j) xj = x[c];
+    R bv = R(-1);
+    int bi = 0x7fffffff;
+    if (!chosen) argmax_merge(bv, bi, s_abs1(xj), t);
+    const int w = block_argmax<R, THREADS>(bv, bi, red_v, red_i, &s_piv, &s_val);
+    const bool have = w >= 0 && w < THREADS;  // no remaining row (short chunk): w is the sentinel
+    if (have && t == w) {
+      out[j] = row_id;
+#pragma unroll
+      for (int c = 0; c < LU_NB; ++c) prow[c] = x[c];
+      chosen = true;
+    }
+    if (!have && t == 0) out[j] = -1;
+    __syncthreads();
+    if (have && !chosen) {
+      const T piv = prow[j];
+      if (!s_is_zero(piv)) {
+        const T l = s_mul(xj, s_inv(piv));
+#pragma unroll
+        for (int c = 0; c < LU_NB; ++c)
+          if (c > j && c < nb) x[c] = s_fnms(x[c], l, prow[c]);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Round 0: chunk c of the panel rows [k, n) -> cand_out[b][c][0..nb)
+template <typename T>
+__global__ __launch_bounds__(256) void tslu_leaf_kernel(int n, const T* A, long lda, long sA, int k, int nb,
+                                                        int nchunks, int* cand_out) {
+  __shared__ int out[LU_NB];
+  const int b = blockIdx.y, c = blockIdx.x, t = threadIdx.x;
+  const int r = k + c * TS_CHUNK + t;
+  const T* Ab = A + (long)b * sA;
+  T x[LU_NB];
+  const bool live = r < n;
+#pragma unroll
+  for (int q = 0; q < LU_NB; ++q) x[q] = (live && q < nb) ? Ab[(long)r * lda + k + q] : s_zero<T>();
+  tslu_select<T>(x, live ? r : -1, nb, out);
+  if (t < nb) cand_out[((long)b * nchunks + c) * LU_NB + t] = out[t];
+}
+
+// Later rounds: group g merges up to 8 candidate sets of cand_in (nsets per matrix)
+template <typename T>
+__global__ __launch_bounds__(256) void tslu_reduce_kernel(int n, const T* A, long lda, long sA, int k, int nb,
+                                                          int nsets, const int* cand_in, int ngroups,
+                                                          int* cand_out) {
+  __shared__ int out[LU_NB];
+  const int b = blockIdx.y, g = blockIdx.x, t = threadIdx.x;
+  const int set = g * 8 + t / LU_NB, slot = t % LU_NB;
+  int r = -1;
+  if (set < nsets && slot < nb) r = cand_in[((long)b * nsets + set) * LU_NB + slot];
+  const T* Ab = A + (long)b * sA;
+  T x[LU_NB];
+#pragma unroll
+  for (int q = 0; q < LU_NB; ++q) x[q] = (r >= 0 && q < nb) ? Ab[(long)r * lda + k + q] : s_zero<T>();
+  tslu_select<T>(x, r, nb, out);
+  if (t < nb) cand_out[((long)b * ngroups + g) * LU_NB + t] = out[t];
+}
+
+// Winners -> ipiv, row swaps inside the panel columns, unpivoted LU of the top
+// block, dinv and the swap table for the other columns.  One workgroup per matrix.
+template <typename T>
+__global__ __launch_bounds__(256) void tslu_finish_kernel(int n, T* A, long lda, long sA, int k, int nb,
+                                                          const int* winners, int* ipiv, int* info, T* dinv,
+                                                          int* swtab) {
+  __shared__ int rows[2 * LU_NB], content[2 * LU_NB];
+  __shared__ int tab[SWTAB];
+  __shared__ T blk[LU_NB][LU_NB + 1];
+  __shared__ T vals[2 * LU_NB][LU_NB];
+  __shared__ T Dout[LU_NB][LU_NB + 1];
+  const int b = blockIdx.x, t = threadIdx.x;
+  T* Ab = A + (long)b * sA;
+  int* ipivb = ipiv + (long)b * n;
+  const int* wb = winners + (long)b * LU_NB;
+  if (t == 0) {
+    // LAPACK-style sequence: at step j swap row k+j with the current position of winner j
+    int cnt = 0;
+    for (int j = 0; j < nb; ++j) {
+      int w = wb[j];
+      if (w < k || w >= n) w = k + j;  // no candidate left (cannot happen for m >= nb): keep the row
+      int p = w;
+      for (int a = 0; a < cnt; ++a)
+        if (content[a] == w) p = rows[a];
+      const int tgt = k + j;
+      ipivb[tgt] = p;
+      if (p != tgt) {
+        int ia = -1, ib = -1;
+        for (int a = 0; a < cnt; ++a) {
+          if (rows[a] == tgt) ia = a;
+          if (rows[a] == p) ib = a;
+        }
+        if (ia < 0) {
+          rows[cnt] = tgt;
+          content[cnt] = tgt;
+          ia = cnt++;
+        }
+        if (ib < 0) {
+          rows[cnt] = p;
+          content[cnt] = p;
+          ib = cnt++;
+        }
+        const int tmp = content[ia];
+        content[ia] = content[ib];
+        content[ib] = tmp;
+      }
+    }
+    // (row, source) table of the net permutation
+    tab[0] = cnt;
+    for (int a = 0; a < cnt; ++a) {
+      tab[1 + 2 * a] = rows[a];
+      tab[2 + 2 * a] = content[a];
+    }
+  }
+  __syncthreads();
+  const int cnt = tab[0];
+  for (int i = t; i < SWTAB; i += 256) swtab[(long)b * SWTAB + i] = tab[i];
+  // swap inside the panel columns
+  for (int idx = t; idx < cnt * nb; idx += 256) {
+    const int a = idx / nb, c = idx - a * nb;
+    vals[a][c] = Ab[(long)tab[2 + 2 * a] * lda + k + c];
+  }
+  __syncthreads();
+  for (int idx = t; idx < cnt * nb; idx += 256) {
+    const int a = idx / nb, c = idx - a * nb;
+    Ab[(long)tab[1 + 2 * a] * lda + k + c] = vals[a][c];
+  }
+  __syncthreads();
+  // unpivoted LU of the top nb x nb block
+  for (int idx = t; idx < LU_NB * LU_NB; idx += 256) {
+    const int r = idx / LU_NB, c = idx - r * LU_NB;
+    blk[r][c] = (r < nb && c < nb) ? Ab[(long)(k + r) * lda + k + c] : s_zero<T>();
+  }
+  __syncthreads();
+  for (int j = 0; j < nb; ++j) {
+    const T piv = blk[j][j];
+    if (t == 0 && s_is_zero(piv) && info[b] == 0) info[b] = k + j + 1;
+    const int r = j + 1 + t / LU_NB, c = t % LU_NB;  // 8 rows x 32 columns per pass
+    for (int rr = r; rr < nb; rr += 256 / LU_NB) {
+      const T l = s_is_zero(piv) ? s_zero<T>() : s_mul(blk[rr][j], s_inv(piv));
+      if (c > j && c < nb) {
+        // read blk[rr][j] before the c == j thread overwrites it
+        blk[rr][c] = s_fnms(blk[rr][c], l, blk[j][c]);
+      }
+    }
+    __syncthreads();
+    for (int rr = j + 1 + t; rr < nb; rr += 256)
+      if (!s_is_zero(piv)) blk[rr][j] = s_mul(blk[rr][j], s_inv(piv));
+    __syncthreads();
+  }
+  for (int idx = t; idx < nb * nb; idx += 256) {
+    const int r = idx / nb, c = idx - r * nb;
+    Ab[(long)(k + r) * lda + k + c] = blk[r][c];
+  }
+  __syncthreads();
+  invert_diag_block<T>(&blk[0][0], LU_NB + 1, &Dout[0][0], LU_NB + 1, nb);
+  __syncthreads();
+  const int nblk = (n + LU_NB - 1) / LU_NB;
+  T* db = dinv + ((long)b * nblk + k / LU_NB) * LU_NB * LU_NB;
+  for (int idx = t; idx < LU_NB * LU_NB; idx += 256) {
+    const int r = idx / LU_NB, c = idx - r * LU_NB;
+    db[idx] = (r < nb && c < nb) ? Dout[r][c] : s_zero<T>();
+  }
+}
+
+// L21 = A21 inv(U11) for the panel rows below the top block, one row per thread
+template <typename T>
+__global__ __launch_bounds__(256) void tslu_l21_kernel(int n, T* A, long lda, long sA, int k, int nb,
+                                                       const T* dinv) {
+  __shared__ T Ui[LU_NB][LU_NB + 1];
+  const int b = blockIdx.y, t = threadIdx.x;
+  const int nblk = (n + LU_NB - 1) / LU_NB;
+  const T* db = dinv + ((long)b * nblk + k / LU_NB) * LU_NB * LU_NB;
+  for (int idx = t; idx < LU_NB * LU_NB; idx += 256) {
+    const int r = idx / LU_NB, c = idx - r * LU_NB;
+    Ui[r][c] = c >= r ? db[idx] : s_zero<T>();
+  }
+  __syncthreads();
+  const int r = k + nb + blockIdx.x * 256 + t;
+  if (r >= n) return;
+  T* row = A + (long)b * sA + (long)r * lda + k;
+  T a[LU_NB];
+#pragma unroll
+  for (int q = 0; q < LU_NB; ++q) a[q] = q < nb ? row[q] : s_zero<T>();
+#pragma unroll
+  for (int c = 0; c < LU_NB; ++c) {
+    if (c < nb) {
+      T acc = s_zero<T>();
+#pragma unroll
+      for (int q = 0; q <= c; ++q) acc = s_add(acc, s_mul(a[q], Ui[q][c]));
+      row[c] = acc;
+    }
+  }
+}
+
+template <typename T>
+static hipError_t launch_tslu_panel(int n, T* A, long lda, long sA, int k, int nb, int* ipiv, int* info, T* dinv,
+                                    int* swtab, int* cand, int batch, hipStream_t s) {
+  const int m = n - k;
+  int nsets = (m + TS_CHUNK - 1) / TS_CHUNK;
+  int* cur = cand;
+  int* nxt = cand + tslu_cand_ints(n, batch);
+  hipLaunchKernelGGL((tslu_leaf_kernel<T>), dim3(nsets, batch), dim3(256), 0, s, n, A, lda, sA, k, nb, nsets, cur);
+  while (nsets > 1) {
+    const int ngroups = (nsets + 7) / 8;
+    hipLaunchKernelGGL((tslu_reduce_kernel<T>), dim3(ngroups, batch), dim3(256), 0, s, n, A, lda, sA, k, nb, nsets,
+                       cur, ngroups, nxt);
+    int* tmp = cur;
+    cur = nxt;
+    nxt = tmp;
+    nsets = ngroups;
+  }
+  hipLaunchKernelGGL((tslu_finish_kernel<T>), dim3(batch), dim3(256), 0, s, n, A, lda, sA, k, nb, cur, ipiv, info,
+                     dinv, swtab);
+  if (m > nb)
+    hipLaunchKernelGGL((tslu_l21_kernel<T>), dim3(ceil_div(m - nb, 256), batch), dim3(256), 0, s, n, A, lda, sA, k,
+                       nb, dinv);
+  return hipGetLastError();
+}
+
 template <typename T, int RPT>
 static void launch_reg_panel(int n, T* A, long lda, long sA, int k, int nb, int* ipiv, int* info, T* dinv,
                              int* swtab, int batch, hipStream_t s) {
@@ -600,8 +856,14 @@ static void launch_reg_panel(int n, T* A, long lda, long sA, int k, int nb, int*
 
 template <typename T>
 static hipError_t launch_panel(int n, T* A, long lda, long sA, int k, int nb, int* ipiv, int* info, T* dinv,
-                               int* swtab, int batch, hipStream_t s) {
+                               int* swtab, int* cand, int batch, hipStream_t s) {
   const int m = n - k;
+  if (m > LU_TSLU_MIN_ROWS && (long)batch * m <= LU_TSLU_MAX_WORK) {
+    static const std::string nm_ts = std::string("getrf_panel_tslu<") + tname<T>() + ">";
+    prof::Scope ps(s, nm_ts.c_str(), flop_mult<T>() * batch * ((double)m * nb * nb - (double)nb * nb * nb / 3.0),
+                   2.0 * sizeof(T) * (double)m * nb * batch);
+    return launch_tslu_panel<T>(n, A, lda, sA, k, nb, ipiv, info, dinv, swtab, cand, batch, s);
+  }
   const size_t lds = (size_t)(m + LU_NB) * LU_NB * sizeof(T);
   static const std::string nm_rows = std::string("getrf_panel_rows<") + tname<T>() + ">";
   static const std::string nm_lds = std::string("getrf_panel_lds<") + tname<T>() + ">";
@@ -639,8 +901,10 @@ static hipError_t launch_panel(int n, T* A, long lda, long sA, int k, int nb, in
   return hipGetLastError();
 }
 
+// tournament-pivoting candidate lists: two buffers of batch x chunks x NB row indices
+
 size_t getrf_scratch_bytes(int n, int batch) {
-  return ((size_t)batch * n + (size_t)batch * SWTAB) * sizeof(int);
+  return ((size_t)batch * n + (size_t)batch * SWTAB + 2 * tslu_cand_ints(n, batch)) * sizeof(int);
 }
 
 // X[k:k+nb, :] = tri(Dinv_k) X[k:k+nb, :] in place; tri = unit lower (forward)
@@ -692,6 +956,7 @@ hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, T* dinv, int* scr
   if (n <= 0 || batch <= 0) return hipSuccess;
   int* ipiv = scratch;
   int* swtab = scratch + (long)batch * n;
+  int* cand = swtab + (long)batch * SWTAB;
   hipLaunchKernelGGL(fill_int_kernel, dim3(ceil_div(batch, 256)), dim3(256), 0, s, info, (long)batch, 0);
   auto gemm_update = [&](int M, int N, int K, const T* Ap, const T* Bp, T* Cp) -> hipError_t {
     if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
@@ -724,7 +989,7 @@ hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, T* dinv, int* scr
     const int bend = n - k0 < nbo ? n : k0 + nbo;
     for (int k = k0; k < bend; k += LU_NB) {
       const int nb = bend - k < LU_NB ? bend - k : LU_NB;
-      hipError_t e = launch_panel<T>(n, A, lda, sA, k, nb, ipiv, info, dinv, swtab, batch, s);
+      hipError_t e = launch_panel<T>(n, A, lda, sA, k, nb, ipiv, info, dinv, swtab, cand, batch, s);
       if (e != hipSuccess) return e;
       const int others = n - nb;
       if (others > 0) {
