@@ -69,8 +69,9 @@ hipError_t gemm(const GemmArgs<T>& a, hipStream_t s) {
     const bool short_k = a.K <= 64 && a.N <= 256;
     prof::Scope ps(s, (short_k ? nm32 : nm64).c_str(), flops, bytes);
     // Tile choice measured on MI355X over the solver's shapes: 64x64 tiles
-    // with 3 workgroups per CU for general shapes; 64x32 tiles with 2 per CU
-    // for the short-K batched updates of the leaf LU and solves.
+    // with 3 workgroups per CU for general shapes (2 for complex128); 64x32
+    // tiles with 2 per CU for the short-K batched updates of the leaf LU and
+    // solves.  128-wide tiles measured slower (one workgroup per CU).
     if (short_k) {
       int tiles_n = ceil_div(a.N, 32), tiles_m = ceil_div(a.M, 64);
       dim3 grid((unsigned)(tiles_m * tiles_n), gy);
@@ -78,7 +79,10 @@ hipError_t gemm(const GemmArgs<T>& a, hipStream_t s) {
     } else {
       int tiles_n = ceil_div(a.N, 64), tiles_m = ceil_div(a.M, 64);
       dim3 grid((unsigned)(tiles_m * tiles_n), gy);
-      hipLaunchKernelGGL((gemm_mfma_kernel<T, 2, 2, 3>), grid, dim3(256), 0, s, a, tiles_n);
+      if constexpr (sizeof(T) == 16)  // 3 workgroups per CU would force complex128 to spill registers
+        hipLaunchKernelGGL((gemm_mfma_kernel<T, 2, 2, 2>), grid, dim3(256), 0, s, a, tiles_n);
+      else
+        hipLaunchKernelGGL((gemm_mfma_kernel<T, 2, 2, 3>), grid, dim3(256), 0, s, a, tiles_n);
     }
   }
   return hipGetLastError();

@@ -984,7 +984,7 @@ hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, T* dinv, int* scr
   // Large matrices are factored in column blocks of LU_NBO: inside a block the
   // 32-wide panels update only the block's columns; the block row's U12 and the
   // trailing update then run once per block, the latter as a GEMM with K = LU_NBO.
-  const int nbo = n > LU_BLOCKED_MIN ? LU_NBO : n;
+  const int nbo = n > LU_BLOCKED_MIN ? LU_NBO : (n >= 256 ? 128 : n);
   for (int k0 = 0; k0 < n; k0 += nbo) {
     const int bend = n - k0 < nbo ? n : k0 + nbo;
     for (int k = k0; k < bend; k += LU_NB) {
