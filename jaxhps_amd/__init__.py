@@ -13,6 +13,8 @@ from ._pdeproblem import PDEProblem
 from ._build_solver import build_solver
 from ._solve import solve
 from ._lib import HPSLibraryError, HPSSingularError
+from ._subtree_recomp import solve_subtree, upward_pass_subtree, downward_pass_subtree
+from ._device_config import local_solve_chunksize_2D, local_solve_chunksize_3D
 
 __all__ = [
     "Domain",
@@ -22,6 +24,11 @@ __all__ = [
     "PDEProblem",
     "build_solver",
     "solve",
+    "solve_subtree",
+    "local_solve_chunksize_2D",
+    "local_solve_chunksize_3D",
+    "upward_pass_subtree",
+    "downward_pass_subtree",
     "HPSLibraryError",
     "HPSSingularError",
 ]

@@ -213,3 +213,46 @@ def test_nosource_iti_build_then_source_solve(gpu, L):
     assert _rel(_np(v), v0) < 1e-11
     for a, b in zip(g_lst, g0):
         assert _rel(_np(a), b) < 1e-10
+
+
+@pytest.mark.parametrize("kind", ["dtn", "iti"])
+def test_subtree_recomputation_matches_build_solve(gpu, kind):
+    """solve_subtree and upward/downward_pass_subtree (_subtree_recomp.py)
+    give the build_solver + solve solution."""
+    import jaxhps_amd as jh
+
+    print(f"subtree recomputation kind={kind}", flush=True)
+    dom, pde = _problem(8, 6, 4, kind, seed=2)
+    rng = np.random.default_rng(4)
+    nb = dom.boundary_points.shape[0]
+    g = rng.standard_normal(nb) + (1j * rng.standard_normal(nb) if kind == "iti" else 0)
+    jh.build_solver(pde)
+    u_ref = _np(jh.solve(pde, g))
+    _, pde2 = _problem(8, 6, 4, kind, seed=2)
+    u_sub = _np(jh.solve_subtree(pde2, g, subtree_height=2))
+    assert _rel(u_sub, u_ref) < 1e-11
+    _, pde3 = _problem(8, 6, 4, kind, seed=2)
+    T = jh.upward_pass_subtree(pde3, subtree_height=2)
+    assert T.shape == (nb, nb)
+    u_ud = _np(jh.downward_pass_subtree(pde3, g, subtree_height=2))
+    assert _rel(u_ud, u_ref) < 1e-11
+    # all-together branch when the tree is no taller than the subtree height
+    _, pde4 = _problem(8, 6, 4, kind, seed=2)
+    assert _rel(_np(jh.solve_subtree(pde4, g, subtree_height=4)), u_ref) < 1e-11
+
+
+def test_nosource_dtn_build_then_source_solve(gpu):
+    """No-source DtN build (_build_solver.py:261-331 with the DtN stages) then
+    solve(source=f) equals the with-source build."""
+    import jaxhps_amd as jh
+
+    print("nosource dtn", flush=True)
+    dom, pde_src = _problem(8, 6, 3, "dtn", nsrc=2)
+    ones = np.ones(dom.interior_points.shape[:2])
+    pde = jh.PDEProblem(dom, D_xx_coefficients=ones, D_yy_coefficients=ones)
+    Ttop = jh.build_solver(pde, return_top_T=True)
+    Ttop_src = jh.build_solver(pde_src, return_top_T=True)
+    assert _rel(_np(Ttop), _np(Ttop_src)) < 1e-11
+    g = np.random.default_rng(7).standard_normal((dom.boundary_points.shape[0], 2))
+    u = _np(jh.solve(pde, g, source=pde_src.source))
+    assert _rel(u, _np(jh.solve(pde_src, g))) < 1e-11
