@@ -23,6 +23,7 @@
 // getrs: gather rows by perm; per block X_k = inv(L_kk) X_k, then the MFMA
 //   GEMM update below; backward X_k = inv(U_kk) X_k, GEMM update above.
 #include <string>
+#include <utility>
 
 #include "hps_kernels.h"
 #include "hps_prof.h"
@@ -465,20 +466,74 @@ __global__ __launch_bounds__(THREADS) void getrf_panel_reg_kernel(int n, T* A, l
   finish_panel<T>(&Lt[0][0], LU_NB + 1, &Dout[0][0], LU_NB + 1, k, nb, n, b, ipivb, dinv, swtab);
 }
 
+template <typename T>
+struct RowsPanelLDS {
+  using R = typename real_of<T>::type;
+  R red_v[4];
+  int red_i[4];
+  int s_piv;
+  R s_val;
+  T xa[LU_NB], xb[LU_NB], prow[LU_NB];
+};
+
+// One column step J of the rows panel.  J is a template parameter so every
+// access to the row registers x[] and every column comparison is resolved at
+// compile time: no predicated selects over the 32 columns.
+template <typename T, int J>
+__device__ __forceinline__ void rows_panel_step(T (&x)[LU_NB], RowsPanelLDS<T>& sh, int t, int m, int nb, int k,
+                                                int* ipivb, int* info, int b) {
+  using R = typename real_of<T>::type;
+  if (J >= nb) return;
+  R bv = R(-1);
+  int bi = 0x7fffffff;
+  if (t < m && t >= J) argmax_merge(bv, bi, s_abs1(x[J]), t);
+  int pr = block_argmax<R, 256>(bv, bi, sh.red_v, sh.red_i, &sh.s_piv, &sh.s_val);
+  // NaN entries never win a comparison; keep the diagonal then
+  if (pr < J || pr >= m) pr = J;
+  if (t == 0) record_pivot(k, J, pr, sh.s_val == R(0), ipivb, info, b);
+  if (pr != J) {
+    if (t == J)
+#pragma unroll
+      for (int c = 0; c < LU_NB; ++c) sh.xa[c] = x[c];
+    if (t == pr)
+#pragma unroll
+      for (int c = 0; c < LU_NB; ++c) sh.xb[c] = x[c];
+    __syncthreads();
+    if (t == J)
+#pragma unroll
+      for (int c = 0; c < LU_NB; ++c) x[c] = sh.xb[c];
+    if (t == pr)
+#pragma unroll
+      for (int c = 0; c < LU_NB; ++c) x[c] = sh.xa[c];
+  }
+  if (t == J)
+#pragma unroll
+    for (int c = J; c < LU_NB; ++c) sh.prow[c] = x[c];
+  __syncthreads();
+  const T piv = sh.prow[J];
+  if (t < m && t > J && !s_is_zero(piv)) {
+    const T l = s_mul(x[J], s_inv(piv));
+    x[J] = l;
+#pragma unroll
+    for (int c = J + 1; c < LU_NB; ++c) x[c] = s_fnms(x[c], l, sh.prow[c]);
+  }
+}
+
+template <typename T, int... J>
+__device__ __forceinline__ void rows_panel_steps(std::integer_sequence<int, J...>, T (&x)[LU_NB],
+                                                 RowsPanelLDS<T>& sh, int t, int m, int nb, int k, int* ipivb,
+                                                 int* info, int b) {
+  (rows_panel_step<T, J>(x, sh, t, m, nb, k, ipivb, info, b), ...);
+}
 
 // Leaf-sized panels (m <= 256): thread t owns panel row t in registers; the
 // pivot row is broadcast through LDS.  No panel storage in LDS, so several
-// workgroups share a CU.
+// workgroups share a CU.  Columns beyond nb are zero and stay zero (their
+// updates subtract l * 0), so the full unrolled sequence serves partial panels.
 template <typename T>
 __global__ __launch_bounds__(256) void getrf_panel_rows_kernel(int n, T* A, long lda, long sA, int k, int nb,
                                                                int* ipiv, int* info, T* dinv, int* swtab) {
-  using R = typename real_of<T>::type;
-  constexpr int THREADS = 256;
-  __shared__ R red_v[THREADS / 64];
-  __shared__ int red_i[THREADS / 64];
-  __shared__ int s_piv;
-  __shared__ R s_val;
-  __shared__ T xa[LU_NB], xb[LU_NB], prow[LU_NB];
+  __shared__ RowsPanelLDS<T> sh;
   __shared__ T Dblk[LU_NB][LU_NB + 1];
   __shared__ T Dout[LU_NB][LU_NB + 1];
   const int b = blockIdx.x;
@@ -487,56 +542,10 @@ __global__ __launch_bounds__(256) void getrf_panel_rows_kernel(int n, T* A, long
   T* P = A + (long)b * sA + (long)k * lda + k;
   int* ipivb = ipiv + (long)b * n;
   const bool live = t < m;
-  // x holds this thread's row; every access uses a compile-time index (the
-  // column loop is a runtime loop and selects column j by predication), so
-  // the row stays in registers
   T x[LU_NB];
 #pragma unroll
   for (int c = 0; c < LU_NB; ++c) x[c] = (live && c < nb) ? P[(long)t * lda + c] : s_zero<T>();
-  for (int j = 0; j < nb; ++j) {
-    T xj = s_zero<T>();
-#pragma unroll
-    for (int c = 0; c < LU_NB; ++c)
-      if (c == j) xj = x[c];
-    R bv = R(-1);
-    int bi = 0x7fffffff;
-    if (live && t >= j) argmax_merge(bv, bi, s_abs1(xj), t);
-    int pr = block_argmax<R, THREADS>(bv, bi, red_v, red_i, &s_piv, &s_val);
-    if (pr < j || pr >= m) pr = j;
-    if (t == 0) record_pivot(k, j, pr, s_val == R(0), ipivb, info, b);
-    if (pr != j) {
-      if (t == j)
-#pragma unroll
-        for (int c = 0; c < LU_NB; ++c) xa[c] = x[c];
-      if (t == pr)
-#pragma unroll
-        for (int c = 0; c < LU_NB; ++c) xb[c] = x[c];
-      __syncthreads();
-      if (t == j)
-#pragma unroll
-        for (int c = 0; c < LU_NB; ++c) x[c] = xb[c];
-      if (t == pr)
-#pragma unroll
-        for (int c = 0; c < LU_NB; ++c) x[c] = xa[c];
-    }
-    if (t == j)
-#pragma unroll
-      for (int c = 0; c < LU_NB; ++c) prow[c] = x[c];
-    __syncthreads();
-    const T piv = prow[j];
-    if (live && t > j && !s_is_zero(piv)) {
-      T xs = s_zero<T>();
-#pragma unroll
-      for (int c = 0; c < LU_NB; ++c)
-        if (c == j) xs = x[c];
-      const T l = s_mul(xs, s_inv(piv));
-#pragma unroll
-      for (int c = 0; c < LU_NB; ++c) {
-        if (c == j) x[c] = l;
-        if (c > j && c < nb) x[c] = s_fnms(x[c], l, prow[c]);
-      }
-    }
-  }
+  rows_panel_steps<T>(std::make_integer_sequence<int, LU_NB>{}, x, sh, t, m, nb, k, ipivb, info, b);
   if (live)
 #pragma unroll
     for (int c = 0; c < LU_NB; ++c)
