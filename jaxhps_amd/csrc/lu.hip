@@ -45,11 +45,11 @@ template <> __device__ __forceinline__ c64 make_conj<c64>(c64 x) { return c64{x.
 
 constexpr int LU_NBO = 256;           // column block of the blocked factorisation
 constexpr int LU_BLOCKED_MIN = 768;   // matrices larger than this are factored in column blocks
-// tournament pivoting for panels taller than this when batch x rows is small
-// enough that one workgroup per matrix would leave most of the GPU idle
-constexpr int LU_TSLU_MIN_ROWS = 1024;
 constexpr int TS_CHUNK = 256;  // rows per first-round tournament chunk
 static size_t tslu_cand_ints(int n, int batch) { return (size_t)batch * ((n + TS_CHUNK - 1) / TS_CHUNK) * LU_NB; }
+// tournament pivoting for panels taller than the register panel holds when
+// batch x rows is small enough that one workgroup per matrix would leave most
+// of the GPU idle
 constexpr long LU_TSLU_MAX_WORK = 64L * 1024;
 
 __global__ void fill_int_kernel(int* p, long n, int v) {
@@ -466,74 +466,114 @@ __global__ __launch_bounds__(THREADS) void getrf_panel_reg_kernel(int n, T* A, l
   finish_panel<T>(&Lt[0][0], LU_NB + 1, &Dout[0][0], LU_NB + 1, k, nb, n, b, ipivb, dinv, swtab);
 }
 
-template <typename T>
-struct RowsPanelLDS {
+// Register panel state in LDS: each wave's argmax candidate (|value|,
+// position, and its row from the column on), double-buffered by column parity
+// so a column needs a single barrier.
+template <typename T, int NW>
+struct PanelLDS {
   using R = typename real_of<T>::type;
-  R red_v[4];
-  int red_i[4];
-  int s_piv;
-  R s_val;
-  T xa[LU_NB], xb[LU_NB], prow[LU_NB];
+  R cv[2][NW];
+  int cp[2][NW];
+  T crow[2][NW][LU_NB];
 };
 
-// One column step J of the rows panel.  J is a template parameter so every
-// access to the row registers x[] and every column comparison is resolved at
-// compile time: no predicated selects over the 32 columns.
-template <typename T, int J>
-__device__ __forceinline__ void rows_panel_step(T (&x)[LU_NB], RowsPanelLDS<T>& sh, int t, int m, int nb, int k,
-                                                int* ipivb, int* info, int b) {
+// One column step J of partial pivoting over rows held in registers, RPT rows
+// per thread of a workgroup of NW waves.  Rows never move between threads: each
+// row's current position in the partially pivoted panel is tracked in pos[]
+// (a row swap is two position updates).  Each wave publishes its best
+// candidate row; after the one barrier every thread picks the winner among the
+// NW candidates and reads the pivot row from its slot.  J is a template parameter so
+// every access to x[] is resolved at compile time.  on_pivot(J, pq, |pivot|,
+// mask) runs after the step with the pivot's position pq (J when no candidate
+// is left: the column is NaN or no live row remains) and the mask of this
+// thread's rows that held it.
+template <typename T, int RPT, int NW, int J, class F>
+__device__ __forceinline__ void panel_step(T (&x)[RPT][LU_NB], int (&pos)[RPT], const bool (&live)[RPT],
+                                           PanelLDS<T, NW>& sh, int nb, F& on_pivot) {
   using R = typename real_of<T>::type;
+  constexpr int NONE = 0x7fffffff;
+  constexpr int par = J & 1;
   if (J >= nb) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   R bv = R(-1);
-  int bi = 0x7fffffff;
-  if (t < m && t >= J) argmax_merge(bv, bi, s_abs1(x[J]), t);
-  int pr = block_argmax<R, 256>(bv, bi, sh.red_v, sh.red_i, &sh.s_piv, &sh.s_val);
-  // NaN entries never win a comparison; keep the diagonal then
-  if (pr < J || pr >= m) pr = J;
-  if (t == 0) record_pivot(k, J, pr, sh.s_val == R(0), ipivb, info, b);
-  if (pr != J) {
-    if (t == J)
+  int bp = NONE;
 #pragma unroll
-      for (int c = 0; c < LU_NB; ++c) sh.xa[c] = x[c];
-    if (t == pr)
+  for (int r = 0; r < RPT; ++r)
+    if (live[r] && pos[r] >= J) argmax_merge(bv, bp, s_abs1(x[r][J]), pos[r]);
+  // ties go to the lower position, as in a sequential scan; NaN never wins
 #pragma unroll
-      for (int c = 0; c < LU_NB; ++c) sh.xb[c] = x[c];
-    __syncthreads();
-    if (t == J)
-#pragma unroll
-      for (int c = 0; c < LU_NB; ++c) x[c] = sh.xb[c];
-    if (t == pr)
-#pragma unroll
-      for (int c = 0; c < LU_NB; ++c) x[c] = sh.xa[c];
+  for (int off = 32; off > 0; off >>= 1) {
+    const R ov = __shfl_xor(bv, off, 64);
+    const int op = __shfl_xor(bp, off, 64);
+    argmax_merge(bv, bp, ov, op);
   }
-  if (t == J)
 #pragma unroll
-    for (int c = J; c < LU_NB; ++c) sh.prow[c] = x[c];
+  for (int r = 0; r < RPT; ++r)
+    if (pos[r] == bp) {
+#pragma unroll
+      for (int c = J; c < LU_NB; ++c) sh.crow[par][wave][c] = x[r][c];
+    }
+  if (lane == 0) {
+    sh.cv[par][wave] = bv;
+    sh.cp[par][wave] = bp;
+  }
   __syncthreads();
-  const T piv = sh.prow[J];
-  if (t < m && t > J && !s_is_zero(piv)) {
-    const T l = s_mul(x[J], s_inv(piv));
-    x[J] = l;
+  R fv = sh.cv[par][0];
+  int fp = sh.cp[par][0], fw = 0;
 #pragma unroll
-    for (int c = J + 1; c < LU_NB; ++c) x[c] = s_fnms(x[c], l, sh.prow[c]);
+  for (int w = 1; w < NW; ++w) {
+    const R v = sh.cv[par][w];
+    const int q = sh.cp[par][w];
+    if (v > fv || (v == fv && q < fp)) {
+      fv = v;
+      fp = q;
+      fw = w;
+    }
+  }
+  // without a candidate the row at position J stays; the column's updates
+  // are then NaN whatever the pivot slot holds
+  const int pq = fp != NONE ? fp : J;
+  unsigned mask = 0;
+#pragma unroll
+  for (int r = 0; r < RPT; ++r) {
+    if (pos[r] == pq) {
+      mask |= 1u << r;
+      pos[r] = J;
+    } else if (pos[r] == J) {
+      pos[r] = pq;
+    }
+  }
+  on_pivot(J, pq, fv, mask);
+  const T* prow = sh.crow[par][fw];
+  const T piv = prow[J];
+  if (!s_is_zero(piv)) {
+    const T inv = s_inv(piv);
+#pragma unroll
+    for (int r = 0; r < RPT; ++r)
+      if (live[r] && pos[r] > J) {
+        const T l = s_mul(x[r][J], inv);
+        x[r][J] = l;
+#pragma unroll
+        for (int c = J + 1; c < LU_NB; ++c) x[r][c] = s_fnms(x[r][c], l, prow[c]);
+      }
   }
 }
 
-template <typename T, int... J>
-__device__ __forceinline__ void rows_panel_steps(std::integer_sequence<int, J...>, T (&x)[LU_NB],
-                                                 RowsPanelLDS<T>& sh, int t, int m, int nb, int k, int* ipivb,
-                                                 int* info, int b) {
-  (rows_panel_step<T, J>(x, sh, t, m, nb, k, ipivb, info, b), ...);
+template <typename T, int RPT, int NW, class F, int... J>
+__device__ __forceinline__ void panel_steps(std::integer_sequence<int, J...>, T (&x)[RPT][LU_NB], int (&pos)[RPT],
+                                            const bool (&live)[RPT], PanelLDS<T, NW>& sh, int nb, F& on_pivot) {
+  (panel_step<T, RPT, NW, J>(x, pos, live, sh, nb, on_pivot), ...);
 }
 
-// Leaf-sized panels (m <= 256): thread t owns panel row t in registers; the
-// pivot row is broadcast through LDS.  No panel storage in LDS, so several
-// workgroups share a CU.  Columns beyond nb are zero and stay zero (their
-// updates subtract l * 0), so the full unrolled sequence serves partial panels.
-template <typename T>
-__global__ __launch_bounds__(256) void getrf_panel_rows_kernel(int n, T* A, long lda, long sA, int k, int nb,
-                                                               int* ipiv, int* info, T* dinv, int* swtab) {
-  __shared__ RowsPanelLDS<T> sh;
+// Register panels (m <= THREADS RPT): thread t owns panel rows t + THREADS r
+// in registers for the whole factorisation and writes each to its final
+// position at the end.  No panel storage in LDS.  Columns beyond nb are zero
+// and stay zero (their updates subtract l * 0).
+template <typename T, int RPT, int THREADS>
+__global__ __launch_bounds__(THREADS) void getrf_panel_rows_kernel(int n, T* A, long lda, long sA, int k, int nb,
+                                                                   int* ipiv, int* info, T* dinv, int* swtab) {
+  using R = typename real_of<T>::type;
+  __shared__ PanelLDS<T, THREADS / 64> sh;
   __shared__ T Dblk[LU_NB][LU_NB + 1];
   __shared__ T Dout[LU_NB][LU_NB + 1];
   const int b = blockIdx.x;
@@ -541,19 +581,31 @@ __global__ __launch_bounds__(256) void getrf_panel_rows_kernel(int n, T* A, long
   const int m = n - k;
   T* P = A + (long)b * sA + (long)k * lda + k;
   int* ipivb = ipiv + (long)b * n;
-  const bool live = t < m;
-  T x[LU_NB];
+  T x[RPT][LU_NB];
+  int pos[RPT];
+  bool live[RPT];
 #pragma unroll
-  for (int c = 0; c < LU_NB; ++c) x[c] = (live && c < nb) ? P[(long)t * lda + c] : s_zero<T>();
-  rows_panel_steps<T>(std::make_integer_sequence<int, LU_NB>{}, x, sh, t, m, nb, k, ipivb, info, b);
-  if (live)
+  for (int r = 0; r < RPT; ++r) {
+    pos[r] = t + THREADS * r;
+    live[r] = pos[r] < m;
 #pragma unroll
-    for (int c = 0; c < LU_NB; ++c)
-      if (c < nb) P[(long)t * lda + c] = x[c];
-  if (t < nb)
+    for (int c = 0; c < LU_NB; ++c) x[r][c] = (live[r] && c < nb) ? P[(long)pos[r] * lda + c] : s_zero<T>();
+  }
+  auto on_pivot = [&](int J, int pq, R fv, unsigned) {
+    if (t == 0) record_pivot(k, J, pq, fv == R(0), ipivb, info, b);
+  };
+  panel_steps<T, RPT, THREADS / 64>(std::make_integer_sequence<int, LU_NB>{}, x, pos, live, sh, nb, on_pivot);
 #pragma unroll
-    for (int c = 0; c < LU_NB; ++c)
-      if (c < nb) Dblk[t][c] = x[c];
+  for (int r = 0; r < RPT; ++r) {
+    if (live[r])
+#pragma unroll
+      for (int c = 0; c < LU_NB; ++c)
+        if (c < nb) P[(long)pos[r] * lda + c] = x[r][c];
+    if (pos[r] < nb)
+#pragma unroll
+      for (int c = 0; c < LU_NB; ++c)
+        if (c < nb) Dblk[pos[r]][c] = x[r][c];
+  }
   finish_panel<T>(&Dblk[0][0], LU_NB + 1, &Dout[0][0], LU_NB + 1, k, nb, n, b, ipivb, dinv, swtab);
 }
 
@@ -619,47 +671,19 @@ __global__ __launch_bounds__(256) void getrf_swap_u12_kernel(int n, T* A, long l
 // ---------------------------------------------------------------------------
 
 // Partial pivoting over the rows held one per thread (x = row, valid if
-// row_id >= 0): the nb pivot rows' ids, in order, to out[0..nb).
+// row_id >= 0): the nb pivot rows' ids, in order, to out[0..nb) (-1 once no
+// valid row remains).
 template <typename T>
-__device__ void tslu_select(T (&x)[LU_NB], int row_id, int nb, int* out) {
+__device__ __forceinline__ void tslu_select(T (&x)[1][LU_NB], int row_id, int nb, int* out) {
   using R = typename real_of<T>::type;
-  constexpr int THREADS = 256;
-  __shared__ R red_v[THREADS / 64];
-  __shared__ int red_i[THREADS / 64];
-  __shared__ int s_piv;
-  __shared__ R s_val;
-  __shared__ T prow[LU_NB];
-  const int t = threadIdx.x;
-  bool chosen = row_id < 0;
-  for (int j = 0; j < nb; ++j) {
-    T xj = s_zero<T>();
-#pragma unroll
-    for (int c = 0; c < LU_NB; ++c)
-      if (c == j) xj = x[c];
-    R bv = R(-1);
-    int bi = 0x7fffffff;
-    if (!chosen) argmax_merge(bv, bi, s_abs1(xj), t);
-    const int w = block_argmax<R, THREADS>(bv, bi, red_v, red_i, &s_piv, &s_val);
-    const bool have = w >= 0 && w < THREADS;  // no remaining row (short chunk): w is the sentinel
-    if (have && t == w) {
-      out[j] = row_id;
-#pragma unroll
-      for (int c = 0; c < LU_NB; ++c) prow[c] = x[c];
-      chosen = true;
-    }
-    if (!have && t == 0) out[j] = -1;
-    __syncthreads();
-    if (have && !chosen) {
-      const T piv = prow[j];
-      if (!s_is_zero(piv)) {
-        const T l = s_mul(xj, s_inv(piv));
-#pragma unroll
-        for (int c = 0; c < LU_NB; ++c)
-          if (c > j && c < nb) x[c] = s_fnms(x[c], l, prow[c]);
-      }
-    }
-    __syncthreads();
-  }
+  __shared__ PanelLDS<T, 4> sh;
+  int pos[1] = {(int)threadIdx.x};
+  const bool live[1] = {row_id >= 0};
+  auto on_pivot = [&](int J, int, R, unsigned mask) {
+    if (mask) out[J] = row_id;
+  };
+  panel_steps<T, 1, 4>(std::make_integer_sequence<int, LU_NB>{}, x, pos, live, sh, nb, on_pivot);
+  __syncthreads();
 }
 
 // Round 0: chunk c of the panel rows [k, n) -> cand_out[b][c][0..nb)
@@ -670,10 +694,10 @@ __global__ __launch_bounds__(256) void tslu_leaf_kernel(int n, const T* A, long 
   const int b = blockIdx.y, c = blockIdx.x, t = threadIdx.x;
   const int r = k + c * TS_CHUNK + t;
   const T* Ab = A + (long)b * sA;
-  T x[LU_NB];
+  T x[1][LU_NB];
   const bool live = r < n;
 #pragma unroll
-  for (int q = 0; q < LU_NB; ++q) x[q] = (live && q < nb) ? Ab[(long)r * lda + k + q] : s_zero<T>();
+  for (int q = 0; q < LU_NB; ++q) x[0][q] = (live && q < nb) ? Ab[(long)r * lda + k + q] : s_zero<T>();
   tslu_select<T>(x, live ? r : -1, nb, out);
   if (t < nb) cand_out[((long)b * nchunks + c) * LU_NB + t] = out[t];
 }
@@ -689,9 +713,9 @@ __global__ __launch_bounds__(256) void tslu_reduce_kernel(int n, const T* A, lon
   int r = -1;
   if (set < nsets && slot < nb) r = cand_in[((long)b * nsets + set) * LU_NB + slot];
   const T* Ab = A + (long)b * sA;
-  T x[LU_NB];
+  T x[1][LU_NB];
 #pragma unroll
-  for (int q = 0; q < LU_NB; ++q) x[q] = (r >= 0 && q < nb) ? Ab[(long)r * lda + k + q] : s_zero<T>();
+  for (int q = 0; q < LU_NB; ++q) x[0][q] = (r >= 0 && q < nb) ? Ab[(long)r * lda + k + q] : s_zero<T>();
   tslu_select<T>(x, r, nb, out);
   if (t < nb) cand_out[((long)b * ngroups + g) * LU_NB + t] = out[t];
 }
@@ -863,27 +887,45 @@ static void launch_reg_panel(int n, T* A, long lda, long sA, int k, int nb, int*
                      ipiv, info, dinv, swtab);
 }
 
+// rows the register panel holds: one per thread, up to 1024 threads (512 for
+// complex128, whose 32-column rows take 128 registers)
+template <typename T>
+static int rows_panel_max() {
+  return sizeof(T) >= 16 ? 512 : 1024;
+}
+
 template <typename T>
 static hipError_t launch_panel(int n, T* A, long lda, long sA, int k, int nb, int* ipiv, int* info, T* dinv,
                                int* swtab, int* cand, int batch, hipStream_t s) {
   const int m = n - k;
-  if (m > LU_TSLU_MIN_ROWS && (long)batch * m <= LU_TSLU_MAX_WORK) {
+  const double pf = flop_mult<T>() * batch * ((double)m * nb * nb - (double)nb * nb * nb / 3.0);
+  if (m <= rows_panel_max<T>()) {
+    static const std::string nm_rows = std::string("getrf_panel_rows<") + tname<T>() + ">";
+    prof::Scope ps(s, nm_rows.c_str(), pf, 2.0 * sizeof(T) * (double)m * nb * batch);
+    if (m <= 256)
+      hipLaunchKernelGGL((getrf_panel_rows_kernel<T, 1, 256>), dim3(batch), dim3(256), 0, s, n, A, lda, sA, k, nb,
+                         ipiv, info, dinv, swtab);
+    else if (m <= 512)
+      hipLaunchKernelGGL((getrf_panel_rows_kernel<T, 1, 512>), dim3(batch), dim3(512), 0, s, n, A, lda, sA, k, nb,
+                         ipiv, info, dinv, swtab);
+    else if constexpr (sizeof(T) < 16)
+      hipLaunchKernelGGL((getrf_panel_rows_kernel<T, 1, 1024>), dim3(batch), dim3(1024), 0, s, n, A, lda, sA, k, nb,
+                         ipiv, info, dinv, swtab);
+    else
+      return hipErrorInvalidValue;  // not reached: rows_panel_max<c128>() is 512
+    return hipGetLastError();
+  }
+  if ((long)batch * m <= LU_TSLU_MAX_WORK) {
     static const std::string nm_ts = std::string("getrf_panel_tslu<") + tname<T>() + ">";
-    prof::Scope ps(s, nm_ts.c_str(), flop_mult<T>() * batch * ((double)m * nb * nb - (double)nb * nb * nb / 3.0),
-                   2.0 * sizeof(T) * (double)m * nb * batch);
+    prof::Scope ps(s, nm_ts.c_str(), pf, 2.0 * sizeof(T) * (double)m * nb * batch);
     return launch_tslu_panel<T>(n, A, lda, sA, k, nb, ipiv, info, dinv, swtab, cand, batch, s);
   }
   const size_t lds = (size_t)(m + LU_NB) * LU_NB * sizeof(T);
-  static const std::string nm_rows = std::string("getrf_panel_rows<") + tname<T>() + ">";
   static const std::string nm_lds = std::string("getrf_panel_lds<") + tname<T>() + ">";
   static const std::string nm_reg = std::string("getrf_panel_reg<") + tname<T>() + ">";
-  const double pf = flop_mult<T>() * batch * ((double)m * nb * nb - (double)nb * nb * nb / 3.0);
-  prof::Scope ps(s, (m <= 256 ? nm_rows : lds <= LU_PANEL_LDS_MAX ? nm_lds : nm_reg).c_str(), pf,
+  prof::Scope ps(s, (lds <= LU_PANEL_LDS_MAX ? nm_lds : nm_reg).c_str(), pf,
                  2.0 * sizeof(T) * (double)m * nb * batch);
-  if (m <= 256) {
-    hipLaunchKernelGGL((getrf_panel_rows_kernel<T>), dim3(batch), dim3(256), 0, s, n, A, lda, sA, k, nb, ipiv, info,
-                       dinv, swtab);
-  } else if (lds <= LU_PANEL_LDS_MAX) {
+  if (lds <= LU_PANEL_LDS_MAX) {
     static bool attr_set = false;
     if (!attr_set) {
       hipFuncSetAttribute((const void*)getrf_panel_lds_kernel<T, 256>, hipFuncAttributeMaxDynamicSharedMemorySize,

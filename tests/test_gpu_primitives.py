@@ -56,7 +56,8 @@ def test_gemm_broadcast_operand(gpu, dtype):
 
 @pytest.mark.parametrize("dtype", DTYPES)
 @pytest.mark.parametrize(
-    "n,batch", [(5, 3), (36, 4), (196, 8), (256, 2), (33, 1), (700, 1), (300, 2), (1100, 1), (2500, 2), (4200, 1)]
+    "n,batch",
+    [(5, 3), (36, 4), (196, 8), (256, 2), (33, 1), (700, 1), (300, 2), (1100, 1), (2500, 2), (4200, 1), (600, 110)],
 )
 def test_getrf_getrs_solve(gpu, dtype, n, batch):
     from jaxhps_amd import _lib
