@@ -126,7 +126,8 @@ __global__ __launch_bounds__(256) void perm_from_ipiv_kernel(int n, const int* i
 }
 
 // After the panel: (row, source) table of its net row permutation, from the
-// swap sequence ipiv[k .. k+nb).  Single thread.
+// swap sequence ipiv[k .. k+nb), into tab (LDS: count, then pairs).  Single
+// thread.
 __device__ void build_swap_table(int k, int nb, const int* ipivb, int* tab) {
   // working lists in LDS: a per-thread array indexed at run time would live in scratch
   __shared__ int rows[2 * LU_NB], src[2 * LU_NB];
@@ -162,11 +163,11 @@ __device__ void build_swap_table(int k, int nb, const int* ipivb, int* tab) {
 
 // Invert the factored nb x nb diagonal block D (row stride ldd) into the
 // packed form out (row stride ldo): strict lower = inv(L11), upper = inv(U11).
-// Thread c < nb forms column c of inv(L11), thread 64 + c column c of
+// Thread c < nb forms column c of inv(L11), thread UOFF + c column c of
 // inv(U11); each substitution works in its own column of `out` (LDS), so no
 // per-thread array is indexed at run time (those would live in scratch).
-template <typename T>
-__device__ void invert_diag_block(const T* D, int ldd, T* out, int ldo, int nb) {
+template <typename T, int UOFF>
+__device__ __forceinline__ void invert_diag_block(const T* D, int ldd, T* out, int ldo, int nb) {
   const int t = threadIdx.x;
   if (t < nb) {
     const int c = t;
@@ -176,8 +177,8 @@ __device__ void invert_diag_block(const T* D, int ldd, T* out, int ldo, int nb) 
       for (int j = c + 1; j < i; ++j) acc = s_fnms(acc, D[i * ldd + j], out[j * ldo + c]);
       out[i * ldo + c] = acc;
     }
-  } else if (t >= 64 && t < 64 + nb) {
-    const int c = t - 64;
+  } else if (t >= UOFF && t < UOFF + nb) {
+    const int c = t - UOFF;
     for (int i = c; i >= 0; --i) {
       T acc = i == c ? s_one<T>() : s_zero<T>();
       for (int j = i + 1; j <= c; ++j) acc = s_fnms(acc, D[i * ldd + j], out[j * ldo + c]);
@@ -186,14 +187,16 @@ __device__ void invert_diag_block(const T* D, int ldd, T* out, int ldo, int nb) 
   }
 }
 
-// Diagonal-block inverse -> dinv, and the panel's swap table.  D is the final
-// factored diagonal block in LDS; Out is LDS scratch of nb rows.
-template <typename T>
-__device__ void finish_panel(const T* D, int ldd, T* Out, int ldo, int k, int nb, int n, int b, const int* ipivb,
-                             T* dinv, int* swtab) {
+// Diagonal-block inverse -> dinv, and the panel's swap table tab (LDS:
+// count, then (row, source) pairs) -> swtab.  D is the final factored
+// diagonal block in LDS; Out is LDS scratch of nb rows.
+template <typename T, int UOFF>
+__device__ __forceinline__ void finish_panel_tab(const T* D, int ldd, T* Out, int ldo, int k, int nb, int n, int b,
+                                                 const int* tab, T* dinv, int* swtab) {
   __syncthreads();
-  invert_diag_block<T>(D, ldd, Out, ldo, nb);
-  if (threadIdx.x == 128) build_swap_table(k, nb, ipivb, swtab + (long)b * SWTAB);
+  invert_diag_block<T, UOFF>(D, ldd, Out, ldo, nb);
+  const int cnt = tab[0] < 2 * LU_NB ? tab[0] : 2 * LU_NB;
+  for (int i = threadIdx.x; i < 1 + 2 * cnt; i += blockDim.x) swtab[(long)b * SWTAB + i] = i == 0 ? cnt : tab[i];
   __syncthreads();
   const int nblk = (n + LU_NB - 1) / LU_NB;
   T* db = dinv + ((long)b * nblk + k / LU_NB) * LU_NB * LU_NB;
@@ -201,6 +204,17 @@ __device__ void finish_panel(const T* D, int ldd, T* Out, int ldo, int k, int nb
     int r = idx / LU_NB, c = idx - r * LU_NB;
     db[idx] = (r < nb && c < nb) ? Out[r * ldo + c] : s_zero<T>();
   }
+}
+
+// finish_panel_tab with the swap table built from ipiv, for panels that swap
+// rows explicitly (at least 96 threads).
+template <typename T>
+__device__ __forceinline__ void finish_panel(const T* D, int ldd, T* Out, int ldo, int k, int nb, int n, int b,
+                                             const int* ipivb, T* dinv, int* swtab) {
+  __shared__ int tab[SWTAB];
+  __syncthreads();
+  if (threadIdx.x == 0) build_swap_table(k, nb, ipivb, tab);
+  finish_panel_tab<T, 64>(D, ldd, Out, ldo, k, nb, n, b, tab, dinv, swtab);
 }
 
 // Short panels: the whole (n-k) x nb panel in LDS.
@@ -466,6 +480,71 @@ __global__ __launch_bounds__(THREADS) void getrf_panel_reg_kernel(int n, T* A, l
   finish_panel<T>(&Lt[0][0], LU_NB + 1, &Dout[0][0], LU_NB + 1, k, nb, n, b, ipivb, dinv, swtab);
 }
 
+// ---------------------------------------------------------------------------
+// Wave-wide argmax of (|value|, position) pairs with DPP lane moves (no LDS
+// round trips): quad exchanges, row shifts by 4 and 8, then the row
+// broadcasts; the wave's winner ends in lane 63 and is read back as a
+// uniform value.  Lanes whose DPP source is outside the pattern keep the
+// identity (-1, NONE), which never wins a merge.
+// ---------------------------------------------------------------------------
+template <int CTRL, int RM>
+__device__ __forceinline__ int dpp_int(int old, int v) {
+  return __builtin_amdgcn_update_dpp(old, v, CTRL, RM, 0xf, false);
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ float dpp_move(float old, float v) {
+  return __int_as_float(dpp_int<CTRL, RM>(__float_as_int(old), __float_as_int(v)));
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ double dpp_move(double old, double v) {
+  const long long o = __double_as_longlong(old), x = __double_as_longlong(v);
+  const int lo = dpp_int<CTRL, RM>((int)o, (int)x);
+  const int hi = dpp_int<CTRL, RM>((int)(o >> 32), (int)(x >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <int CTRL, int RM, typename R>
+__device__ __forceinline__ void dpp_argmax_step(R& bv, int& bp) {
+  const R ov = dpp_move<CTRL, RM>(R(-1), bv);
+  const int op = dpp_int<CTRL, RM>(0x7fffffff, bp);
+  argmax_merge(bv, bp, ov, op);
+}
+__device__ __forceinline__ float readlane63(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
+__device__ __forceinline__ double readlane63(double v) {
+  const long long x = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)x, 63), hi = __builtin_amdgcn_readlane((int)(x >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <typename R>
+__device__ __forceinline__ void wave_argmax(R& bv, int& bp) {
+  dpp_argmax_step<0xb1, 0xf>(bv, bp);   // quad_perm [1,0,3,2]
+  dpp_argmax_step<0x4e, 0xf>(bv, bp);   // quad_perm [2,3,0,1]
+  dpp_argmax_step<0x114, 0xf>(bv, bp);  // row_shr:4
+  dpp_argmax_step<0x118, 0xf>(bv, bp);  // row_shr:8
+  dpp_argmax_step<0x142, 0xa>(bv, bp);  // row_bcast:15 into rows 1, 3
+  dpp_argmax_step<0x143, 0xc>(bv, bp);  // row_bcast:31 into rows 2, 3
+  bv = readlane63(bv);
+  bp = __builtin_amdgcn_readlane(bp, 63);
+}
+
+// 1 / a for a value every lane holds alike: the branch of Smith's algorithm is
+// taken on the first lane's comparison, so only one side executes.
+__device__ __forceinline__ double s_inv_uniform(double a) { return 1.0 / a; }
+__device__ __forceinline__ float s_inv_uniform(float a) { return 1.0f / a; }
+template <typename C>
+__device__ __forceinline__ C s_inv_uniform_c(C a) {
+  using R = decltype(a.re);
+  if (__builtin_amdgcn_readfirstlane((int)(fabs(a.re) >= fabs(a.im)))) {
+    const R r = a.im / a.re, d = a.re + a.im * r;
+    const R id = R(1) / d;
+    return C{id, -r * id};
+  }
+  const R r = a.re / a.im, d = a.re * r + a.im;
+  const R id = R(1) / d;
+  return C{r * id, -id};
+}
+__device__ __forceinline__ c128 s_inv_uniform(c128 a) { return s_inv_uniform_c(a); }
+__device__ __forceinline__ c64 s_inv_uniform(c64 a) { return s_inv_uniform_c(a); }
+
 // Register panel state in LDS: each wave's argmax candidate (|value|,
 // position, and its row from the column on), double-buffered by column parity
 // so a column needs a single barrier.
@@ -501,12 +580,7 @@ __device__ __forceinline__ void panel_step(T (&x)[RPT][LU_NB], int (&pos)[RPT], 
   for (int r = 0; r < RPT; ++r)
     if (live[r] && pos[r] >= J) argmax_merge(bv, bp, s_abs1(x[r][J]), pos[r]);
   // ties go to the lower position, as in a sequential scan; NaN never wins
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const R ov = __shfl_xor(bv, off, 64);
-    const int op = __shfl_xor(bp, off, 64);
-    argmax_merge(bv, bp, ov, op);
-  }
+  wave_argmax(bv, bp);
 #pragma unroll
   for (int r = 0; r < RPT; ++r)
     if (pos[r] == bp) {
@@ -547,7 +621,7 @@ __device__ __forceinline__ void panel_step(T (&x)[RPT][LU_NB], int (&pos)[RPT], 
   const T* prow = sh.crow[par][fw];
   const T piv = prow[J];
   if (!s_is_zero(piv)) {
-    const T inv = s_inv(piv);
+    const T inv = s_inv_uniform(piv);
 #pragma unroll
     for (int r = 0; r < RPT; ++r)
       if (live[r] && pos[r] > J) {
@@ -595,18 +669,31 @@ __global__ __launch_bounds__(THREADS) void getrf_panel_rows_kernel(int n, T* A, 
     if (t == 0) record_pivot(k, J, pq, fv == R(0), ipivb, info, b);
   };
   panel_steps<T, RPT, THREADS / 64>(std::make_integer_sequence<int, LU_NB>{}, x, pos, live, sh, nb, on_pivot);
+  // net row permutation of the panel: every row whose position changed
+  __shared__ int tab[SWTAB];
+  if (t == 0) tab[0] = 0;
+  __syncthreads();
 #pragma unroll
   for (int r = 0; r < RPT; ++r) {
-    if (live[r])
+    const int orig = t + THREADS * r;
+    if (live[r]) {
 #pragma unroll
       for (int c = 0; c < LU_NB; ++c)
         if (c < nb) P[(long)pos[r] * lda + c] = x[r][c];
+      if (pos[r] != orig) {
+        // at most 2 nb rows move (two per swap)
+        const int i = atomicAdd(&tab[0], 1);
+        if (i < 2 * LU_NB) {
+          tab[1 + 2 * i] = k + pos[r];
+          tab[2 + 2 * i] = k + orig;
+        }
+      }
+    }
     if (pos[r] < nb)
 #pragma unroll
-      for (int c = 0; c < LU_NB; ++c)
-        if (c < nb) Dblk[pos[r]][c] = x[r][c];
+      for (int c = 0; c < LU_NB; ++c) Dblk[pos[r]][c] = c < nb ? x[r][c] : s_zero<T>();
   }
-  finish_panel<T>(&Dblk[0][0], LU_NB + 1, &Dout[0][0], LU_NB + 1, k, nb, n, b, ipivb, dinv, swtab);
+  finish_panel_tab<T, 64>(&Dblk[0][0], LU_NB + 1, &Dout[0][0], LU_NB + 1, k, nb, n, b, tab, dinv, swtab);
 }
 
 // Apply the panel's row permutation table to columns outside the panel and,
@@ -720,109 +807,117 @@ __global__ __launch_bounds__(256) void tslu_reduce_kernel(int n, const T* A, lon
   if (t < nb) cand_out[((long)b * ngroups + g) * LU_NB + t] = out[t];
 }
 
-// Winners -> ipiv, row swaps inside the panel columns, unpivoted LU of the top
-// block, dinv and the swap table for the other columns.  One workgroup per matrix.
+// Unpivoted elimination step J of the top block, row r per lane, the pivot
+// row broadcast through LDS (double-buffered: one barrier per column).
+template <typename T, int J>
+__device__ __forceinline__ void nopiv_step(T (&x)[LU_NB], int r, int nb, T (&prow)[2][LU_NB], int k, int* info,
+                                           int b) {
+  constexpr int par = J & 1;
+  if (J >= nb) return;
+  if (r == J)
+#pragma unroll
+    for (int c = J; c < LU_NB; ++c) prow[par][c] = x[c];
+  __syncthreads();
+  const T piv = prow[par][J];
+  if (r == 0 && s_is_zero(piv) && info[b] == 0) info[b] = k + J + 1;
+  if (r > J && r < nb && !s_is_zero(piv)) {
+    const T l = s_mul(x[J], s_inv(piv));
+    x[J] = l;
+#pragma unroll
+    for (int c = J + 1; c < LU_NB; ++c) x[c] = s_fnms(x[c], l, prow[par][c]);
+  }
+}
+
+template <typename T, int... J>
+__device__ __forceinline__ void nopiv_steps(std::integer_sequence<int, J...>, T (&x)[LU_NB], int r, int nb,
+                                            T (&prow)[2][LU_NB], int k, int* info, int b) {
+  (nopiv_step<T, J>(x, r, nb, prow, k, info, b), ...);
+}
+
+// Winners -> ipiv and the swap table, row swaps inside the panel columns,
+// unpivoted LU of the top block (it reproduces the final round's
+// elimination), dinv.  One wave per matrix.  The LAPACK-style swap sequence
+// (step j exchanges row k+j with the current position of winner j) is run on
+// the at most 64 positions it touches, one per lane: lane j < 32 is position
+// k+j, lane 32+j the original position of winner j when that lies below the
+// top block.
 template <typename T>
-__global__ __launch_bounds__(256) void tslu_finish_kernel(int n, T* A, long lda, long sA, int k, int nb,
-                                                          const int* winners, int* ipiv, int* info, T* dinv,
-                                                          int* swtab) {
-  __shared__ int rows[2 * LU_NB], content[2 * LU_NB];
+__global__ __launch_bounds__(64) void tslu_finish_kernel(int n, T* A, long lda, long sA, int k, int nb,
+                                                         const int* winners, int* ipiv, int* info, T* dinv,
+                                                         int* swtab) {
   __shared__ int tab[SWTAB];
-  __shared__ T blk[LU_NB][LU_NB + 1];
   __shared__ T vals[2 * LU_NB][LU_NB];
+  __shared__ T prow[2][LU_NB];
+  __shared__ T blk[LU_NB][LU_NB + 1];
   __shared__ T Dout[LU_NB][LU_NB + 1];
-  const int b = blockIdx.x, t = threadIdx.x;
+  const int b = blockIdx.x, lane = threadIdx.x;
   T* Ab = A + (long)b * sA;
   int* ipivb = ipiv + (long)b * n;
   const int* wb = winners + (long)b * LU_NB;
-  if (t == 0) {
-    // LAPACK-style sequence: at step j swap row k+j with the current position of winner j
-    int cnt = 0;
-    for (int j = 0; j < nb; ++j) {
-      int w = wb[j];
-      if (w < k || w >= n) w = k + j;  // no candidate left (cannot happen for m >= nb): keep the row
-      int p = w;
-      for (int a = 0; a < cnt; ++a)
-        if (content[a] == w) p = rows[a];
-      const int tgt = k + j;
-      ipivb[tgt] = p;
-      if (p != tgt) {
-        int ia = -1, ib = -1;
-        for (int a = 0; a < cnt; ++a) {
-          if (rows[a] == tgt) ia = a;
-          if (rows[a] == p) ib = a;
-        }
-        if (ia < 0) {
-          rows[cnt] = tgt;
-          content[cnt] = tgt;
-          ia = cnt++;
-        }
-        if (ib < 0) {
-          rows[cnt] = p;
-          content[cnt] = p;
-          ib = cnt++;
-        }
-        const int tmp = content[ia];
-        content[ia] = content[ib];
-        content[ib] = tmp;
-      }
-    }
-    // (row, source) table of the net permutation
-    tab[0] = cnt;
-    for (int a = 0; a < cnt; ++a) {
-      tab[1 + 2 * a] = rows[a];
-      tab[2 + 2 * a] = content[a];
-    }
+  auto winner = [&](int j) {
+    const int w = wb[j];
+    return (w < k || w >= n) ? k + j : w;  // no candidate left (cannot happen for m >= nb): keep the row
+  };
+  int spos = -1;
+  if (lane < nb) {
+    spos = k + lane;
+  } else if (lane >= LU_NB && lane - LU_NB < nb) {
+    const int w = winner(lane - LU_NB);
+    if (w >= k + nb) spos = w;
   }
+  int scont = spos;
+  for (int j = 0; j < nb; ++j) {
+    const int w = winner(j);
+    const unsigned long long hit = __ballot(spos >= 0 && scont == w);
+    if (hit == 0ull) {
+      if (lane == 0) ipivb[k + j] = k + j;  // not reached: every winner holds one lane
+      continue;
+    }
+    const int L = __ffsll((long long)hit) - 1;
+    const int p = __shfl(spos, L);
+    if (lane == 0) ipivb[k + j] = p;
+    const int cj = __shfl(scont, j), cL = __shfl(scont, L);
+    if (lane == j)
+      scont = cL;
+    else if (lane == L)
+      scont = cj;
+  }
+  // net permutation: (row, source) for every touched position whose content changed
+  const bool moved = spos >= 0 && scont != spos;
+  const unsigned long long mv = __ballot(moved);
+  if (moved) {
+    const int i = __popcll(mv & ((1ull << lane) - 1ull));
+    tab[1 + 2 * i] = spos;
+    tab[2 + 2 * i] = scont;
+  }
+  if (lane == 0) tab[0] = __popcll(mv);
   __syncthreads();
   const int cnt = tab[0];
-  for (int i = t; i < SWTAB; i += 256) swtab[(long)b * SWTAB + i] = tab[i];
   // swap inside the panel columns
-  for (int idx = t; idx < cnt * nb; idx += 256) {
+  for (int idx = lane; idx < cnt * nb; idx += 64) {
     const int a = idx / nb, c = idx - a * nb;
     vals[a][c] = Ab[(long)tab[2 + 2 * a] * lda + k + c];
   }
   __syncthreads();
-  for (int idx = t; idx < cnt * nb; idx += 256) {
+  for (int idx = lane; idx < cnt * nb; idx += 64) {
     const int a = idx / nb, c = idx - a * nb;
     Ab[(long)tab[1 + 2 * a] * lda + k + c] = vals[a][c];
   }
   __syncthreads();
-  // unpivoted LU of the top nb x nb block
-  for (int idx = t; idx < LU_NB * LU_NB; idx += 256) {
-    const int r = idx / LU_NB, c = idx - r * LU_NB;
-    blk[r][c] = (r < nb && c < nb) ? Ab[(long)(k + r) * lda + k + c] : s_zero<T>();
-  }
-  __syncthreads();
-  for (int j = 0; j < nb; ++j) {
-    const T piv = blk[j][j];
-    if (t == 0 && s_is_zero(piv) && info[b] == 0) info[b] = k + j + 1;
-    const int r = j + 1 + t / LU_NB, c = t % LU_NB;  // 8 rows x 32 columns per pass
-    for (int rr = r; rr < nb; rr += 256 / LU_NB) {
-      const T l = s_is_zero(piv) ? s_zero<T>() : s_mul(blk[rr][j], s_inv(piv));
-      if (c > j && c < nb) {
-        // read blk[rr][j] before the c == j thread overwrites it
-        blk[rr][c] = s_fnms(blk[rr][c], l, blk[j][c]);
-      }
+  // unpivoted LU of the top nb x nb block, row r per lane
+  T x[LU_NB];
+  const int r = lane;
+#pragma unroll
+  for (int c = 0; c < LU_NB; ++c) x[c] = (r < nb && c < nb) ? Ab[(long)(k + r) * lda + k + c] : s_zero<T>();
+  nopiv_steps<T>(std::make_integer_sequence<int, LU_NB>{}, x, r, nb, prow, k, info, b);
+  if (r < nb) {
+#pragma unroll
+    for (int c = 0; c < LU_NB; ++c) {
+      if (c < nb) Ab[(long)(k + r) * lda + k + c] = x[c];
+      blk[r][c] = x[c];
     }
-    __syncthreads();
-    for (int rr = j + 1 + t; rr < nb; rr += 256)
-      if (!s_is_zero(piv)) blk[rr][j] = s_mul(blk[rr][j], s_inv(piv));
-    __syncthreads();
   }
-  for (int idx = t; idx < nb * nb; idx += 256) {
-    const int r = idx / nb, c = idx - r * nb;
-    Ab[(long)(k + r) * lda + k + c] = blk[r][c];
-  }
-  __syncthreads();
-  invert_diag_block<T>(&blk[0][0], LU_NB + 1, &Dout[0][0], LU_NB + 1, nb);
-  __syncthreads();
-  const int nblk = (n + LU_NB - 1) / LU_NB;
-  T* db = dinv + ((long)b * nblk + k / LU_NB) * LU_NB * LU_NB;
-  for (int idx = t; idx < LU_NB * LU_NB; idx += 256) {
-    const int r = idx / LU_NB, c = idx - r * LU_NB;
-    db[idx] = (r < nb && c < nb) ? Dout[r][c] : s_zero<T>();
-  }
+  finish_panel_tab<T, LU_NB>(&blk[0][0], LU_NB + 1, &Dout[0][0], LU_NB + 1, k, nb, n, b, tab, dinv, swtab);
 }
 
 // L21 = A21 inv(U11) for the panel rows below the top block, one row per thread
@@ -872,7 +967,7 @@ static hipError_t launch_tslu_panel(int n, T* A, long lda, long sA, int k, int n
     nxt = tmp;
     nsets = ngroups;
   }
-  hipLaunchKernelGGL((tslu_finish_kernel<T>), dim3(batch), dim3(256), 0, s, n, A, lda, sA, k, nb, cur, ipiv, info,
+  hipLaunchKernelGGL((tslu_finish_kernel<T>), dim3(batch), dim3(64), 0, s, n, A, lda, sA, k, nb, cur, ipiv, info,
                      dinv, swtab);
   if (m > nb)
     hipLaunchKernelGGL((tslu_l21_kernel<T>), dim3(ceil_div(m - nb, 256), batch), dim3(256), 0, s, n, A, lda, sA, k,
