@@ -70,6 +70,71 @@ __device__ __forceinline__ void argmax_merge(R& bv, int& bi, R v, int i) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Wave-wide argmax of (|value|, position) pairs with DPP lane moves (no LDS
+// round trips): quad exchanges, row shifts by 4 and 8, then the row
+// broadcasts; the wave's winner ends in lane 63 and is read back as a
+// uniform value.  Lanes whose DPP source is outside the pattern keep the
+// identity (-1, NONE), which never wins a merge.
+// ---------------------------------------------------------------------------
+template <int CTRL, int RM>
+__device__ __forceinline__ int dpp_int(int old, int v) {
+  return __builtin_amdgcn_update_dpp(old, v, CTRL, RM, 0xf, false);
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ float dpp_move(float old, float v) {
+  return __int_as_float(dpp_int<CTRL, RM>(__float_as_int(old), __float_as_int(v)));
+}
+template <int CTRL, int RM>
+__device__ __forceinline__ double dpp_move(double old, double v) {
+  const long long o = __double_as_longlong(old), x = __double_as_longlong(v);
+  const int lo = dpp_int<CTRL, RM>((int)o, (int)x);
+  const int hi = dpp_int<CTRL, RM>((int)(o >> 32), (int)(x >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <int CTRL, int RM, typename R>
+__device__ __forceinline__ void dpp_argmax_step(R& bv, int& bp) {
+  const R ov = dpp_move<CTRL, RM>(R(-1), bv);
+  const int op = dpp_int<CTRL, RM>(0x7fffffff, bp);
+  argmax_merge(bv, bp, ov, op);
+}
+__device__ __forceinline__ float readlane63(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
+__device__ __forceinline__ double readlane63(double v) {
+  const long long x = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)x, 63), hi = __builtin_amdgcn_readlane((int)(x >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <typename R>
+__device__ __forceinline__ void wave_argmax(R& bv, int& bp) {
+  dpp_argmax_step<0xb1, 0xf>(bv, bp);   // quad_perm [1,0,3,2]
+  dpp_argmax_step<0x4e, 0xf>(bv, bp);   // quad_perm [2,3,0,1]
+  dpp_argmax_step<0x114, 0xf>(bv, bp);  // row_shr:4
+  dpp_argmax_step<0x118, 0xf>(bv, bp);  // row_shr:8
+  dpp_argmax_step<0x142, 0xa>(bv, bp);  // row_bcast:15 into rows 1, 3
+  dpp_argmax_step<0x143, 0xc>(bv, bp);  // row_bcast:31 into rows 2, 3
+  bv = readlane63(bv);
+  bp = __builtin_amdgcn_readlane(bp, 63);
+}
+
+// 1 / a for a value every lane holds alike: the branch of Smith's algorithm is
+// taken on the first lane's comparison, so only one side executes.
+__device__ __forceinline__ double s_inv_uniform(double a) { return 1.0 / a; }
+__device__ __forceinline__ float s_inv_uniform(float a) { return 1.0f / a; }
+template <typename C>
+__device__ __forceinline__ C s_inv_uniform_c(C a) {
+  using R = decltype(a.re);
+  if (__builtin_amdgcn_readfirstlane((int)(fabs(a.re) >= fabs(a.im)))) {
+    const R r = a.im / a.re, d = a.re + a.im * r;
+    const R id = R(1) / d;
+    return C{id, -r * id};
+  }
+  const R r = a.re / a.im, d = a.re * r + a.im;
+  const R id = R(1) / d;
+  return C{r * id, -id};
+}
+__device__ __forceinline__ c128 s_inv_uniform(c128 a) { return s_inv_uniform_c(a); }
+__device__ __forceinline__ c64 s_inv_uniform(c64 a) { return s_inv_uniform_c(a); }
+
 // Block-wide argmax of (value, index) pairs; every thread gets the winner.
 template <typename R, int THREADS>
 __device__ __forceinline__ int block_argmax(R bv, int bi, R* red_v, int* red_i, int* s_out, R* s_val) {
@@ -163,38 +228,89 @@ __device__ void build_swap_table(int k, int nb, const int* ipivb, int* tab) {
 
 // Invert the factored nb x nb diagonal block D (row stride ldd) into the
 // packed form out (row stride ldo): strict lower = inv(L11), upper = inv(U11).
-// Thread c < nb forms column c of inv(L11), thread UOFF + c column c of
-// inv(U11); each substitution works in its own column of `out` (LDS), so no
-// per-thread array is indexed at run time (those would live in scratch).
-template <typename T, int UOFF>
+// Needs 256 threads: four consecutive lanes (a DPP quad) form one column, of
+// inv(L11) for threads < 128 and of inv(U11) above; lane s of the quad keeps
+// the column's entries j = 4q + s in registers.  Each entry is one unrolled
+// substitution step whose dot product is split over the quad and summed with
+// two quad exchanges.  Entries not yet formed are zero, so the sums need no
+// bounds: for column c the rows above c stay zero in inv(L11) and the rows
+// below c stay zero in inv(U11).  Rows and columns >= nb hold garbage the
+// caller discards.
+template <typename T>
+__device__ __forceinline__ T quad_sum(T v);
+template <>
+__device__ __forceinline__ double quad_sum(double v) {
+  v += dpp_move<0xb1, 0xf>(0.0, v);
+  return v + dpp_move<0x4e, 0xf>(0.0, v);
+}
+template <>
+__device__ __forceinline__ float quad_sum(float v) {
+  v += dpp_move<0xb1, 0xf>(0.0f, v);
+  return v + dpp_move<0x4e, 0xf>(0.0f, v);
+}
+template <>
+__device__ __forceinline__ c128 quad_sum(c128 v) { return c128{quad_sum(v.re), quad_sum(v.im)}; }
+template <>
+__device__ __forceinline__ c64 quad_sum(c64 v) { return c64{quad_sum(v.re), quad_sum(v.im)}; }
+
+template <typename T>
 __device__ __forceinline__ void invert_diag_block(const T* D, int ldd, T* out, int ldo, int nb) {
+  constexpr int P = 4, Q = LU_NB / P;
+  __shared__ T invd[LU_NB];
   const int t = threadIdx.x;
-  if (t < nb) {
-    const int c = t;
-    // unit lower forward substitution; the unit diagonal stays implicit
-    for (int i = c + 1; i < nb; ++i) {
-      T acc = s_neg(D[i * ldd + c]);
-      for (int j = c + 1; j < i; ++j) acc = s_fnms(acc, D[i * ldd + j], out[j * ldo + c]);
-      out[i * ldo + c] = acc;
+  if (t < LU_NB) invd[t] = t < nb ? s_inv(D[t * ldd + t]) : s_zero<T>();
+  __syncthreads();
+  if (t >= 256) return;
+  const bool upper = t >= 128;
+  const int c = (t & 127) / P, sl = t % P;
+  T xs[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) xs[q] = s_zero<T>();
+  if (!upper) {
+#pragma unroll
+    for (int i = 0; i < LU_NB; ++i) {
+      T acc = s_zero<T>();
+#pragma unroll
+      for (int q = 0; q * P < i; ++q) acc = s_fnms(acc, D[i * ldd + q * P + sl], xs[q]);
+      acc = quad_sum(acc);
+      if (sl == i % P) xs[i / P] = i == c ? s_one<T>() : acc;
     }
-  } else if (t >= UOFF && t < UOFF + nb) {
-    const int c = t - UOFF;
-    for (int i = c; i >= 0; --i) {
-      T acc = i == c ? s_one<T>() : s_zero<T>();
-      for (int j = i + 1; j <= c; ++j) acc = s_fnms(acc, D[i * ldd + j], out[j * ldo + c]);
-      out[i * ldo + c] = s_mul(acc, s_inv(D[i * ldd + i]));
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+      if (q * P + sl > c) out[(q * P + sl) * ldo + c] = xs[q];
+  } else {
+#pragma unroll
+    for (int i = LU_NB - 1; i >= 0; --i) {
+      T acc = s_zero<T>();
+#pragma unroll
+      for (int q = (i + 1) / P; q < Q; ++q) acc = s_fnms(acc, D[i * ldd + q * P + sl], xs[q]);
+      acc = quad_sum(acc);
+      if (i == c) acc = s_add(acc, s_one<T>());
+      if (sl == i % P) xs[i / P] = s_mul(acc, invd[i]);
     }
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+      if (q * P + sl <= c) out[(q * P + sl) * ldo + c] = xs[q];
   }
 }
 
 // Diagonal-block inverse -> dinv, and the panel's swap table tab (LDS:
 // count, then (row, source) pairs) -> swtab.  D is the final factored
-// diagonal block in LDS; Out is LDS scratch of nb rows.
-template <typename T, int UOFF>
-__device__ __forceinline__ void finish_panel_tab(const T* D, int ldd, T* Out, int ldo, int k, int nb, int n, int b,
+// diagonal block in LDS (a 32 x 32 array; zeroed outside nb x nb here); Out
+// is LDS scratch of nb rows.  At least 256 threads.
+template <typename T>
+__device__ __forceinline__ void finish_panel_tab(T* D, int ldd, T* Out, int ldo, int k, int nb, int n, int b,
                                                  const int* tab, T* dinv, int* swtab) {
   __syncthreads();
-  invert_diag_block<T, UOFF>(D, ldd, Out, ldo, nb);
+  if (nb < LU_NB) {
+    // the inversion reads the whole 32 x 32 block: zero it outside nb x nb
+    for (int idx = threadIdx.x; idx < LU_NB * LU_NB; idx += blockDim.x) {
+      const int r = idx / LU_NB, c = idx - r * LU_NB;
+      if (r >= nb || c >= nb) D[r * ldd + c] = s_zero<T>();
+    }
+    __syncthreads();
+  }
+  invert_diag_block<T>(D, ldd, Out, ldo, nb);
   const int cnt = tab[0] < 2 * LU_NB ? tab[0] : 2 * LU_NB;
   for (int i = threadIdx.x; i < 1 + 2 * cnt; i += blockDim.x) swtab[(long)b * SWTAB + i] = i == 0 ? cnt : tab[i];
   __syncthreads();
@@ -207,14 +323,14 @@ __device__ __forceinline__ void finish_panel_tab(const T* D, int ldd, T* Out, in
 }
 
 // finish_panel_tab with the swap table built from ipiv, for panels that swap
-// rows explicitly (at least 96 threads).
+// rows explicitly (at least 256 threads).
 template <typename T>
-__device__ __forceinline__ void finish_panel(const T* D, int ldd, T* Out, int ldo, int k, int nb, int n, int b,
+__device__ __forceinline__ void finish_panel(T* D, int ldd, T* Out, int ldo, int k, int nb, int n, int b,
                                              const int* ipivb, T* dinv, int* swtab) {
   __shared__ int tab[SWTAB];
   __syncthreads();
   if (threadIdx.x == 0) build_swap_table(k, nb, ipivb, tab);
-  finish_panel_tab<T, 64>(D, ldd, Out, ldo, k, nb, n, b, tab, dinv, swtab);
+  finish_panel_tab<T>(D, ldd, Out, ldo, k, nb, n, b, tab, dinv, swtab);
 }
 
 // Short panels: the whole (n-k) x nb panel in LDS.
@@ -480,71 +596,6 @@ __global__ __launch_bounds__(THREADS) void getrf_panel_reg_kernel(int n, T* A, l
   finish_panel<T>(&Lt[0][0], LU_NB + 1, &Dout[0][0], LU_NB + 1, k, nb, n, b, ipivb, dinv, swtab);
 }
 
-// ---------------------------------------------------------------------------
-// Wave-wide argmax of (|value|, position) pairs with DPP lane moves (no LDS
-// round trips): quad exchanges, row shifts by 4 and 8, then the row
-// broadcasts; the wave's winner ends in lane 63 and is read back as a
-// uniform value.  Lanes whose DPP source is outside the pattern keep the
-// identity (-1, NONE), which never wins a merge.
-// ---------------------------------------------------------------------------
-template <int CTRL, int RM>
-__device__ __forceinline__ int dpp_int(int old, int v) {
-  return __builtin_amdgcn_update_dpp(old, v, CTRL, RM, 0xf, false);
-}
-template <int CTRL, int RM>
-__device__ __forceinline__ float dpp_move(float old, float v) {
-  return __int_as_float(dpp_int<CTRL, RM>(__float_as_int(old), __float_as_int(v)));
-}
-template <int CTRL, int RM>
-__device__ __forceinline__ double dpp_move(double old, double v) {
-  const long long o = __double_as_longlong(old), x = __double_as_longlong(v);
-  const int lo = dpp_int<CTRL, RM>((int)o, (int)x);
-  const int hi = dpp_int<CTRL, RM>((int)(o >> 32), (int)(x >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-template <int CTRL, int RM, typename R>
-__device__ __forceinline__ void dpp_argmax_step(R& bv, int& bp) {
-  const R ov = dpp_move<CTRL, RM>(R(-1), bv);
-  const int op = dpp_int<CTRL, RM>(0x7fffffff, bp);
-  argmax_merge(bv, bp, ov, op);
-}
-__device__ __forceinline__ float readlane63(float v) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63)); }
-__device__ __forceinline__ double readlane63(double v) {
-  const long long x = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)x, 63), hi = __builtin_amdgcn_readlane((int)(x >> 32), 63);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-template <typename R>
-__device__ __forceinline__ void wave_argmax(R& bv, int& bp) {
-  dpp_argmax_step<0xb1, 0xf>(bv, bp);   // quad_perm [1,0,3,2]
-  dpp_argmax_step<0x4e, 0xf>(bv, bp);   // quad_perm [2,3,0,1]
-  dpp_argmax_step<0x114, 0xf>(bv, bp);  // row_shr:4
-  dpp_argmax_step<0x118, 0xf>(bv, bp);  // row_shr:8
-  dpp_argmax_step<0x142, 0xa>(bv, bp);  // row_bcast:15 into rows 1, 3
-  dpp_argmax_step<0x143, 0xc>(bv, bp);  // row_bcast:31 into rows 2, 3
-  bv = readlane63(bv);
-  bp = __builtin_amdgcn_readlane(bp, 63);
-}
-
-// 1 / a for a value every lane holds alike: the branch of Smith's algorithm is
-// taken on the first lane's comparison, so only one side executes.
-__device__ __forceinline__ double s_inv_uniform(double a) { return 1.0 / a; }
-__device__ __forceinline__ float s_inv_uniform(float a) { return 1.0f / a; }
-template <typename C>
-__device__ __forceinline__ C s_inv_uniform_c(C a) {
-  using R = decltype(a.re);
-  if (__builtin_amdgcn_readfirstlane((int)(fabs(a.re) >= fabs(a.im)))) {
-    const R r = a.im / a.re, d = a.re + a.im * r;
-    const R id = R(1) / d;
-    return C{id, -r * id};
-  }
-  const R r = a.re / a.im, d = a.re * r + a.im;
-  const R id = R(1) / d;
-  return C{r * id, -id};
-}
-__device__ __forceinline__ c128 s_inv_uniform(c128 a) { return s_inv_uniform_c(a); }
-__device__ __forceinline__ c64 s_inv_uniform(c64 a) { return s_inv_uniform_c(a); }
-
 // Register panel state in LDS: each wave's argmax candidate (|value|,
 // position, and its row from the column on), double-buffered by column parity
 // so a column needs a single barrier.
@@ -693,7 +744,7 @@ __global__ __launch_bounds__(THREADS) void getrf_panel_rows_kernel(int n, T* A, 
 #pragma unroll
       for (int c = 0; c < LU_NB; ++c) Dblk[pos[r]][c] = c < nb ? x[r][c] : s_zero<T>();
   }
-  finish_panel_tab<T, 64>(&Dblk[0][0], LU_NB + 1, &Dout[0][0], LU_NB + 1, k, nb, n, b, tab, dinv, swtab);
+  finish_panel_tab<T>(&Dblk[0][0], LU_NB + 1, &Dout[0][0], LU_NB + 1, k, nb, n, b, tab, dinv, swtab);
 }
 
 // Apply the panel's row permutation table to columns outside the panel and,
@@ -842,7 +893,7 @@ __device__ __forceinline__ void nopiv_steps(std::integer_sequence<int, J...>, T 
 // k+j, lane 32+j the original position of winner j when that lies below the
 // top block.
 template <typename T>
-__global__ __launch_bounds__(64) void tslu_finish_kernel(int n, T* A, long lda, long sA, int k, int nb,
+__global__ __launch_bounds__(256) void tslu_finish_kernel(int n, T* A, long lda, long sA, int k, int nb,
                                                          const int* winners, int* ipiv, int* info, T* dinv,
                                                          int* swtab) {
   __shared__ int tab[SWTAB];
@@ -850,7 +901,7 @@ __global__ __launch_bounds__(64) void tslu_finish_kernel(int n, T* A, long lda, 
   __shared__ T prow[2][LU_NB];
   __shared__ T blk[LU_NB][LU_NB + 1];
   __shared__ T Dout[LU_NB][LU_NB + 1];
-  const int b = blockIdx.x, lane = threadIdx.x;
+  const int b = blockIdx.x, lane = threadIdx.x, tid = threadIdx.x;
   T* Ab = A + (long)b * sA;
   int* ipivb = ipiv + (long)b * n;
   const int* wb = winners + (long)b * LU_NB;
@@ -866,40 +917,42 @@ __global__ __launch_bounds__(64) void tslu_finish_kernel(int n, T* A, long lda, 
     if (w >= k + nb) spos = w;
   }
   int scont = spos;
-  for (int j = 0; j < nb; ++j) {
-    const int w = winner(j);
-    const unsigned long long hit = __ballot(spos >= 0 && scont == w);
-    if (hit == 0ull) {
-      if (lane == 0) ipivb[k + j] = k + j;  // not reached: every winner holds one lane
-      continue;
+  if (tid < 64) {  // the swap simulation runs in the first wave
+    for (int j = 0; j < nb; ++j) {
+      const int w = winner(j);
+      const unsigned long long hit = __ballot(spos >= 0 && scont == w);
+      if (hit == 0ull) {
+        if (lane == 0) ipivb[k + j] = k + j;  // not reached: every winner holds one lane
+        continue;
+      }
+      const int L = __ffsll((long long)hit) - 1;
+      const int p = __shfl(spos, L);
+      if (lane == 0) ipivb[k + j] = p;
+      const int cj = __shfl(scont, j), cL = __shfl(scont, L);
+      if (lane == j)
+        scont = cL;
+      else if (lane == L)
+        scont = cj;
     }
-    const int L = __ffsll((long long)hit) - 1;
-    const int p = __shfl(spos, L);
-    if (lane == 0) ipivb[k + j] = p;
-    const int cj = __shfl(scont, j), cL = __shfl(scont, L);
-    if (lane == j)
-      scont = cL;
-    else if (lane == L)
-      scont = cj;
+    // net permutation: (row, source) for every touched position whose content changed
+    const bool moved = spos >= 0 && scont != spos;
+    const unsigned long long mv = __ballot(moved);
+    if (moved) {
+      const int i = __popcll(mv & ((1ull << lane) - 1ull));
+      tab[1 + 2 * i] = spos;
+      tab[2 + 2 * i] = scont;
+    }
+    if (lane == 0) tab[0] = __popcll(mv);
   }
-  // net permutation: (row, source) for every touched position whose content changed
-  const bool moved = spos >= 0 && scont != spos;
-  const unsigned long long mv = __ballot(moved);
-  if (moved) {
-    const int i = __popcll(mv & ((1ull << lane) - 1ull));
-    tab[1 + 2 * i] = spos;
-    tab[2 + 2 * i] = scont;
-  }
-  if (lane == 0) tab[0] = __popcll(mv);
   __syncthreads();
   const int cnt = tab[0];
   // swap inside the panel columns
-  for (int idx = lane; idx < cnt * nb; idx += 64) {
+  for (int idx = tid; idx < cnt * nb; idx += 256) {
     const int a = idx / nb, c = idx - a * nb;
     vals[a][c] = Ab[(long)tab[2 + 2 * a] * lda + k + c];
   }
   __syncthreads();
-  for (int idx = lane; idx < cnt * nb; idx += 64) {
+  for (int idx = tid; idx < cnt * nb; idx += 256) {
     const int a = idx / nb, c = idx - a * nb;
     Ab[(long)tab[1 + 2 * a] * lda + k + c] = vals[a][c];
   }
@@ -917,7 +970,7 @@ __global__ __launch_bounds__(64) void tslu_finish_kernel(int n, T* A, long lda, 
       blk[r][c] = x[c];
     }
   }
-  finish_panel_tab<T, LU_NB>(&blk[0][0], LU_NB + 1, &Dout[0][0], LU_NB + 1, k, nb, n, b, tab, dinv, swtab);
+  finish_panel_tab<T>(&blk[0][0], LU_NB + 1, &Dout[0][0], LU_NB + 1, k, nb, n, b, tab, dinv, swtab);
 }
 
 // L21 = A21 inv(U11) for the panel rows below the top block, one row per thread
@@ -967,7 +1020,7 @@ static hipError_t launch_tslu_panel(int n, T* A, long lda, long sA, int k, int n
     nxt = tmp;
     nsets = ngroups;
   }
-  hipLaunchKernelGGL((tslu_finish_kernel<T>), dim3(batch), dim3(64), 0, s, n, A, lda, sA, k, nb, cur, ipiv, info,
+  hipLaunchKernelGGL((tslu_finish_kernel<T>), dim3(batch), dim3(256), 0, s, n, A, lda, sA, k, nb, cur, ipiv, info,
                      dinv, swtab);
   if (m > nb)
     hipLaunchKernelGGL((tslu_l21_kernel<T>), dim3(ceil_div(m - nb, 256), batch), dim3(256), 0, s, n, A, lda, sA, k,
