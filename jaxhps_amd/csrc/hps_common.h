@@ -65,6 +65,8 @@ __host__ __device__ inline c64 s_mul(c64 a, c64 b) {
 
 // a - b*c
 template <typename T> __host__ __device__ inline T s_fnms(T a, T b, T c) { return s_sub(a, s_mul(b, c)); }
+// a*b + c
+template <typename T> __host__ __device__ inline T s_fma(T a, T b, T c) { return s_add(s_mul(a, b), c); }
 
 __host__ __device__ inline double s_scale(double a, double s) { return a * s; }
 __host__ __device__ inline float s_scale(float a, double s) { return a * (float)s; }

@@ -52,6 +52,11 @@ static size_t tslu_cand_ints(int n, int batch) { return (size_t)batch * ((n + TS
 // of the GPU idle
 constexpr long LU_TSLU_MAX_WORK = 64L * 1024;
 
+// Column block of the two-level factorisation and solves: 32-wide steps
+// update only their block; the rest is updated once per block by a GEMM with
+// K = the block width.
+static int lu_outer_block(int n) { return n > LU_BLOCKED_MIN ? LU_NBO : (n >= 256 ? 128 : n); }
+
 __global__ void fill_int_kernel(int* p, long n, int v) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -1183,7 +1188,7 @@ hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, T* dinv, int* scr
   // Large matrices are factored in column blocks of LU_NBO: inside a block the
   // 32-wide panels update only the block's columns; the block row's U12 and the
   // trailing update then run once per block, the latter as a GEMM with K = LU_NBO.
-  const int nbo = n > LU_BLOCKED_MIN ? LU_NBO : (n >= 256 ? 128 : n);
+  const int nbo = lu_outer_block(n);
   for (int k0 = 0; k0 < n; k0 += nbo) {
     const int bend = n - k0 < nbo ? n : k0 + nbo;
     for (int k = k0; k < bend; k += LU_NB) {
@@ -1400,7 +1405,7 @@ hipError_t getrs_perm(int n, int nrhs, const T* LU, long lda, long sA, const int
   // Column blocks of LU_NBO for large matrices (as in getrf): the 32-row
   // steps update only the rows of their block; the rest of X is updated once
   // per block by a GEMM with K = LU_NBO.
-  const int nbo = n > LU_BLOCKED_MIN ? LU_NBO : n;
+  const int nbo = lu_outer_block(n);
   for (int k0 = 0; k0 < n; k0 += nbo) {
     const int bend = n - k0 < nbo ? n : k0 + nbo;
     for (int k = k0; k < bend; k += LU_NB) {
@@ -1490,7 +1495,7 @@ hipError_t getrs_conjt(int n, int nrhs, const T* LU, long lda, long sA, const in
     g.batch = batch;
     return gemm<T>(g, s);
   };
-  const int nbo = n > LU_BLOCKED_MIN ? LU_NBO : n;
+  const int nbo = lu_outer_block(n);
   // forward: U^H Y = R
   for (int k0 = 0; k0 < n; k0 += nbo) {
     const int bend = n - k0 < nbo ? n : k0 + nbo;
