@@ -66,6 +66,14 @@ hipError_t gemm(const GemmArgs<T>& a, hipStream_t s) {
   } else {
     static const std::string nm64 = std::string("gemm_mfma_64x64<") + tname<T>() + ">";
     static const std::string nm32 = std::string("gemm_mfma_64x32<") + tname<T>() + ">";
+    static const std::string nmk32 = std::string("gemm_k32<") + tname<T>() + ">";
+    if (a.K <= 32) {
+      prof::Scope ps(s, nmk32.c_str(), flops, bytes);
+      int tiles_n = ceil_div(a.N, 32), tiles_m = ceil_div(a.M, 64);
+      dim3 grid((unsigned)(tiles_m * tiles_n), gy);
+      hipLaunchKernelGGL((gemm_k32_kernel<T>), grid, dim3(256), 0, s, a, tiles_n);
+      return hipGetLastError();
+    }
     const bool short_k = a.K <= 64 && a.N <= 256;
     prof::Scope ps(s, (short_k ? nm32 : nm64).c_str(), flops, bytes);
     // Tile choice measured on MI355X over the solver's shapes: 64x64 tiles

@@ -171,6 +171,109 @@ __global__ __launch_bounds__(256, MINB) void gemm_mfma_kernel(GemmArgs<T> p, int
   }
 }
 
+// Short-K updates (K <= 32: the 32-wide steps of the blocked LU and solves):
+// a 64 x 32 tile whose whole A panel (64 x K), B panel (K x 32) and C tile
+// are requested at once, one LDS pass, one barrier, then the MFMAs and the
+// epilogue from the prefetched C.  Four waves in a 2 x 2 arrangement, each
+// owning two 16 x 16 blocks stacked along M.
+template <typename T>
+__global__ __launch_bounds__(256, 2) void gemm_k32_kernel(GemmArgs<T> p, int tiles_n) {
+  using R = typename real_of<T>::type;
+  using M = Mfma<R>;
+  using acc_t = typename M::acc_t;
+  constexpr bool CPLX = is_cplx<T>::value;
+  constexpr int BM = 64, BN = 32, KB = 32;
+  constexpr int LA = BM * KB / 256, LB = KB * BN / 256;
+  __shared__ R As_re[KB][BM + 1];
+  __shared__ R Bs_re[KB][BN + 1];
+  __shared__ R As_im[CPLX ? KB : 1][CPLX ? BM + 1 : 1];
+  __shared__ R Bs_im[CPLX ? KB : 1][CPLX ? BN + 1 : 1];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int tm = blockIdx.x / tiles_n, tn = blockIdx.x % tiles_n;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int li = lane & 15, lk = lane >> 4;
+  for (int b = blockIdx.y; b < p.batch; b += gridDim.y) {
+    const T* A = p.A + (long)b * p.sA;
+    const T* B = p.B + (long)b * p.sB;
+    const T* C = p.C ? p.C + (long)b * p.sC : nullptr;
+    T* D = p.D + (long)b * p.sD;
+    T ra[LA], rb[LB], rc[2][4];
+#pragma unroll
+    for (int i = 0; i < LA; ++i) {
+      const int e = tid + 256 * i, r = e >> 5, c = e & 31;
+      const int gr = m0 + r;
+      ra[i] = (gr < p.M && c < p.K) ? A[(long)gr * p.lda + c] : s_zero<T>();
+    }
+#pragma unroll
+    for (int i = 0; i < LB; ++i) {
+      const int e = tid + 256 * i, r = e >> 5, c = e & 31;
+      const int gc = n0 + c;
+      rb[i] = (r < p.K && gc < p.N) ? B[(long)r * p.ldb + gc] : s_zero<T>();
+    }
+    const bool use_c = p.beta != 0.0;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wr * 32 + i * 16 + M::out_row(lk, r);
+        const int col = n0 + wc * 16 + li;
+        rc[i][r] = (use_c && row < p.M && col < p.N) ? C[(long)row * p.ldc + col] : s_zero<T>();
+      }
+#pragma unroll
+    for (int i = 0; i < LA; ++i) {
+      const int e = tid + 256 * i, r = e >> 5, c = e & 31;
+      As_re[c][r] = s_re(ra[i]);
+      if constexpr (CPLX) As_im[c][r] = s_im(ra[i]);
+    }
+#pragma unroll
+    for (int i = 0; i < LB; ++i) {
+      const int e = tid + 256 * i, r = e >> 5, c = e & 31;
+      Bs_re[r][c] = s_re(rb[i]);
+      if constexpr (CPLX) Bs_im[r][c] = s_im(rb[i]);
+    }
+    __syncthreads();
+    acc_t acc_re[2], acc_im[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      acc_re[i] = acc_t{0, 0, 0, 0};
+      acc_im[i] = acc_t{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int kk = 0; kk < KB; kk += 4) {
+      const R b_re = Bs_re[kk + lk][wc * 16 + li];
+      const R b_im = CPLX ? Bs_im[kk + lk][wc * 16 + li] : R(0);
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const R a_re = As_re[kk + lk][wr * 32 + i * 16 + li];
+        acc_re[i] = M::mma(a_re, b_re, acc_re[i]);
+        if constexpr (CPLX) {
+          const R a_im = As_im[kk + lk][wr * 32 + i * 16 + li];
+          acc_re[i] = M::mma(-a_im, b_im, acc_re[i]);
+          acc_im[i] = M::mma(a_re, b_im, acc_im[i]);
+          acc_im[i] = M::mma(a_im, b_re, acc_im[i]);
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wr * 32 + i * 16 + M::out_row(lk, r);
+        const int col = n0 + wc * 16 + li;
+        if (row < p.M && col < p.N) {
+          T v = make_val<T>(acc_re[i][r], CPLX ? acc_im[i][r] : R(0));
+          v = s_scale(v, p.alpha);
+          if (use_c) v = s_add(v, s_scale(rc[i][r], p.beta));
+          D[(long)row * p.ldd + col] = v;
+        }
+      }
+    // the next batch element rewrites the tiles
+    __syncthreads();
+  }
+}
+
 template <typename R>
 __device__ __forceinline__ R wave_sum(R v) {
 #pragma unroll
