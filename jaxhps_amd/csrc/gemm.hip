@@ -19,6 +19,8 @@ namespace hps {
 template <typename T>
 hipError_t gemm(const GemmArgs<T>& a, hipStream_t s) {
   if (a.M <= 0 || a.N <= 0 || a.batch <= 0) return hipSuccess;
+  // the triangular block step has one kernel: plain A, K <= 32, N >= 3
+  if (a.triA != 0 && (a.transA || a.K > 32 || a.N <= 16 || a.M > 32)) return hipErrorInvalidValue;
   unsigned gy = (unsigned)(a.batch < 65535 ? a.batch : 65535);
   const double flops = 2.0 * a.M * a.N * (double)a.K * a.batch * flop_mult<T>();
   const double bytes = sizeof(T) * ((double)a.M * a.K * (a.sA ? a.batch : 1) + (double)a.K * a.N * (a.sB ? a.batch : 1) +
@@ -67,11 +69,17 @@ hipError_t gemm(const GemmArgs<T>& a, hipStream_t s) {
     static const std::string nm64 = std::string("gemm_mfma_64x64<") + tname<T>() + ">";
     static const std::string nm32 = std::string("gemm_mfma_64x32<") + tname<T>() + ">";
     static const std::string nmk32 = std::string("gemm_k32<") + tname<T>() + ">";
+    static const std::string nmtri = std::string("trsm_k32<") + tname<T>() + ">";
     if (a.K <= 32) {
-      prof::Scope ps(s, nmk32.c_str(), flops, bytes);
+      prof::Scope ps(s, (a.triA ? nmtri : nmk32).c_str(), flops, bytes);
       int tiles_n = ceil_div(a.N, 32), tiles_m = ceil_div(a.M, 64);
       dim3 grid((unsigned)(tiles_m * tiles_n), gy);
-      hipLaunchKernelGGL((gemm_k32_kernel<T>), grid, dim3(256), 0, s, a, tiles_n);
+      if (a.triA == 1)
+        hipLaunchKernelGGL((gemm_k32_kernel<T, 1>), grid, dim3(256), 0, s, a, tiles_n);
+      else if (a.triA == 2)
+        hipLaunchKernelGGL((gemm_k32_kernel<T, 2>), grid, dim3(256), 0, s, a, tiles_n);
+      else
+        hipLaunchKernelGGL((gemm_k32_kernel<T>), grid, dim3(256), 0, s, a, tiles_n);
       return hipGetLastError();
     }
     const bool short_k = a.K <= 64 && a.N <= 256;

@@ -175,8 +175,11 @@ __global__ __launch_bounds__(256, MINB) void gemm_mfma_kernel(GemmArgs<T> p, int
 // a 64 x 32 tile whose whole A panel (64 x K), B panel (K x 32) and C tile
 // are requested at once, one LDS pass, one barrier, then the MFMAs and the
 // epilogue from the prefetched C.  Four waves in a 2 x 2 arrangement, each
-// owning two 16 x 16 blocks stacked along M.
-template <typename T>
+// owning two 16 x 16 blocks stacked along M.  TRI = 1 / 2 reads A as the
+// unit-lower / upper part of a packed diagonal-block inverse (the triangular
+// solves' block step); every B element of a tile is read before any D element
+// is written, so D may alias B.
+template <typename T, int TRI = 0>
 __global__ __launch_bounds__(256, 2) void gemm_k32_kernel(GemmArgs<T> p, int tiles_n) {
   using R = typename real_of<T>::type;
   using M = Mfma<R>;
@@ -204,7 +207,16 @@ __global__ __launch_bounds__(256, 2) void gemm_k32_kernel(GemmArgs<T> p, int til
     for (int i = 0; i < LA; ++i) {
       const int e = tid + 256 * i, r = e >> 5, c = e & 31;
       const int gr = m0 + r;
-      ra[i] = (gr < p.M && c < p.K) ? A[(long)gr * p.lda + c] : s_zero<T>();
+      T v = s_zero<T>();
+      if (gr < p.M && c < p.K) {
+        if (TRI == 1)
+          v = c < gr ? A[(long)gr * p.lda + c] : (c == gr ? s_one<T>() : s_zero<T>());
+        else if (TRI == 2)
+          v = c >= gr ? A[(long)gr * p.lda + c] : s_zero<T>();
+        else
+          v = A[(long)gr * p.lda + c];
+      }
+      ra[i] = v;
     }
 #pragma unroll
     for (int i = 0; i < LB; ++i) {

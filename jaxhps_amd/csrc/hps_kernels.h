@@ -23,6 +23,9 @@ struct GemmArgs {
   long ldd = 0, sD = 0;
   int batch = 1;
   int transA = 0;
+  // triA: A is a packed 32 x 32 diagonal-block inverse used as its unit-lower
+  // part (1) or its upper part (2); needs K <= 32.  D may then alias B.
+  int triA = 0;
 };
 
 template <typename T>

@@ -698,7 +698,8 @@ __device__ __forceinline__ void panel_steps(std::integer_sequence<int, J...>, T 
 // Register panels (m <= THREADS RPT): thread t owns panel rows t + THREADS r
 // in registers for the whole factorisation and writes each to its final
 // position at the end.  No panel storage in LDS.  Columns beyond nb are zero
-// and stay zero (their updates subtract l * 0).
+// and stay zero (their updates subtract l * 0).  The workgroup then applies
+// the panel's row permutation to the rest of its matrix itself.
 template <typename T, int RPT, int THREADS>
 __global__ __launch_bounds__(THREADS) void getrf_panel_rows_kernel(int n, T* A, long lda, long sA, int k, int nb,
                                                                    int* ipiv, int* info, T* dinv, int* swtab) {
@@ -750,6 +751,30 @@ __global__ __launch_bounds__(THREADS) void getrf_panel_rows_kernel(int n, T* A, 
       for (int c = 0; c < LU_NB; ++c) Dblk[pos[r]][c] = c < nb ? x[r][c] : s_zero<T>();
   }
   finish_panel_tab<T>(&Dblk[0][0], LU_NB + 1, &Dout[0][0], LU_NB + 1, k, nb, n, b, tab, dinv, swtab);
+  // the panel's row permutation on every other column of the matrix, 32
+  // columns at a time: all moved rows are read before any is written
+  constexpr int RG = THREADS / 32, NV = 2 * LU_NB / RG;
+  const int cnt = tab[0] < 2 * LU_NB ? tab[0] : 2 * LU_NB;
+  const int others = n - nb;
+  T* Ab = A + (long)b * sA;
+  const int cc = t & 31, a0 = t >> 5;
+  for (int c0 = 0; c0 < others; c0 += 32) {
+    const int oc = c0 + cc;
+    const int col = oc < k ? oc : oc + nb;
+    const bool ok = oc < others;
+    T v[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int a = a0 + RG * i;
+      v[i] = (ok && a < cnt) ? Ab[(long)tab[2 + 2 * a] * lda + col] : s_zero<T>();
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int a = a0 + RG * i;
+      if (ok && a < cnt) Ab[(long)tab[1 + 2 * a] * lda + col] = v[i];
+    }
+  }
 }
 
 // Apply the panel's row permutation table to columns outside the panel and,
@@ -1047,11 +1072,14 @@ static int rows_panel_max() {
   return sizeof(T) >= 16 ? 512 : 1024;
 }
 
+// *swapped: the panel kernel already applied its row permutation to the other
+// columns (the register panels do).
 template <typename T>
 static hipError_t launch_panel(int n, T* A, long lda, long sA, int k, int nb, int* ipiv, int* info, T* dinv,
-                               int* swtab, int* cand, int batch, hipStream_t s) {
+                               int* swtab, int* cand, int batch, hipStream_t s, bool* swapped) {
   const int m = n - k;
   const double pf = flop_mult<T>() * batch * ((double)m * nb * nb - (double)nb * nb * nb / 3.0);
+  *swapped = m <= rows_panel_max<T>();
   if (m <= rows_panel_max<T>()) {
     static const std::string nm_rows = std::string("getrf_panel_rows<") + tname<T>() + ">";
     prof::Scope ps(s, nm_rows.c_str(), pf, 2.0 * sizeof(T) * (double)m * nb * batch);
@@ -1154,6 +1182,40 @@ __global__ __launch_bounds__(256) void trsm_inv_kernel(int n, int nrhs, const T*
   }
 }
 
+// X[k:k+nb, :ncols] = tri(Dinv_k) X[k:k+nb, :ncols] in place, tri = unit lower
+// or upper: an MFMA product (the K <= 32 GEMM reading the packed inverse as a
+// triangle) for wide X, the LDS kernel above for ncols <= 16.
+template <typename T>
+static hipError_t trsm_block(bool upper, int n, int ncols, const T* dinv, T* X, long ldx, long sX, int k, int nb,
+                             int batch, hipStream_t s, const std::string& label) {
+  if (ncols > 16) {
+    GemmArgs<T> g;
+    g.M = nb;
+    g.N = ncols;
+    g.K = nb;
+    g.A = dinv + (long)(k / LU_NB) * LU_NB * LU_NB;
+    g.lda = LU_NB;
+    g.sA = (long)ceil_div(n, LU_NB) * LU_NB * LU_NB;
+    g.B = X + (long)k * ldx;
+    g.ldb = ldx;
+    g.sB = sX;
+    g.D = X + (long)k * ldx;
+    g.ldd = ldx;
+    g.sD = sX;
+    g.batch = batch;
+    g.triA = upper ? 2 : 1;
+    return gemm<T>(g, s);
+  }
+  prof::Scope ps(s, label.c_str(), flop_mult<T>() * batch * (double)nb * nb * ncols,
+                 2.0 * sizeof(T) * (double)nb * ncols * batch);
+  dim3 tgrid(ceil_div(ncols, 8), batch);
+  if (upper)
+    hipLaunchKernelGGL((trsm_inv_kernel<T, true>), tgrid, dim3(256), 0, s, n, ncols, dinv, X, ldx, sX, k, nb);
+  else
+    hipLaunchKernelGGL((trsm_inv_kernel<T, false>), tgrid, dim3(256), 0, s, n, ncols, dinv, X, ldx, sX, k, nb);
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, T* dinv, int* scratch, int* info, int batch,
                     hipStream_t s) {
@@ -1193,10 +1255,18 @@ hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, T* dinv, int* scr
     const int bend = n - k0 < nbo ? n : k0 + nbo;
     for (int k = k0; k < bend; k += LU_NB) {
       const int nb = bend - k < LU_NB ? bend - k : LU_NB;
-      hipError_t e = launch_panel<T>(n, A, lda, sA, k, nb, ipiv, info, dinv, swtab, cand, batch, s);
+      bool swapped = false;
+      hipError_t e = launch_panel<T>(n, A, lda, sA, k, nb, ipiv, info, dinv, swtab, cand, batch, s, &swapped);
       if (e != hipSuccess) return e;
       const int others = n - nb;
-      if (others > 0) {
+      if (swapped) {
+        // U12 = inv(L11) A12 for the block's columns right of the panel
+        if (bend - k - nb > 0) {
+          static const std::string nm = std::string("getrf_u12<") + tname<T>() + ">";
+          e = trsm_block<T>(false, n, bend - k - nb, dinv, A + k + nb, lda, sA, k, nb, batch, s, nm);
+          if (e != hipSuccess) return e;
+        }
+      } else if (others > 0) {
         static const std::string nm = std::string("getrf_swap_u12<") + tname<T>() + ">";
         prof::Scope ps(s, nm.c_str(), flop_mult<T>() * batch * (double)(bend - k - nb) * nb * nb,
                        3.0 * sizeof(T) * (double)nb * others * batch);
@@ -1211,17 +1281,12 @@ hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, T* dinv, int* scr
     if (bend < n) {
       // U12 = inv(L11) A12 for the block row, block by block with the inverted diagonal blocks
       const int ncols = n - bend;
-      dim3 tgrid(ceil_div(ncols, 8), batch);
       for (int k = k0; k < bend; k += LU_NB) {
         const int nb = bend - k < LU_NB ? bend - k : LU_NB;
-        {
-          static const std::string nm = std::string("getrf_u12<") + tname<T>() + ">";
-          prof::Scope ps(s, nm.c_str(), flop_mult<T>() * batch * (double)nb * nb * ncols,
-                         2.0 * sizeof(T) * (double)nb * ncols * batch);
-          hipLaunchKernelGGL((trsm_inv_kernel<T, false>), tgrid, dim3(256), 0, s, n, ncols, dinv, A + bend, lda, sA,
-                             k, nb);
-        }
-        hipError_t e = gemm_update(bend - k - nb, ncols, nb, A + (long)(k + nb) * lda + k, A + (long)k * lda + bend,
+        static const std::string nm = std::string("getrf_u12<") + tname<T>() + ">";
+        hipError_t e = trsm_block<T>(false, n, ncols, dinv, A + bend, lda, sA, k, nb, batch, s, nm);
+        if (e != hipSuccess) return e;
+        e = gemm_update(bend - k - nb, ncols, nb, A + (long)(k + nb) * lda + k, A + (long)k * lda + bend,
                                    A + (long)(k + nb) * lda + bend);
         if (e != hipSuccess) return e;
       }
@@ -1378,7 +1443,6 @@ hipError_t getrs_perm(int n, int nrhs, const T* LU, long lda, long sA, const int
     hipLaunchKernelGGL((gather_rows_kernel<T>), grid, dim3(64), 0, s, n, nrhs, perm + (long)b0 * n,
                        R + (long)b0 * sR, ldr, sR, X + (long)b0 * sX, ldx, sX);
   }
-  dim3 tgrid(ceil_div(nrhs, 8), batch);
   auto gemm_update = [&](int M, int K, const T* Ap, const T* Bp, T* Cp) -> hipError_t {
     if (M <= 0 || K <= 0) return hipSuccess;
     GemmArgs<T> g;
@@ -1410,13 +1474,10 @@ hipError_t getrs_perm(int n, int nrhs, const T* LU, long lda, long sA, const int
     const int bend = n - k0 < nbo ? n : k0 + nbo;
     for (int k = k0; k < bend; k += LU_NB) {
       const int nb = bend - k < LU_NB ? bend - k : LU_NB;
-      {
-        static const std::string nm = std::string("trsm_lower<") + tname<T>() + ">";
-        prof::Scope ps(s, nm.c_str(), flop_mult<T>() * batch * (double)nb * nb * nrhs,
-                       2.0 * sizeof(T) * (double)nb * nrhs * batch);
-        hipLaunchKernelGGL((trsm_inv_kernel<T, false>), tgrid, dim3(256), 0, s, n, nrhs, dinv, X, ldx, sX, k, nb);
-      }
-      hipError_t e = gemm_update(bend - k - nb, nb, LU + (long)(k + nb) * lda + k, X + (long)k * ldx,
+      static const std::string nm = std::string("trsm_lower<") + tname<T>() + ">";
+      hipError_t e = trsm_block<T>(false, n, nrhs, dinv, X, ldx, sX, k, nb, batch, s, nm);
+      if (e != hipSuccess) return e;
+      e = gemm_update(bend - k - nb, nb, LU + (long)(k + nb) * lda + k, X + (long)k * ldx,
                                  X + (long)(k + nb) * ldx);
       if (e != hipSuccess) return e;
     }
@@ -1431,13 +1492,10 @@ hipError_t getrs_perm(int n, int nrhs, const T* LU, long lda, long sA, const int
     const int last = ((bend - 1 - k0) / LU_NB) * LU_NB + k0;
     for (int k = last; k >= k0; k -= LU_NB) {
       const int nb = bend - k < LU_NB ? bend - k : LU_NB;
-      {
-        static const std::string nm = std::string("trsm_upper<") + tname<T>() + ">";
-        prof::Scope ps(s, nm.c_str(), flop_mult<T>() * batch * (double)nb * nb * nrhs,
-                       2.0 * sizeof(T) * (double)nb * nrhs * batch);
-        hipLaunchKernelGGL((trsm_inv_kernel<T, true>), tgrid, dim3(256), 0, s, n, nrhs, dinv, X, ldx, sX, k, nb);
-      }
-      hipError_t e = gemm_update(k - k0, nb, LU + (long)k0 * lda + k, X + (long)k * ldx, X + (long)k0 * ldx);
+      static const std::string nm = std::string("trsm_upper<") + tname<T>() + ">";
+      hipError_t e = trsm_block<T>(true, n, nrhs, dinv, X, ldx, sX, k, nb, batch, s, nm);
+      if (e != hipSuccess) return e;
+      e = gemm_update(k - k0, nb, LU + (long)k0 * lda + k, X + (long)k * ldx, X + (long)k0 * ldx);
       if (e != hipSuccess) return e;
     }
     hipError_t e = gemm_update(k0, bend - k0, LU + k0, X + (long)k0 * ldx, X);

@@ -83,6 +83,23 @@ def test_getrf_getrs_solve(gpu, dtype, n, batch):
     assert rec < TOL[dtype] * n * 10, rec
 
 
+@pytest.mark.parametrize("dtype", [torch.complex128, torch.float32])
+@pytest.mark.parametrize("n,batch", [(100, 3), (256, 2), (700, 1)])
+def test_getrs_many_rhs(gpu, dtype, n, batch):
+    from jaxhps_amd import _lib
+
+    print(f"getrs many rhs dtype={dtype} n={n} batch={batch}", flush=True)
+    g = torch.Generator().manual_seed(7 * n + batch)
+    A = _rand((batch, n, n), dtype, g)
+    R = _rand((batch, n, 45), dtype, g)
+    LU = A.to(gpu).contiguous()
+    perm, dinv, info = _lib.getrf(LU)
+    X = _lib.getrs(LU, perm, dinv, R.to(gpu).contiguous())
+    ref = torch.linalg.solve(_ref64(A), _ref64(R))
+    rel = ((_ref64(X) - ref).abs().max() / ref.abs().max()).item()
+    assert rel < TOL[dtype] * n * 50, rel
+
+
 def test_getrf_reports_singular(gpu):
     from jaxhps_amd import _lib
 
