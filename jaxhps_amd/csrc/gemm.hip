@@ -72,14 +72,21 @@ hipError_t gemm(const GemmArgs<T>& a, hipStream_t s) {
     static const std::string nmtri = std::string("trsm_k32<") + tname<T>() + ">";
     if (a.K <= 32) {
       prof::Scope ps(s, (a.triA ? nmtri : nmk32).c_str(), flops, bytes);
-      int tiles_n = ceil_div(a.N, 32), tiles_m = ceil_div(a.M, 64);
-      dim3 grid((unsigned)(tiles_m * tiles_n), gy);
-      if (a.triA == 1)
-        hipLaunchKernelGGL((gemm_k32_kernel<T, 1>), grid, dim3(256), 0, s, a, tiles_n);
-      else if (a.triA == 2)
-        hipLaunchKernelGGL((gemm_k32_kernel<T, 2>), grid, dim3(256), 0, s, a, tiles_n);
-      else
+      if (a.M <= 32) {
+        // 32 x 64 tiles (the triangular block steps and other 32-row products)
+        int tiles_n = ceil_div(a.N, 64);
+        dim3 grid((unsigned)tiles_n, gy);
+        if (a.triA == 1)
+          hipLaunchKernelGGL((gemm_k32_kernel<T, 1, 1, 2>), grid, dim3(256), 0, s, a, tiles_n);
+        else if (a.triA == 2)
+          hipLaunchKernelGGL((gemm_k32_kernel<T, 2, 1, 2>), grid, dim3(256), 0, s, a, tiles_n);
+        else
+          hipLaunchKernelGGL((gemm_k32_kernel<T, 0, 1, 2>), grid, dim3(256), 0, s, a, tiles_n);
+      } else {
+        int tiles_n = ceil_div(a.N, 32), tiles_m = ceil_div(a.M, 64);
+        dim3 grid((unsigned)(tiles_m * tiles_n), gy);
         hipLaunchKernelGGL((gemm_k32_kernel<T>), grid, dim3(256), 0, s, a, tiles_n);
+      }
       return hipGetLastError();
     }
     const bool short_k = a.K <= 64 && a.N <= 256;

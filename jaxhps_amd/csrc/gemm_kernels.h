@@ -172,20 +172,20 @@ __global__ __launch_bounds__(256, MINB) void gemm_mfma_kernel(GemmArgs<T> p, int
 }
 
 // Short-K updates (K <= 32: the 32-wide steps of the blocked LU and solves):
-// a 64 x 32 tile whose whole A panel (64 x K), B panel (K x 32) and C tile
-// are requested at once, one LDS pass, one barrier, then the MFMAs and the
-// epilogue from the prefetched C.  Four waves in a 2 x 2 arrangement, each
-// owning two 16 x 16 blocks stacked along M.  TRI = 1 / 2 reads A as the
-// unit-lower / upper part of a packed diagonal-block inverse (the triangular
-// solves' block step); every B element of a tile is read before any D element
-// is written, so D may alias B.
-template <typename T, int TRI = 0>
+// a (32 TM) x (32 TN) tile whose whole A panel (BM x K), B panel (K x BN) and
+// C tile are requested at once, one LDS pass, one barrier, then the MFMAs and
+// the epilogue from the prefetched C.  Four waves in a 2 x 2 arrangement, each
+// owning TM x TN 16 x 16 blocks.  TRI = 1 / 2 reads A as the unit-lower /
+// upper part of a packed diagonal-block inverse (the triangular solves' block
+// step, M <= 32: TM = 1); every B element of a tile is read before any D
+// element is written, so D may alias B.
+template <typename T, int TRI = 0, int TM = 2, int TN = 1>
 __global__ __launch_bounds__(256, 2) void gemm_k32_kernel(GemmArgs<T> p, int tiles_n) {
   using R = typename real_of<T>::type;
   using M = Mfma<R>;
   using acc_t = typename M::acc_t;
   constexpr bool CPLX = is_cplx<T>::value;
-  constexpr int BM = 64, BN = 32, KB = 32;
+  constexpr int BM = 32 * TM, BN = 32 * TN, KB = 32;
   constexpr int LA = BM * KB / 256, LB = KB * BN / 256;
   __shared__ R As_re[KB][BM + 1];
   __shared__ R Bs_re[KB][BN + 1];
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(256, 2) void gemm_k32_kernel(GemmArgs<T> p, int til
     const T* B = p.B + (long)b * p.sB;
     const T* C = p.C ? p.C + (long)b * p.sC : nullptr;
     T* D = p.D + (long)b * p.sD;
-    T ra[LA], rb[LB], rc[2][4];
+    T ra[LA], rb[LB], rc[TM][TN][4];
 #pragma unroll
     for (int i = 0; i < LA; ++i) {
       const int e = tid + 256 * i, r = e >> 5, c = e & 31;
@@ -220,19 +220,21 @@ __global__ __launch_bounds__(256, 2) void gemm_k32_kernel(GemmArgs<T> p, int til
     }
 #pragma unroll
     for (int i = 0; i < LB; ++i) {
-      const int e = tid + 256 * i, r = e >> 5, c = e & 31;
+      const int e = tid + 256 * i, r = e / BN, c = e % BN;
       const int gc = n0 + c;
       rb[i] = (r < p.K && gc < p.N) ? B[(long)r * p.ldb + gc] : s_zero<T>();
     }
     const bool use_c = p.beta != 0.0;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wr * 32 + i * 16 + M::out_row(lk, r);
-        const int col = n0 + wc * 16 + li;
-        rc[i][r] = (use_c && row < p.M && col < p.N) ? C[(long)row * p.ldc + col] : s_zero<T>();
-      }
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wr * (BM / 2) + i * 16 + M::out_row(lk, r);
+          const int col = n0 + wc * (BN / 2) + j * 16 + li;
+          rc[i][j][r] = (use_c && row < p.M && col < p.N) ? C[(long)row * p.ldc + col] : s_zero<T>();
+        }
 #pragma unroll
     for (int i = 0; i < LA; ++i) {
       const int e = tid + 256 * i, r = e >> 5, c = e & 31;
@@ -241,46 +243,59 @@ __global__ __launch_bounds__(256, 2) void gemm_k32_kernel(GemmArgs<T> p, int til
     }
 #pragma unroll
     for (int i = 0; i < LB; ++i) {
-      const int e = tid + 256 * i, r = e >> 5, c = e & 31;
+      const int e = tid + 256 * i, r = e / BN, c = e % BN;
       Bs_re[r][c] = s_re(rb[i]);
       if constexpr (CPLX) Bs_im[r][c] = s_im(rb[i]);
     }
     __syncthreads();
-    acc_t acc_re[2], acc_im[2];
+    acc_t acc_re[TM][TN], acc_im[TM][TN];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      acc_re[i] = acc_t{0, 0, 0, 0};
-      acc_im[i] = acc_t{0, 0, 0, 0};
-    }
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        acc_re[i][j] = acc_t{0, 0, 0, 0};
+        acc_im[i][j] = acc_t{0, 0, 0, 0};
+      }
 #pragma unroll
     for (int kk = 0; kk < KB; kk += 4) {
-      const R b_re = Bs_re[kk + lk][wc * 16 + li];
-      const R b_im = CPLX ? Bs_im[kk + lk][wc * 16 + li] : R(0);
+      R a_re[TM], a_im[TM], b_re[TN], b_im[TN];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        const R a_re = As_re[kk + lk][wr * 32 + i * 16 + li];
-        acc_re[i] = M::mma(a_re, b_re, acc_re[i]);
-        if constexpr (CPLX) {
-          const R a_im = As_im[kk + lk][wr * 32 + i * 16 + li];
-          acc_re[i] = M::mma(-a_im, b_im, acc_re[i]);
-          acc_im[i] = M::mma(a_re, b_im, acc_im[i]);
-          acc_im[i] = M::mma(a_im, b_re, acc_im[i]);
-        }
+      for (int i = 0; i < TM; ++i) {
+        a_re[i] = As_re[kk + lk][wr * (BM / 2) + i * 16 + li];
+        a_im[i] = CPLX ? As_im[kk + lk][wr * (BM / 2) + i * 16 + li] : R(0);
       }
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        b_re[j] = Bs_re[kk + lk][wc * (BN / 2) + j * 16 + li];
+        b_im[j] = CPLX ? Bs_im[kk + lk][wc * (BN / 2) + j * 16 + li] : R(0);
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          acc_re[i][j] = M::mma(a_re[i], b_re[j], acc_re[i][j]);
+          if constexpr (CPLX) {
+            acc_re[i][j] = M::mma(-a_im[i], b_im[j], acc_re[i][j]);
+            acc_im[i][j] = M::mma(a_re[i], b_im[j], acc_im[i][j]);
+            acc_im[i][j] = M::mma(a_im[i], b_re[j], acc_im[i][j]);
+          }
+        }
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < TM; ++i)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wr * 32 + i * 16 + M::out_row(lk, r);
-        const int col = n0 + wc * 16 + li;
-        if (row < p.M && col < p.N) {
-          T v = make_val<T>(acc_re[i][r], CPLX ? acc_im[i][r] : R(0));
-          v = s_scale(v, p.alpha);
-          if (use_c) v = s_add(v, s_scale(rc[i][r], p.beta));
-          D[(long)row * p.ldd + col] = v;
+      for (int j = 0; j < TN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wr * (BM / 2) + i * 16 + M::out_row(lk, r);
+          const int col = n0 + wc * (BN / 2) + j * 16 + li;
+          if (row < p.M && col < p.N) {
+            T v = make_val<T>(acc_re[i][j][r], CPLX ? acc_im[i][j][r] : R(0));
+            v = s_scale(v, p.alpha);
+            if (use_c) v = s_add(v, s_scale(rc[i][j][r], p.beta));
+            D[(long)row * p.ldd + col] = v;
+          }
         }
-      }
     // the next batch element rewrites the tiles
     __syncthreads();
   }

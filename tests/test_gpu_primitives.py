@@ -117,7 +117,8 @@ def test_getrf_reports_singular(gpu):
 @pytest.mark.parametrize("dtype", DTYPES)
 @pytest.mark.parametrize(
     "batch,M,N,K",
-    [(3, 70, 1, 45), (2, 33, 2, 300), (4, 100, 9, 64), (2, 130, 70, 33), (2, 130, 70, 32), (3, 65, 97, 7)],
+    [(3, 70, 1, 45), (2, 33, 2, 300), (4, 100, 9, 64), (2, 130, 70, 33), (2, 130, 70, 32), (3, 65, 97, 7),
+     (2, 30, 70, 20)],
 )
 def test_gemm_conjt_matches_reference(gpu, dtype, batch, M, N, K):
     """D = alpha A^H B + beta C with A stored K x M."""
