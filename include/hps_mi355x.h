@@ -128,6 +128,10 @@ int hps_leaf_solve(hps_handle_t h, int dtype, int kind, int nl, int n, int nb, i
  *   B_out (nm, ne, ni) optional copy of B.  T_out/h_out may be NULL.
  * upd (host int[4*n_upd]): the nonzero blocks (r0, nr, k0, nk) of B; the
  *   products B S and B gt use only these (n_upd = 0: B dense).
+ * d12_blk / d21_blk (host int[4*n], ItI only): the nonzero rectangles of D12
+ *   and D21 (an F unknown's equation sees only the G unknowns of its
+ *   partner child and vice versa), each row in exactly one rectangle; the
+ *   products D12 D21, D12 Y_G and D21 X_F use only these (n = 0: dense).
  * Replaces _uniform_quad_merge_DtN / _ItI with assemble_merge_outputs_*
  *   (merge/_uniform_2D_DtN.py:207-348, merge/_uniform_2D_ItI.py:183-375,
  *   merge/_schur_complement.py:7-237) and _uniform_oct_merge_DtN
@@ -135,9 +139,10 @@ int hps_leaf_solve(hps_handle_t h, int dtype, int kind, int nl, int n, int nb, i
 size_t hps_merge_workspace(int dtype, int kind, int nm, int ni, int ne, int nsrc);
 int hps_merge_level(hps_handle_t h, int dtype, int kind, int nm, int nc, int nec, int ni, int ne, int nsrc,
                     const void* T_in, const void* h_in, const int* int_rows, const int* int_cols, const int* ext,
-                    const int* cols_final, const int* final_of_internal, int n_upd, const int* upd, void* S_out,
-                    void* T_out, void* h_out, void* gt_out, void* F_lu, int* perm, void* F_dinv, int* info,
-                    void* D12_out, void* D21_out, void* B_out, void* ws, size_t ws_bytes);
+                    const int* cols_final, const int* final_of_internal, int n_upd, const int* upd, int n_d12,
+                    const int* d12_blk, int n_d21, const int* d21_blk, void* S_out, void* T_out, void* h_out,
+                    void* gt_out, void* F_lu, int* perm, void* F_dinv, int* info, void* D12_out, void* D21_out,
+                    void* B_out, void* ws, size_t ws_bytes);
 
 /* ---- up pass (one tree level) --------------------------------------------
  * New outgoing data through the factors a merge level kept (hps_merge_level

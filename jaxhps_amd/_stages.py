@@ -162,10 +162,13 @@ def merge_level(T_in, h_in, tables: MergeTables, keep_factors=False, want_T=True
     B = torch.empty((nm, ne, ni), dtype=dt, device=dev) if keep_factors else None
     ws = workspace(lib.hps_merge_workspace(dtype_code(dt), kind, nm, ni, ne, nsrc), dev)
     upd = np.ascontiguousarray(tables.upd, dtype=np.int32)
+    d12 = np.ascontiguousarray(tables.d12_blk, dtype=np.int32)
+    d21 = np.ascontiguousarray(tables.d21_blk, dtype=np.int32)
     rc = lib.hps_merge_level(
         hd, dtype_code(dt), kind, nm, nc, nec, ni, ne, nsrc,
         ptr(T_in), ptr(h_in), ptr(rows), ptr(cols), ptr(ext), ptr(cols_final), ptr(final_of_internal),
         upd.shape[0], upd.ctypes.data_as(ctypes.c_void_p),
+        d12.shape[0], d12.ctypes.data_as(ctypes.c_void_p), d21.shape[0], d21.ctypes.data_as(ctypes.c_void_p),
         ptr(S), ptr(T), ptr(ho), ptr(gt), ptr(F_lu), ptr(perm), ptr(dinv), ptr(info),
         ptr(D12), ptr(D21), ptr(B), ptr(ws), ws.numel(),
     )

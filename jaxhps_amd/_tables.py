@@ -86,6 +86,13 @@ class MergeTables:
             np.arange(self.ni, dtype=np.int32) if final_of_internal is None else final_of_internal.astype(np.int32)
         )
         self.upd = _sparse_blocks(self.ext, self.cols_final, block) if block else np.zeros((0, 4), np.int32)
+        # ItI couplings: rectangles worth a product of their own once a block
+        # spans at least one 64-row GEMM tile
+        self.d12_blk = self.d21_blk = np.zeros((0, 4), np.int32)
+        if kind == "iti" and block and block >= _COUPLING_MIN_BLOCK:
+            nf = self.ni // 2
+            self.d12_blk = _coupling_blocks(int_rows[:nf], int_cols[nf:], block)
+            self.d21_blk = _coupling_blocks(int_rows[nf:], int_cols[:nf], block)
         self.adj_src = _adj_src(nc, nec, int_rows, ext)
         self._dev = {}
 
@@ -135,6 +142,38 @@ def _sparse_blocks(ext, cols_final, blk):
             rects.append((e * blk, (e1 - e + 1) * blk, us[i] * blk, (us[j] - us[i] + 1) * blk))
             i = j + 1
         e = e1 + 1
+    return np.array(rects, dtype=np.int32).reshape(-1, 4)
+
+
+_COUPLING_MIN_BLOCK = 64
+
+
+def _coupling_blocks(rows, cols, blk):
+    """Nonzero rectangles of the coupling block gather(rows, cols): entry
+    (i, j) is nonzero only where an equation of rows[i] reads the child that
+    unknown cols[j] feeds.  One rectangle per row block (its nonzero column
+    blocks must form one run), consecutive row blocks with the same run merged;
+    empty when a row block's columns do not form a single run."""
+    nr, nk = rows.shape[0], cols.shape[0]
+    if nr % blk or nk % blk:
+        return np.zeros((0, 4), np.int32)
+    rb = [set().union(*[_children(rows[i]) for i in range(b * blk, (b + 1) * blk)]) for b in range(nr // blk)]
+    kb = [set().union(*[_children(cols[i]) for i in range(b * blk, (b + 1) * blk)]) for b in range(nk // blk)]
+    runs = []
+    for r in rb:
+        ks = [k for k in range(len(kb)) if r & kb[k]]
+        if not ks or ks != list(range(ks[0], ks[-1] + 1)):
+            return np.zeros((0, 4), np.int32)
+        runs.append((ks[0], ks[-1] + 1))
+    rects = []
+    b = 0
+    while b < len(runs):
+        e = b
+        while e + 1 < len(runs) and runs[e + 1] == runs[b]:
+            e += 1
+        k0, k1 = runs[b]
+        rects.append((b * blk, (e - b + 1) * blk, k0 * blk, (k1 - k0) * blk))
+        b = e + 1
     return np.array(rects, dtype=np.int32).reshape(-1, 4)
 
 

@@ -568,12 +568,33 @@ static hipError_t gemm_call(hipStream_t s, int M, int N, int K, double alpha, co
 
 }  // namespace hps
 
+// Products with the block-sparse ItI couplings: D = I + [[0, D12], [D21, 0]]
+// where row block i of D12 (D21) is nonzero only in one contiguous run of
+// column blocks.  blk lists (r0, nr, k0, nk) rectangles that cover every row
+// exactly once; n_blk = 0 means dense.  Out[r] = C[r] + alpha Dxy[r, k] X[k].
+template <typename T>
+static hipError_t coupling_gemm(hipStream_t s, int n_blk, const int* blk, int nF, int w, double alpha, const T* Dxy,
+                                long sD, const T* X, long ldx, long sX, const T* C, long ldc, long sC, T* Out,
+                                long ldo, long sO, int nm) {
+  if (n_blk <= 0)
+    return gemm_call<T>(s, nF, w, nF, alpha, Dxy, nF, sD, X, ldx, sX, C ? 1.0 : 0.0, C, ldc, sC, Out, ldo, sO, nm);
+  for (int u = 0; u < n_blk; ++u) {
+    const int r0 = blk[4 * u], nr = blk[4 * u + 1], k0 = blk[4 * u + 2], nk = blk[4 * u + 3];
+    hipError_t e = gemm_call<T>(s, nr, w, nk, alpha, Dxy + (long)r0 * nF + k0, nF, sD, X + (long)k0 * ldx, ldx, sX,
+                                C ? 1.0 : 0.0, C ? C + (long)r0 * ldc : nullptr, ldc, sC, Out + (long)r0 * ldo, ldo,
+                                sO, nm);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 template <typename T>
 static int merge_t(hps_handle_t h, int kind, int nm, int nc, int nec, int ni, int ne, int nsrc, const void* T_in,
                    const void* h_in, const int* int_rows, const int* int_cols, const int* ext,
-                   const int* cols_final, const int* final_of_internal, int n_upd, const int* upd, void* S_out,
-                   void* T_out, void* h_out, void* gt_out, void* F_lu, int* perm, void* F_dinv, int* info,
-                   void* D12_out, void* D21_out, void* B_out, void* ws) {
+                   const int* cols_final, const int* final_of_internal, int n_upd, const int* upd, int n_d12,
+                   const int* d12_blk, int n_d21, const int* d21_blk, void* S_out, void* T_out, void* h_out,
+                   void* gt_out, void* F_lu, int* perm, void* F_dinv, int* info, void* D12_out, void* D21_out,
+                   void* B_out, void* ws) {
   hipStream_t s = h->stream;
   const T* Tin = (const T*)T_in;
   const int4* rows = (const int4*)int_rows;
@@ -612,8 +633,8 @@ static int merge_t(hps_handle_t h, int kind, int nm, int nc, int nec, int ni, in
     HPS_CHECK_HIP(h, gather_launch<T>(s, "D12", Tin, nc, nec, rows, nF, cols + nF, nF, 1.0, 0, D12, nF, sNF2, nm));
     HPS_CHECK_HIP(h, gather_launch<T>(s, "D21", Tin, nc, nec, rows + nF, nF, cols, nF, 1.0, 0, D21, nF, sNF2, nm));
     // W = I - D12 D21
-    HPS_CHECK_HIP(h, gemm_call<T>(s, nF, nF, nF, -1.0, D12, nF, sNF2, D21, nF, sNF2, 0.0, nullptr, 0, 0, Fl, nF,
-                                  sNF2, nm));
+    HPS_CHECK_HIP(h, coupling_gemm<T>(s, n_d12, d12_blk, nF, nF, -1.0, D12, sNF2, D21, nF, sNF2, nullptr, 0, 0, Fl, nF,
+                                      sNF2, nm));
     hipLaunchKernelGGL((add_identity_kernel<T>), dim3(ceil_div(nF, 256), nm), dim3(256), 0, s, Fl, nF, sNF2, nm);
     HPS_CHECK_HIP(h, hipGetLastError());
   }
@@ -642,13 +663,13 @@ static int merge_t(hps_handle_t h, int kind, int nm, int nc, int nec, int ni, in
     } else {
       const long sZ = (long)nF * w;
       // Z = Y_F - D12 Y_G
-      HPS_CHECK_HIP(h, gemm_call<T>(s, nF, w, nF, -1.0, D12, nF, sNF2, Yp + (long)nF * w, w, sY, 1.0, Yp, w, sY, Zp, w,
-                                    sZ, nm));
+      HPS_CHECK_HIP(h, coupling_gemm<T>(s, n_d12, d12_blk, nF, w, -1.0, D12, sNF2, Yp + (long)nF * w, w, sY, Yp, w, sY,
+                                        Zp, w, sZ, nm));
       // X_F = W^{-1} Z  (into Y_F)
       HPS_CHECK_HIP(h, getrs_perm<T>(nF, w, Fl, nF, sNF2, perm, dinv, Zp, w, sZ, Yp, w, sY, nm, s));
       // X_G = Y_G - D21 X_F  (in place)
-      HPS_CHECK_HIP(h, gemm_call<T>(s, nF, w, nF, -1.0, D21, nF, sNF2, Yp, w, sY, 1.0, Yp + (long)nF * w, w, sY,
-                                    Yp + (long)nF * w, w, sY, nm));
+      HPS_CHECK_HIP(h, coupling_gemm<T>(s, n_d21, d21_blk, nF, w, -1.0, D21, sNF2, Yp, w, sY, Yp + (long)nF * w, w,
+                                        sY, Yp + (long)nF * w, w, sY, nm));
       // internal -> final unknown order
       static const std::string nms = std::string("row_scatter<") + tname<T>() + ">";
       prof::Scope ps(s, nms.c_str(), 0.0, 2.0 * es * (double)ni * w * nm);
@@ -695,9 +716,10 @@ static int merge_t(hps_handle_t h, int kind, int nm, int nc, int nec, int ni, in
 
 int hps_merge_level(hps_handle_t h, int dtype, int kind, int nm, int nc, int nec, int ni, int ne, int nsrc,
                     const void* T_in, const void* h_in, const int* int_rows, const int* int_cols, const int* ext,
-                    const int* cols_final, const int* final_of_internal, int n_upd, const int* upd, void* S_out,
-                    void* T_out, void* h_out, void* gt_out, void* F_lu, int* perm, void* F_dinv, int* info,
-                    void* D12_out, void* D21_out, void* B_out, void* ws, size_t ws_bytes) {
+                    const int* cols_final, const int* final_of_internal, int n_upd, const int* upd, int n_d12,
+                    const int* d12_blk, int n_d21, const int* d21_blk, void* S_out, void* T_out, void* h_out,
+                    void* gt_out, void* F_lu, int* perm, void* F_dinv, int* info, void* D12_out, void* D21_out,
+                    void* B_out, void* ws, size_t ws_bytes) {
   if (!h) return HPS_ERR_ARG;
   HPS_ARG(h, kind == HPS_KIND_DTN || kind == HPS_KIND_ITI, "kind must be DtN or ItI");
   HPS_ARG(h, nm > 0 && nc > 0 && nec > 0 && ni > 0 && ne > 0 && nsrc >= 0, "bad merge sizes");
@@ -710,13 +732,31 @@ int hps_merge_level(hps_handle_t h, int dtype, int kind, int nm, int nc, int nec
     HPS_ARG(h, upd[4 * u] >= 0 && upd[4 * u + 1] > 0 && upd[4 * u] + upd[4 * u + 1] <= ne && upd[4 * u + 2] >= 0 &&
                    upd[4 * u + 3] > 0 && upd[4 * u + 2] + upd[4 * u + 3] <= ni,
             "update block out of range");
+  HPS_ARG(h, n_d12 >= 0 && n_d21 >= 0 && (n_d12 == 0 || d12_blk) && (n_d21 == 0 || d21_blk), "bad coupling blocks");
+  HPS_ARG(h, kind == HPS_KIND_ITI || (n_d12 == 0 && n_d21 == 0), "coupling blocks are for ItI merges");
+  // every row of D12 / D21 must lie in exactly one rectangle
+  for (int pass = 0; pass < 2; ++pass) {
+    const int nb_ = pass == 0 ? n_d12 : n_d21;
+    const int* bl = pass == 0 ? d12_blk : d21_blk;
+    if (nb_ == 0) continue;
+    long covered = 0;
+    for (int u = 0; u < nb_; ++u) {
+      const int r0 = bl[4 * u], nr = bl[4 * u + 1], k0 = bl[4 * u + 2], nk = bl[4 * u + 3];
+      HPS_ARG(h, r0 >= 0 && nr > 0 && r0 + nr <= ni / 2 && k0 >= 0 && nk > 0 && k0 + nk <= ni / 2,
+              "coupling block out of range");
+      for (int v = 0; v < u; ++v)
+        HPS_ARG(h, r0 >= bl[4 * v] + bl[4 * v + 1] || r0 + nr <= bl[4 * v], "coupling blocks overlap in rows");
+      covered += nr;
+    }
+    HPS_ARG(h, covered == ni / 2, "coupling blocks must cover every row once");
+  }
   if (ws_bytes < hps_merge_workspace(dtype, kind, nm, ni, ne, nsrc)) {
     h->err = "merge workspace too small";
     return HPS_ERR_WORKSPACE;
   }
   DISPATCH(dtype, merge_t, h, kind, nm, nc, nec, ni, ne, nsrc, T_in, h_in, int_rows, int_cols, ext, cols_final,
-           final_of_internal, n_upd, upd, S_out, T_out, h_out, gt_out, F_lu, perm, F_dinv, info, D12_out, D21_out,
-           B_out, ws);
+           final_of_internal, n_upd, upd, n_d12, d12_blk, n_d21, d21_blk, S_out, T_out, h_out, gt_out, F_lu, perm,
+           F_dinv, info, D12_out, D21_out, B_out, ws);
 }
 
 // ---------------------------------------------------------------------------

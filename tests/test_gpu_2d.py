@@ -98,6 +98,26 @@ def test_merge_stage_matches_oracle(gpu, kind, p, L):
     assert _rel(_np(Ttop), ref["Ttop"]) < 1e-10
 
 
+def test_iti_block_sparse_couplings_match_oracle(gpu):
+    """Merges whose children have >= 64 nodes per side multiply only the
+    nonzero rectangles of D12 / D21; fed the oracle's leaf maps, the GPU merge
+    stage must reproduce the oracle's dense one to rounding."""
+    from jaxhps_amd._tables import quad_merge_tables
+    from jaxhps_amd.merge import merge_stage_uniform_2D_ItI
+
+    print("iti block-sparse couplings p=16 L=4", flush=True)
+    assert len(quad_merge_tables(112, "iti").d12_blk) > 0
+    dom, pde = _problem(16, 14, 4, "iti")
+    ref = _oracle(pde, "iti")
+    T = torch.as_tensor(ref["T"]).to(gpu)
+    h = torch.as_tensor(ref["h"][..., 0]).to(gpu)
+    S_lst, g_lst, Ttop = merge_stage_uniform_2D_ItI(T, h, 4, return_T=True)
+    for lvl in range(4):
+        assert _rel(_np(S_lst[lvl]), ref["S"][lvl]) < 1e-11
+        assert _rel(_np(g_lst[lvl]), ref["g"][lvl][..., 0]) < 1e-11
+    assert _rel(_np(Ttop), ref["Ttop"]) < 1e-11
+
+
 @pytest.mark.parametrize("p,L", [(6, 1), (8, 2), (16, 3)])
 def test_dtn_end_to_end_closed_form(gpu, p, L):
     import jaxhps_amd as jh

@@ -131,6 +131,15 @@ def _emulate_merge(Tc, hc, tb):
         nF = ni // 2
         D12 = _gather(Tc, tb.int_rows[:nF], tb.int_cols[nF:])
         D21 = _gather(Tc, tb.int_rows[nF:], tb.int_cols[:nF])
+        # the coupling rectangles hold every nonzero and cover each row once
+        for blk, Dm in ((tb.d12_blk, D12), (tb.d21_blk, D21)):
+            if len(blk):
+                Ds, seen = np.zeros_like(Dm), np.zeros(nF, dtype=int)
+                for r0, nr, k0, nk in blk:
+                    Ds[r0:r0 + nr, k0:k0 + nk] = Dm[r0:r0 + nr, k0:k0 + nk]
+                    seen[r0:r0 + nr] += 1
+                assert np.array_equal(Ds, Dm) and (seen == 1).all()
+                assert Ds.size > 2 * sum(int(nr) * int(nk) for _, nr, _, nk in blk) - 1  # half the block is zero
         W = np.eye(nF) - D12 @ D21
         out = []
         for y in (Y, Yh):
@@ -153,7 +162,7 @@ def _emulate_merge(Tc, hc, tb):
 
 
 @pytest.mark.parametrize("kind", ["dtn", "iti"])
-@pytest.mark.parametrize("m", [1, 3])
+@pytest.mark.parametrize("m", [1, 3, 64])
 def test_merge_tables_emulation_matches_oracle(kind, m):
     from jaxhps_amd._tables import quad_merge_tables
 
