@@ -1408,6 +1408,94 @@ __global__ __launch_bounds__(256) void getrs_small_kernel(int n, int nrhs, const
   }
 }
 
+// In-place triangular solve of the diagonal block [k0, k0 + nbk) of X for few
+// right-hand sides (the inner steps of the blocked solve of large systems):
+// unit-lower L forward or U backward, 32-row sub-blocks through the stored
+// inverses, the rows of the block below (above) updated by per-thread dot
+// products; the X block lives in LDS.  One workgroup per matrix.
+template <typename T, bool UPPER>
+__global__ __launch_bounds__(256) void getrs_block_kernel(int n, int nrhs, const T* LU, long lda, long sA,
+                                                          const T* dinv, T* X, long ldx, long sX, int k0, int nbk) {
+  extern __shared__ char smem_raw[];
+  T* xs = (T*)smem_raw;  // (nbk, nrhs)
+  __shared__ T Di[LU_NB][LU_NB + 1];
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x;
+  const T* A = LU + (long)b * sA;
+  const int nblk = (n + LU_NB - 1) / LU_NB;
+  const T* db = dinv + (long)b * nblk * LU_NB * LU_NB;
+  T* Xb = X + (long)b * sX;
+  for (int idx = tid; idx < nbk * nrhs; idx += 256) {
+    const int i = idx / nrhs, c = idx - i * nrhs;
+    xs[idx] = Xb[(long)(k0 + i) * ldx + c];
+  }
+  __syncthreads();
+  const int nsub = (nbk + LU_NB - 1) / LU_NB;
+  for (int sb = 0; sb < nsub; ++sb) {
+    const int kl = (UPPER ? nsub - 1 - sb : sb) * LU_NB;  // sub-block start within the block
+    const int nb = nbk - kl < LU_NB ? nbk - kl : LU_NB;
+    const T* dk = db + (long)((k0 + kl) / LU_NB) * LU_NB * LU_NB;
+    for (int idx = tid; idx < LU_NB * LU_NB; idx += 256) {
+      const int r = idx / LU_NB, q = idx - r * LU_NB;
+      T v;
+      if (UPPER)
+        v = q >= r ? dk[idx] : s_zero<T>();
+      else
+        v = q < r ? dk[idx] : (q == r ? s_one<T>() : s_zero<T>());
+      Di[r][q] = v;
+    }
+    __syncthreads();
+    T y[LU_SMALL_MAX_RHS / 4];
+    const int per = (nb * nrhs + 255) / 256;  // <= 2 for nb <= 32, nrhs <= 16
+    for (int e = 0; e < per; ++e) {
+      const int idx = tid + 256 * e;
+      T acc = s_zero<T>();
+      if (idx < nb * nrhs) {
+        const int r = idx / nrhs, c = idx - r * nrhs;
+        for (int q = 0; q < nb; ++q) acc = s_add(acc, s_mul(Di[r][q], xs[(kl + q) * nrhs + c]));
+      }
+      y[e] = acc;
+    }
+    __syncthreads();
+    for (int e = 0; e < per; ++e) {
+      const int idx = tid + 256 * e;
+      if (idx < nb * nrhs) xs[(kl + idx / nrhs) * nrhs + idx % nrhs] = y[e];
+    }
+    __syncthreads();
+    const int r0 = UPPER ? 0 : kl + nb;
+    const int r1 = UPPER ? kl : nbk;
+    for (int i = r0 + tid; i < r1; i += 256) {
+      for (int c = 0; c < nrhs; ++c) {
+        T acc = xs[i * nrhs + c];
+        const T* row = A + (long)(k0 + i) * lda + k0 + kl;
+        for (int q = 0; q < nb; ++q) acc = s_fnms(acc, row[q], xs[(kl + q) * nrhs + c]);
+        xs[i * nrhs + c] = acc;
+      }
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < nbk * nrhs; idx += 256) {
+    const int i = idx / nrhs, c = idx - i * nrhs;
+    Xb[(long)(k0 + i) * ldx + c] = xs[idx];
+  }
+}
+
+template <typename T, bool UPPER>
+static hipError_t getrs_block(int n, int nrhs, const T* LU, long lda, long sA, const T* dinv, T* X, long ldx, long sX,
+                              int k0, int nbk, int batch, hipStream_t s) {
+  static const std::string nm = std::string(UPPER ? "getrs_block_upper<" : "getrs_block_lower<") + tname<T>() + ">";
+  prof::Scope ps(s, nm.c_str(), flop_mult<T>() * batch * (double)nbk * nbk * nrhs,
+                 sizeof(T) * (double)batch * ((double)nbk * nbk / 2 + 2.0 * nbk * nrhs));
+  const int nblk = (n + LU_NB - 1) / LU_NB;
+  for (int b0 = 0; b0 < batch; b0 += 65535) {
+    const int bc = batch - b0 < 65535 ? batch - b0 : 65535;
+    hipLaunchKernelGGL((getrs_block_kernel<T, UPPER>), dim3(bc), dim3(256), (size_t)nbk * nrhs * sizeof(T), s, n,
+                       nrhs, LU + (long)b0 * sA, lda, sA, dinv + (long)b0 * nblk * LU_NB * LU_NB, X + (long)b0 * sX,
+                       ldx, sX, k0, nbk);
+  }
+  return hipGetLastError();
+}
+
 template <typename T>
 static bool getrs_small_ok(int n, int nrhs) {
   return n <= LU_SMALL_MAX_N && nrhs <= LU_SMALL_MAX_RHS && (size_t)n * nrhs * sizeof(T) <= 64 * 1024;
@@ -1470,9 +1558,15 @@ hipError_t getrs_perm(int n, int nrhs, const T* LU, long lda, long sA, const int
   // steps update only the rows of their block; the rest of X is updated once
   // per block by a GEMM with K = LU_NBO.
   const int nbo = lu_outer_block(n);
+  // few right-hand sides: each outer block's inner steps in one kernel
+  const bool few = nrhs <= LU_SMALL_MAX_RHS && (size_t)nbo * nrhs * sizeof(T) <= 64 * 1024;
   for (int k0 = 0; k0 < n; k0 += nbo) {
     const int bend = n - k0 < nbo ? n : k0 + nbo;
-    for (int k = k0; k < bend; k += LU_NB) {
+    if (few) {
+      hipError_t e = getrs_block<T, false>(n, nrhs, LU, lda, sA, dinv, X, ldx, sX, k0, bend - k0, batch, s);
+      if (e != hipSuccess) return e;
+    }
+    for (int k = k0; k < bend && !few; k += LU_NB) {
       const int nb = bend - k < LU_NB ? bend - k : LU_NB;
       static const std::string nm = std::string("trsm_lower<") + tname<T>() + ">";
       hipError_t e = trsm_block<T>(false, n, nrhs, dinv, X, ldx, sX, k, nb, batch, s, nm);
@@ -1490,7 +1584,11 @@ hipError_t getrs_perm(int n, int nrhs, const T* LU, long lda, long sA, const int
     const int k0 = bo * nbo;
     const int bend = n - k0 < nbo ? n : k0 + nbo;
     const int last = ((bend - 1 - k0) / LU_NB) * LU_NB + k0;
-    for (int k = last; k >= k0; k -= LU_NB) {
+    if (few) {
+      hipError_t e = getrs_block<T, true>(n, nrhs, LU, lda, sA, dinv, X, ldx, sX, k0, bend - k0, batch, s);
+      if (e != hipSuccess) return e;
+    }
+    for (int k = last; k >= k0 && !few; k -= LU_NB) {
       const int nb = bend - k < LU_NB ? bend - k : LU_NB;
       static const std::string nm = std::string("trsm_upper<") + tname<T>() + ">";
       hipError_t e = trsm_block<T>(true, n, nrhs, dinv, X, ldx, sX, k, nb, batch, s, nm);
