@@ -233,9 +233,8 @@ __device__ void build_swap_table(int k, int nb, const int* ipivb, int* tab) {
 
 // Invert the factored nb x nb diagonal block D (row stride ldd) into the
 // packed form out (row stride ldo): strict lower = inv(L11), upper = inv(U11).
-// Needs 256 threads: four consecutive lanes (a DPP quad) form one column, of
-// inv(L11) for threads < 128 and of inv(U11) above; lane s of the quad keeps
-// the column's entries j = 4q + s in registers.  Each entry is one unrolled
+// Four consecutive lanes (a DPP quad) form one column; lane s of the quad
+// keeps the column's entries j = 4q + s in registers.  Each entry is one unrolled
 // substitution step whose dot product is split over the quad and summed with
 // two quad exchanges.  Entries not yet formed are zero, so the sums need no
 // bounds: for column c the rows above c stay zero in inv(L11) and the rows
@@ -265,44 +264,49 @@ __device__ __forceinline__ void invert_diag_block(const T* D, int ldd, T* out, i
   const int t = threadIdx.x;
   if (t < LU_NB) invd[t] = t < nb ? s_inv(D[t * ldd + t]) : s_zero<T>();
   __syncthreads();
-  if (t >= 256) return;
-  const bool upper = t >= 128;
-  const int c = (t & 127) / P, sl = t % P;
-  T xs[Q];
+  // 64 column jobs (32 of inv(L11), then 32 of inv(U11)) over the quads; with
+  // 64, 128 or 256 threads every wave takes whole rounds of one kind
+  const int nquad = (int)blockDim.x / P < 64 ? (int)blockDim.x / P : 64;
+  const int sl = t % P;
+  for (int job = t / P; job < 64 && t < nquad * P; job += nquad) {
+    const bool upper = job >= LU_NB;
+    const int c = job % LU_NB;
+    T xs[Q];
 #pragma unroll
-  for (int q = 0; q < Q; ++q) xs[q] = s_zero<T>();
-  if (!upper) {
+    for (int q = 0; q < Q; ++q) xs[q] = s_zero<T>();
+    if (!upper) {
 #pragma unroll
-    for (int i = 0; i < LU_NB; ++i) {
-      T acc = s_zero<T>();
+      for (int i = 0; i < LU_NB; ++i) {
+        T acc = s_zero<T>();
 #pragma unroll
-      for (int q = 0; q * P < i; ++q) acc = s_fnms(acc, D[i * ldd + q * P + sl], xs[q]);
-      acc = quad_sum(acc);
-      if (sl == i % P) xs[i / P] = i == c ? s_one<T>() : acc;
+        for (int q = 0; q * P < i; ++q) acc = s_fnms(acc, D[i * ldd + q * P + sl], xs[q]);
+        acc = quad_sum(acc);
+        if (sl == i % P) xs[i / P] = i == c ? s_one<T>() : acc;
+      }
+#pragma unroll
+      for (int q = 0; q < Q; ++q)
+        if (q * P + sl > c) out[(q * P + sl) * ldo + c] = xs[q];
+    } else {
+#pragma unroll
+      for (int i = LU_NB - 1; i >= 0; --i) {
+        T acc = s_zero<T>();
+#pragma unroll
+        for (int q = (i + 1) / P; q < Q; ++q) acc = s_fnms(acc, D[i * ldd + q * P + sl], xs[q]);
+        acc = quad_sum(acc);
+        if (i == c) acc = s_add(acc, s_one<T>());
+        if (sl == i % P) xs[i / P] = s_mul(acc, invd[i]);
+      }
+#pragma unroll
+      for (int q = 0; q < Q; ++q)
+        if (q * P + sl <= c) out[(q * P + sl) * ldo + c] = xs[q];
     }
-#pragma unroll
-    for (int q = 0; q < Q; ++q)
-      if (q * P + sl > c) out[(q * P + sl) * ldo + c] = xs[q];
-  } else {
-#pragma unroll
-    for (int i = LU_NB - 1; i >= 0; --i) {
-      T acc = s_zero<T>();
-#pragma unroll
-      for (int q = (i + 1) / P; q < Q; ++q) acc = s_fnms(acc, D[i * ldd + q * P + sl], xs[q]);
-      acc = quad_sum(acc);
-      if (i == c) acc = s_add(acc, s_one<T>());
-      if (sl == i % P) xs[i / P] = s_mul(acc, invd[i]);
-    }
-#pragma unroll
-    for (int q = 0; q < Q; ++q)
-      if (q * P + sl <= c) out[(q * P + sl) * ldo + c] = xs[q];
   }
 }
 
 // Diagonal-block inverse -> dinv, and the panel's swap table tab (LDS:
 // count, then (row, source) pairs) -> swtab.  D is the final factored
 // diagonal block in LDS (a 32 x 32 array; zeroed outside nb x nb here); Out
-// is LDS scratch of nb rows.  At least 256 threads.
+// is LDS scratch of nb rows.  64, 128, 256 or more threads.
 template <typename T>
 __device__ __forceinline__ void finish_panel_tab(T* D, int ldd, T* Out, int ldo, int k, int nb, int n, int b,
                                                  const int* tab, T* dinv, int* swtab) {
@@ -328,7 +332,7 @@ __device__ __forceinline__ void finish_panel_tab(T* D, int ldd, T* Out, int ldo,
 }
 
 // finish_panel_tab with the swap table built from ipiv, for panels that swap
-// rows explicitly (at least 256 threads).
+// rows explicitly.
 template <typename T>
 __device__ __forceinline__ void finish_panel(T* D, int ldd, T* Out, int ldo, int k, int nb, int n, int b,
                                              const int* ipivb, T* dinv, int* swtab) {
@@ -751,14 +755,14 @@ __global__ __launch_bounds__(THREADS) void getrf_panel_rows_kernel(int n, T* A, 
       for (int c = 0; c < LU_NB; ++c) Dblk[pos[r]][c] = c < nb ? x[r][c] : s_zero<T>();
   }
   finish_panel_tab<T>(&Dblk[0][0], LU_NB + 1, &Dout[0][0], LU_NB + 1, k, nb, n, b, tab, dinv, swtab);
-  // the panel's row permutation on every other column of the matrix, 32
+  // the panel's row permutation on every other column of the matrix, CW
   // columns at a time: all moved rows are read before any is written
-  constexpr int RG = THREADS / 32, NV = 2 * LU_NB / RG;
+  constexpr int CW = THREADS >= 128 ? 32 : 16, RG = THREADS / CW, NV = 2 * LU_NB / RG;
   const int cnt = tab[0] < 2 * LU_NB ? tab[0] : 2 * LU_NB;
   const int others = n - nb;
   T* Ab = A + (long)b * sA;
-  const int cc = t & 31, a0 = t >> 5;
-  for (int c0 = 0; c0 < others; c0 += 32) {
+  const int cc = t % CW, a0 = t / CW;
+  for (int c0 = 0; c0 < others; c0 += CW) {
     const int oc = c0 + cc;
     const int col = oc < k ? oc : oc + nb;
     const bool ok = oc < others;
@@ -1083,7 +1087,11 @@ static hipError_t launch_panel(int n, T* A, long lda, long sA, int k, int nb, in
   if (m <= rows_panel_max<T>()) {
     static const std::string nm_rows = std::string("getrf_panel_rows<") + tname<T>() + ">";
     prof::Scope ps(s, nm_rows.c_str(), pf, 2.0 * sizeof(T) * (double)m * nb * batch);
-    if (m <= 256)
+    // short panels in smaller workgroups: more of them share a CU
+    if (m <= 128)
+      hipLaunchKernelGGL((getrf_panel_rows_kernel<T, 1, 128>), dim3(batch), dim3(128), 0, s, n, A, lda, sA, k, nb,
+                         ipiv, info, dinv, swtab);
+    else if (m <= 256)
       hipLaunchKernelGGL((getrf_panel_rows_kernel<T, 1, 256>), dim3(batch), dim3(256), 0, s, n, A, lda, sA, k, nb,
                          ipiv, info, dinv, swtab);
     else if (m <= 512)
