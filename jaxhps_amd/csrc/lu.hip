@@ -842,19 +842,23 @@ __global__ __launch_bounds__(256) void getrf_swap_u12_kernel(int n, T* A, long l
 // other row gets its multipliers L21 = A21 inv(U11) independently.
 // ---------------------------------------------------------------------------
 
-// Partial pivoting over the rows held one per thread (x = row, valid if
-// row_id >= 0): the nb pivot rows' ids, in order, to out[0..nb) (-1 once no
-// valid row remains).
-template <typename T>
-__device__ __forceinline__ void tslu_select(T (&x)[1][LU_NB], int row_id, int nb, int* out) {
+// Partial pivoting over the rows held one per thread of a THREADS-wide
+// workgroup (x = row, valid if row_id >= 0): the nb pivot rows' ids, in
+// order, to out[0..nb) (-1 once no valid row remains); pos ends as the row's
+// position in the eliminated candidate block, its x as that row of the block's
+// LU.  With info set, an exactly zero pivot is reported there.
+template <typename T, int THREADS>
+__device__ __forceinline__ void tslu_select(T (&x)[1][LU_NB], int (&pos)[1], int row_id, int nb, int* out,
+                                            int* info, int b, int k) {
   using R = typename real_of<T>::type;
-  __shared__ PanelLDS<T, 4> sh;
-  int pos[1] = {(int)threadIdx.x};
+  __shared__ PanelLDS<T, THREADS / 64> sh;
+  pos[0] = (int)threadIdx.x;
   const bool live[1] = {row_id >= 0};
-  auto on_pivot = [&](int J, int, R, unsigned mask) {
+  auto on_pivot = [&](int J, int, R fv, unsigned mask) {
     if (mask) out[J] = row_id;
+    if (info && threadIdx.x == 0 && fv == R(0) && info[b] == 0) info[b] = k + J + 1;
   };
-  panel_steps<T, 1, 4>(std::make_integer_sequence<int, LU_NB>{}, x, pos, live, sh, nb, on_pivot);
+  panel_steps<T, 1, THREADS / 64>(std::make_integer_sequence<int, LU_NB>{}, x, pos, live, sh, nb, on_pivot);
   __syncthreads();
 }
 
@@ -867,144 +871,108 @@ __global__ __launch_bounds__(256) void tslu_leaf_kernel(int n, const T* A, long 
   const int r = k + c * TS_CHUNK + t;
   const T* Ab = A + (long)b * sA;
   T x[1][LU_NB];
+  int pos[1];
   const bool live = r < n;
 #pragma unroll
   for (int q = 0; q < LU_NB; ++q) x[0][q] = (live && q < nb) ? Ab[(long)r * lda + k + q] : s_zero<T>();
-  tslu_select<T>(x, live ? r : -1, nb, out);
+  tslu_select<T, 256>(x, pos, live ? r : -1, nb, out, nullptr, b, k);
   if (t < nb) cand_out[((long)b * nchunks + c) * LU_NB + t] = out[t];
 }
 
-// Later rounds: group g merges up to 8 candidate sets of cand_in (nsets per matrix)
-template <typename T>
-__global__ __launch_bounds__(256) void tslu_reduce_kernel(int n, const T* A, long lda, long sA, int k, int nb,
-                                                          int nsets, const int* cand_in, int ngroups,
-                                                          int* cand_out) {
+// Later rounds: group g merges up to TS_GROUP candidate sets of cand_in
+// (nsets per matrix).  The last round (one group) also finishes the panel:
+// its winners' rows already hold the LU of the top block in selection order
+// (the elimination the unpivoted factorisation would repeat), so it writes
+// them to the top rows, runs the LAPACK-style swap sequence on the at most 64
+// positions it touches (one lane each in the first wave: lane j < 32 is
+// position k+j, lane 32+j the original position of winner j below the top
+// block), moves the displaced top rows, and writes dinv and the swap table.
+constexpr int TS_GROUP = 16;
+
+template <typename T, bool LAST>
+__global__ __launch_bounds__(512) void tslu_reduce_kernel(int n, T* A, long lda, long sA, int k, int nb, int nsets,
+                                                          const int* cand_in, int ngroups, int* cand_out, int* ipiv,
+                                                          int* info, T* dinv, int* swtab) {
   __shared__ int out[LU_NB];
   const int b = blockIdx.y, g = blockIdx.x, t = threadIdx.x;
-  const int set = g * 8 + t / LU_NB, slot = t % LU_NB;
+  const int set = g * TS_GROUP + t / LU_NB, slot = t % LU_NB;
   int r = -1;
   if (set < nsets && slot < nb) r = cand_in[((long)b * nsets + set) * LU_NB + slot];
-  const T* Ab = A + (long)b * sA;
+  T* Ab = A + (long)b * sA;
   T x[1][LU_NB];
+  int pos[1];
 #pragma unroll
   for (int q = 0; q < LU_NB; ++q) x[0][q] = (r >= 0 && q < nb) ? Ab[(long)r * lda + k + q] : s_zero<T>();
-  tslu_select<T>(x, r, nb, out);
-  if (t < nb) cand_out[((long)b * ngroups + g) * LU_NB + t] = out[t];
-}
-
-// Unpivoted elimination step J of the top block, row r per lane, the pivot
-// row broadcast through LDS (double-buffered: one barrier per column).
-template <typename T, int J>
-__device__ __forceinline__ void nopiv_step(T (&x)[LU_NB], int r, int nb, T (&prow)[2][LU_NB], int k, int* info,
-                                           int b) {
-  constexpr int par = J & 1;
-  if (J >= nb) return;
-  if (r == J)
-#pragma unroll
-    for (int c = J; c < LU_NB; ++c) prow[par][c] = x[c];
-  __syncthreads();
-  const T piv = prow[par][J];
-  if (r == 0 && s_is_zero(piv) && info[b] == 0) info[b] = k + J + 1;
-  if (r > J && r < nb && !s_is_zero(piv)) {
-    const T l = s_mul(x[J], s_inv(piv));
-    x[J] = l;
-#pragma unroll
-    for (int c = J + 1; c < LU_NB; ++c) x[c] = s_fnms(x[c], l, prow[par][c]);
-  }
-}
-
-template <typename T, int... J>
-__device__ __forceinline__ void nopiv_steps(std::integer_sequence<int, J...>, T (&x)[LU_NB], int r, int nb,
-                                            T (&prow)[2][LU_NB], int k, int* info, int b) {
-  (nopiv_step<T, J>(x, r, nb, prow, k, info, b), ...);
-}
-
-// Winners -> ipiv and the swap table, row swaps inside the panel columns,
-// unpivoted LU of the top block (it reproduces the final round's
-// elimination), dinv.  One wave per matrix.  The LAPACK-style swap sequence
-// (step j exchanges row k+j with the current position of winner j) is run on
-// the at most 64 positions it touches, one per lane: lane j < 32 is position
-// k+j, lane 32+j the original position of winner j when that lies below the
-// top block.
-template <typename T>
-__global__ __launch_bounds__(256) void tslu_finish_kernel(int n, T* A, long lda, long sA, int k, int nb,
-                                                         const int* winners, int* ipiv, int* info, T* dinv,
-                                                         int* swtab) {
-  __shared__ int tab[SWTAB];
-  __shared__ T vals[2 * LU_NB][LU_NB];
-  __shared__ T prow[2][LU_NB];
-  __shared__ T blk[LU_NB][LU_NB + 1];
-  __shared__ T Dout[LU_NB][LU_NB + 1];
-  const int b = blockIdx.x, lane = threadIdx.x, tid = threadIdx.x;
-  T* Ab = A + (long)b * sA;
-  int* ipivb = ipiv + (long)b * n;
-  const int* wb = winners + (long)b * LU_NB;
-  auto winner = [&](int j) {
-    const int w = wb[j];
-    return (w < k || w >= n) ? k + j : w;  // no candidate left (cannot happen for m >= nb): keep the row
-  };
-  int spos = -1;
-  if (lane < nb) {
-    spos = k + lane;
-  } else if (lane >= LU_NB && lane - LU_NB < nb) {
-    const int w = winner(lane - LU_NB);
-    if (w >= k + nb) spos = w;
-  }
-  int scont = spos;
-  if (tid < 64) {  // the swap simulation runs in the first wave
-    for (int j = 0; j < nb; ++j) {
-      const int w = winner(j);
-      const unsigned long long hit = __ballot(spos >= 0 && scont == w);
-      if (hit == 0ull) {
-        if (lane == 0) ipivb[k + j] = k + j;  // not reached: every winner holds one lane
-        continue;
+  tslu_select<T, 512>(x, pos, r, nb, out, LAST ? info : nullptr, b, k);
+  if constexpr (!LAST) {
+    if (t < nb) cand_out[((long)b * ngroups + g) * LU_NB + t] = out[t];
+  } else {
+    __shared__ int tab[SWTAB];
+    __shared__ T vals[2 * LU_NB][LU_NB];
+    __shared__ T Dblk[LU_NB][LU_NB + 1];
+    __shared__ T Dout[LU_NB][LU_NB + 1];
+    int* ipivb = ipiv + (long)b * n;
+    const int lane = t;
+    auto winner = [&](int j) {
+      const int w = out[j];
+      return (w < k || w >= n) ? k + j : w;  // no candidate left (cannot happen for m >= nb): keep the row
+    };
+    if (t < 64) {
+      int spos = -1;
+      if (lane < nb) {
+        spos = k + lane;
+      } else if (lane >= LU_NB && lane - LU_NB < nb) {
+        const int w = winner(lane - LU_NB);
+        if (w >= k + nb) spos = w;
       }
-      const int L = __ffsll((long long)hit) - 1;
-      const int p = __shfl(spos, L);
-      if (lane == 0) ipivb[k + j] = p;
-      const int cj = __shfl(scont, j), cL = __shfl(scont, L);
-      if (lane == j)
-        scont = cL;
-      else if (lane == L)
-        scont = cj;
+      int scont = spos;
+      for (int j = 0; j < nb; ++j) {
+        const int w = winner(j);
+        const unsigned long long hit = __ballot(spos >= 0 && scont == w);
+        if (hit == 0ull) {
+          if (lane == 0) ipivb[k + j] = k + j;  // not reached: every winner holds one lane
+          continue;
+        }
+        const int L = __ffsll((long long)hit) - 1;
+        const int p = __shfl(spos, L);
+        if (lane == 0) ipivb[k + j] = p;
+        const int cj = __shfl(scont, j), cL = __shfl(scont, L);
+        if (lane == j)
+          scont = cL;
+        else if (lane == L)
+          scont = cj;
+      }
+      const bool moved = spos >= 0 && scont != spos;
+      const unsigned long long mv = __ballot(moved);
+      if (moved) {
+        const int i = __popcll(mv & ((1ull << lane) - 1ull));
+        tab[1 + 2 * i] = spos;
+        tab[2 + 2 * i] = scont;
+      }
+      if (lane == 0) tab[0] = __popcll(mv);
     }
-    // net permutation: (row, source) for every touched position whose content changed
-    const bool moved = spos >= 0 && scont != spos;
-    const unsigned long long mv = __ballot(moved);
-    if (moved) {
-      const int i = __popcll(mv & ((1ull << lane) - 1ull));
-      tab[1 + 2 * i] = spos;
-      tab[2 + 2 * i] = scont;
+    __syncthreads();
+    const int cnt = tab[0];
+    // rows displaced below the top block come from original top rows: read
+    // their panel columns before the winners overwrite the top rows
+    for (int idx = t; idx < cnt * LU_NB; idx += 512) {
+      const int a = idx / LU_NB, c = idx - a * LU_NB;
+      if (tab[1 + 2 * a] >= k + nb && c < nb) vals[a][c] = Ab[(long)tab[2 + 2 * a] * lda + k + c];
     }
-    if (lane == 0) tab[0] = __popcll(mv);
-  }
-  __syncthreads();
-  const int cnt = tab[0];
-  // swap inside the panel columns
-  for (int idx = tid; idx < cnt * nb; idx += 256) {
-    const int a = idx / nb, c = idx - a * nb;
-    vals[a][c] = Ab[(long)tab[2 + 2 * a] * lda + k + c];
-  }
-  __syncthreads();
-  for (int idx = tid; idx < cnt * nb; idx += 256) {
-    const int a = idx / nb, c = idx - a * nb;
-    Ab[(long)tab[1 + 2 * a] * lda + k + c] = vals[a][c];
-  }
-  __syncthreads();
-  // unpivoted LU of the top nb x nb block, row r per lane
-  T x[LU_NB];
-  const int r = lane;
+    __syncthreads();
+    for (int idx = t; idx < cnt * LU_NB; idx += 512) {
+      const int a = idx / LU_NB, c = idx - a * LU_NB;
+      if (tab[1 + 2 * a] >= k + nb && c < nb) Ab[(long)tab[1 + 2 * a] * lda + k + c] = vals[a][c];
+    }
+    if (r >= 0 && pos[0] < nb) {
 #pragma unroll
-  for (int c = 0; c < LU_NB; ++c) x[c] = (r < nb && c < nb) ? Ab[(long)(k + r) * lda + k + c] : s_zero<T>();
-  nopiv_steps<T>(std::make_integer_sequence<int, LU_NB>{}, x, r, nb, prow, k, info, b);
-  if (r < nb) {
-#pragma unroll
-    for (int c = 0; c < LU_NB; ++c) {
-      if (c < nb) Ab[(long)(k + r) * lda + k + c] = x[c];
-      blk[r][c] = x[c];
+      for (int c = 0; c < LU_NB; ++c) {
+        if (c < nb) Ab[(long)(k + pos[0]) * lda + k + c] = x[0][c];
+        Dblk[pos[0]][c] = c < nb ? x[0][c] : s_zero<T>();
+      }
     }
+    finish_panel_tab<T>(&Dblk[0][0], LU_NB + 1, &Dout[0][0], LU_NB + 1, k, nb, n, b, tab, dinv, swtab);
   }
-  finish_panel_tab<T>(&blk[0][0], LU_NB + 1, &Dout[0][0], LU_NB + 1, k, nb, n, b, tab, dinv, swtab);
 }
 
 // L21 = A21 inv(U11) for the panel rows below the top block, one row per thread
@@ -1045,17 +1013,20 @@ static hipError_t launch_tslu_panel(int n, T* A, long lda, long sA, int k, int n
   int* cur = cand;
   int* nxt = cand + tslu_cand_ints(n, batch);
   hipLaunchKernelGGL((tslu_leaf_kernel<T>), dim3(nsets, batch), dim3(256), 0, s, n, A, lda, sA, k, nb, nsets, cur);
-  while (nsets > 1) {
-    const int ngroups = (nsets + 7) / 8;
-    hipLaunchKernelGGL((tslu_reduce_kernel<T>), dim3(ngroups, batch), dim3(256), 0, s, n, A, lda, sA, k, nb, nsets,
-                       cur, ngroups, nxt);
+  for (;;) {
+    const int ngroups = (nsets + TS_GROUP - 1) / TS_GROUP;
+    if (ngroups == 1) {
+      hipLaunchKernelGGL((tslu_reduce_kernel<T, true>), dim3(1, batch), dim3(512), 0, s, n, A, lda, sA, k, nb, nsets,
+                         cur, 1, nullptr, ipiv, info, dinv, swtab);
+      break;
+    }
+    hipLaunchKernelGGL((tslu_reduce_kernel<T, false>), dim3(ngroups, batch), dim3(512), 0, s, n, A, lda, sA, k, nb,
+                       nsets, cur, ngroups, nxt, ipiv, info, dinv, swtab);
     int* tmp = cur;
     cur = nxt;
     nxt = tmp;
     nsets = ngroups;
   }
-  hipLaunchKernelGGL((tslu_finish_kernel<T>), dim3(batch), dim3(256), 0, s, n, A, lda, sA, k, nb, cur, ipiv, info,
-                     dinv, swtab);
   if (m > nb)
     hipLaunchKernelGGL((tslu_l21_kernel<T>), dim3(ceil_div(m - nb, 256), batch), dim3(256), 0, s, n, A, lda, sA, k,
                        nb, dinv);
