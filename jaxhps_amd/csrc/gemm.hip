@@ -105,7 +105,8 @@ hipError_t gemm(const GemmArgs<T>& a, hipStream_t s) {
       if constexpr (sizeof(T) == 16)  // 3 workgroups per CU would force complex128 to spill registers
         hipLaunchKernelGGL((gemm_mfma_kernel<T, 2, 2, 2>), grid, dim3(256), 0, s, a, tiles_n);
       else
-        hipLaunchKernelGGL((gemm_mfma_kernel<T, 2, 2, 3>), grid, dim3(256), 0, s, a, tiles_n);
+        hipLaunchKernelGGL((gemm_mfma_kernel<T, 2, 2, 3, 2, false, is_cplx<T>::value>), grid, dim3(256), 0, s, a,
+                           tiles_n);
     }
   }
   return hipGetLastError();

@@ -47,7 +47,10 @@ template <> __device__ __forceinline__ c64 s_conj<c64>(c64 x) { return c64{x.re,
 
 // TA: op(A) = A^H with A stored K x M; the tile load then walks M fastest so
 // the reads stay coalesced.
-template <typename T, int TM, int TN, int MINB = 1, int WR = 2, bool TA = false>
+// K3M (complex only): three real products per complex one (P1 = Ar Br,
+// P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi); Re = P1 - P2, Im = P3 - P1 - P2), a
+// quarter fewer MFMAs for one more accumulator per block.
+template <typename T, int TM, int TN, int MINB = 1, int WR = 2, bool TA = false, bool K3M = false>
 __global__ __launch_bounds__(256, MINB) void gemm_mfma_kernel(GemmArgs<T> p, int tiles_n) {
   using R = typename real_of<T>::type;
   using M = Mfma<R>;
@@ -74,13 +77,15 @@ __global__ __launch_bounds__(256, MINB) void gemm_mfma_kernel(GemmArgs<T> p, int
   for (int b = blockIdx.y; b < p.batch; b += gridDim.y) {
     const T* A = p.A + (long)b * p.sA;
     const T* B = p.B + (long)b * p.sB;
-    acc_t acc_re[TM][TN], acc_im[TM][TN];
+    static_assert(!K3M || CPLX, "three-product form is for complex types");
+    acc_t acc_re[TM][TN], acc_im[TM][TN], acc_x[K3M ? TM : 1][K3M ? TN : 1];
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
       for (int j = 0; j < TN; ++j) {
         acc_re[i][j] = acc_t{0, 0, 0, 0};
         acc_im[i][j] = acc_t{0, 0, 0, 0};
+        if constexpr (K3M) acc_x[i][j] = acc_t{0, 0, 0, 0};
       }
     T ra[LA], rb[LB];
     auto fetch = [&](int k0) {
@@ -136,19 +141,44 @@ __global__ __launch_bounds__(256, MINB) void gemm_mfma_kernel(GemmArgs<T> p, int
           b_re[t] = Bs_re[kk + lk][wc * (BN / WC) + t * 16 + li];
           if constexpr (CPLX) b_im[t] = Bs_im[kk + lk][wc * (BN / WC) + t * 16 + li];
         }
+        if constexpr (K3M) {
+          R a_s[TM], b_s[TN];
 #pragma unroll
-        for (int i = 0; i < TM; ++i)
+          for (int i = 0; i < TM; ++i) a_s[i] = a_re[i] + a_im[i];
 #pragma unroll
-          for (int j = 0; j < TN; ++j) {
-            acc_re[i][j] = M::mma(a_re[i], b_re[j], acc_re[i][j]);
-            if constexpr (CPLX) {
-              acc_re[i][j] = M::mma(-a_im[i], b_im[j], acc_re[i][j]);
-              acc_im[i][j] = M::mma(a_re[i], b_im[j], acc_im[i][j]);
-              acc_im[i][j] = M::mma(a_im[i], b_re[j], acc_im[i][j]);
+          for (int j = 0; j < TN; ++j) b_s[j] = b_re[j] + b_im[j];
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) {
+              acc_re[i][j] = M::mma(a_re[i], b_re[j], acc_re[i][j]);
+              acc_x[i][j] = M::mma(a_im[i], b_im[j], acc_x[i][j]);
+              acc_im[i][j] = M::mma(a_s[i], b_s[j], acc_im[i][j]);
             }
-          }
+        } else {
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) {
+              acc_re[i][j] = M::mma(a_re[i], b_re[j], acc_re[i][j]);
+              if constexpr (CPLX) {
+                acc_re[i][j] = M::mma(-a_im[i], b_im[j], acc_re[i][j]);
+                acc_im[i][j] = M::mma(a_re[i], b_im[j], acc_im[i][j]);
+                acc_im[i][j] = M::mma(a_im[i], b_re[j], acc_im[i][j]);
+              }
+            }
+        }
       }
       __syncthreads();
+    }
+    if constexpr (K3M) {
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+          acc_im[i][j] = acc_im[i][j] - acc_re[i][j] - acc_x[i][j];
+          acc_re[i][j] = acc_re[i][j] - acc_x[i][j];
+        }
     }
 
     const T* C = p.C ? p.C + (long)b * p.sC : nullptr;
