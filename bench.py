@@ -177,6 +177,26 @@ def _cpu_baseline_3d():
                       f"median of {n} runs, BLAS threads <= {cores}"}
 
 
+def _sine_field():
+    """q = theta . basis with the transposed product written as a weighted row
+    sum (the library GEMV of torch on this image takes ~170 ms for the
+    25 x 1M transposed product)."""
+    import torch
+
+    class SineField(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, theta, basis_flat):
+            ctx.save_for_backward(basis_flat)
+            return theta @ basis_flat
+
+        @staticmethod
+        def backward(ctx, g):
+            (basis_flat,) = ctx.saved_tensors
+            return (basis_flat * g).sum(dim=1), None
+
+    return SineField.apply
+
+
 def adjoint_bench(L=6, warmup=1, reps=2):
     """Inverse-scattering gradients: config E's problem (k = 40, p = 16, 25 sine
     coefficients, 8 seeded thetas); one gradient = build + solve + adjoint of
@@ -193,12 +213,15 @@ def adjoint_bench(L=6, warmup=1, reps=2):
     dom, kw, g, basis, thetas, theta_star = wl.config_E(L=L, p=16, k=k)
     dev = torch.device("cuda", 0)
     basis_t = torch.as_tensor(basis, device=dev)
+    basis_flat = basis_t.reshape(basis_t.shape[0], -1).contiguous()
+    field_shape = basis_t.shape[1:]
+    sine_field = _sine_field()
     g_dev = torch.as_tensor(g, device=dev)
     dkw = {key: (torch.as_tensor(v, device=dev) if isinstance(v, np.ndarray) else v) for key, v in kw.items()}
     ones = torch.ones(dom.interior_points.shape[:2], dtype=torch.float64, device=dev)
     pde = jh.PDEProblem(dom, I_coefficients=ones, **dkw)
     with torch.no_grad():
-        I_star = k**2 * (1.0 - torch.einsum("j,jln->ln", torch.as_tensor(theta_star, device=dev), basis_t))
+        I_star = k**2 * (1.0 - (torch.as_tensor(theta_star, device=dev) @ basis_flat).reshape(field_shape))
         u_obs = differentiable_solve(pde, g_dev, I=I_star).clone()
     thetas_t = torch.as_tensor(thetas, device=dev)
 
@@ -206,7 +229,7 @@ def adjoint_bench(L=6, warmup=1, reps=2):
         grads = []
         for th in thetas_t:
             th = th.clone().requires_grad_(True)
-            I = k**2 * (1.0 - torch.einsum("j,jln->ln", th, basis_t))
+            I = k**2 * (1.0 - sine_field(th, basis_flat).reshape(field_shape))
             u = differentiable_solve(pde, g_dev, I=I)
             J = ((u - u_obs).abs() ** 2).sum()
             J.backward()

@@ -67,7 +67,7 @@ class RefineOps:
         self.M = dev(M)
         self.MT = dev(M.T)
         self.P = dev(pde.P)
-        self.coeffs = torch.stack([to_device(a, device, dtype) for _, a in clist]).contiguous()
+        self.coeffs = None  # set by refine_ops from the current coefficient fields
         # leaf adjacency (2D trees; the refinement residuals need it)
         self.nbr = torch.as_tensor(gauss_neighbors_2D(dom.L, dom.q)).to(device) if dom.bool_2D else None
         self.nb, self.nq = pde.P.shape
@@ -96,10 +96,19 @@ class RefineOps:
 
 
 def refine_ops(pde, device, dtype):
-    key = ("refine", str(device), dtype)
-    if key not in pde._dev_cache:
-        pde._dev_cache[key] = RefineOps(pde, device, dtype)
-    return pde._dev_cache[key]
+    """The operators are kept across PDEProblem.reset() (they depend on the
+    domain and on which coefficient fields are present); the coefficient
+    fields themselves are re-read after a reset."""
+    names = tuple(nm for nm, _ in pde.coefficient_list())
+    skey = ("refine_static", str(device), dtype, names)
+    ops = pde._dev_cache.get(skey)
+    if ops is None:
+        ops = pde._dev_cache[skey] = RefineOps(pde, device, dtype)
+    ckey = ("refine", str(device), dtype)
+    if ckey not in pde._dev_cache:
+        pde._dev_cache[ckey] = torch.stack([to_device(a, device, dtype) for _, a in pde.coefficient_list()]).contiguous()
+    ops.coeffs = pde._dev_cache[ckey]
+    return ops
 
 
 def residual_iti(ops: RefineOps, u, f, g_root, out_dtype):

@@ -39,8 +39,8 @@ def pde_problem_chunk(pde_problem, start: int, end: int, L: int):
     dom.n_leaves = end - start
     dom.interior_points = _slice(pde_problem.domain.interior_points, start, end)
     new.domain = dom
-    # operators only: the refinement cache holds coefficient fields
-    new._dev_cache = {k: v for k, v in pde_problem._dev_cache.items() if k[0] != "refine"}
+    # operators only: the refinement caches hold whole-tree data
+    new._dev_cache = {k: v for k, v in pde_problem._dev_cache.items() if not str(k[0]).startswith("refine")}
     new.reset()
     return new
 
