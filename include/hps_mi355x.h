@@ -245,6 +245,21 @@ int hps_adjoint_up_level(hps_handle_t h, int dtype, int kind, int nm, int nc, in
 int hps_coef_gradient(hps_handle_t h, int dtype, int nl, int n, int row0, int nsrc, int nops, int ncoef,
                       const int* coef_ops, const void* R, int nrow_R, const void* u, const void* lam, double* grad);
 
+/* ---- HPS grid -> points ----------------------------------------------------
+ * out (npts, nsrc) = the interpolant of F (n_leaves, p^dim, nsrc) on leaf
+ * leaf[n] at pts[n] (dim coordinates, host-chosen leaf: the first in leaf
+ * order whose closed box holds the point).  boxes (n_leaves, 2 dim) =
+ * [lo, hi] per coordinate; cheb (p) the Chebyshev nodes on [-1, 1], ascending;
+ * flip_y (2D) runs the leaf's y nodes top down; grid_to_leaf (p^dim) maps the
+ * tensor-grid position (x slowest) to the leaf's point order.  Weights are
+ * the barycentric factors with exact coincidences replaced by EPS.  FP64 /
+ * complex128.  Device pointers.  Replaces interp_from_hps_2D / _3D
+ * (_interpolation_methods.py:24-219) behind Domain.interp_from_interior_points
+ * (_domain.py:209-294). */
+int hps_interp_points(hps_handle_t h, int dtype, int npts, int dim, int p, int nsrc, const double* pts,
+                      const int* leaf, const double* boxes, const double* cheb, int flip_y, const int* grid_to_leaf,
+                      const void* F, void* out);
+
 /* ---- learned leaf-ItI surrogate (no counterpart in the reference) ---------
  * A small MLP maps a leaf's coefficient samples to a rank-r complex
  * correction U V^H of the q = 0 leaf ItI map.  FP32 tensors in memory, BF16

@@ -45,5 +45,23 @@ class Domain:
             self.boundary_points = compute_boundary_Gauss_points_uniform_3D(root, L, q)
             self.n_leaves = 8**L
 
+    def interp_to_interior_points(self, values, sample_points_x, sample_points_y, sample_points_z=None):
+        """Values on meshgrid(x, y[, z], indexing="ij") -> (n_leaves, p^d) on
+        the HPS grid (``_domain.py:99-195``); a torch tensor on the GPU."""
+        from ._interp import interp_to_interior_points
+
+        return interp_to_interior_points(self, values, sample_points_x, sample_points_y, sample_points_z)
+
+    def interp_to_boundary_points(self, sample_points, sample_values, sample_f):
+        """Not implemented in the reference either (``_domain.py:197-204``)."""
+        raise NotImplementedError("interp_to_boundary_points is not implemented yet.")
+
+    def interp_from_interior_points(self, samples, eval_points_x, eval_points_y, eval_points_z=None):
+        """HPS samples (n_leaves, p^d[, nsrc]) -> (vals, target_pts) on a
+        rectangular grid, in the reference's layouts (``_domain.py:206-294``)."""
+        from ._interp import interp_from_interior_points
+
+        return interp_from_interior_points(self, samples, eval_points_x, eval_points_y, eval_points_z)
+
     def __repr__(self) -> str:
         return f"Domain(p={self.p}, q={self.q}, L={self.L}, {'2D' if self.bool_2D else '3D'}, n_leaves={self.n_leaves})"

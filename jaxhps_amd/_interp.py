@@ -1,0 +1,197 @@
+"""HPS grid <-> rectangular grid: ``Domain.interp_to_interior_points`` and
+``Domain.interp_from_interior_points`` (``_domain.py:99-294``,
+``_interpolation_methods.py:24-340``).
+
+To the HPS grid: the reference builds one dense (p^d x n_x n_y [n_z])
+barycentric matrix per leaf.  It is a tensor product, so here every leaf
+column's x factor (p x n_x) is stacked into one matrix and the whole grid of
+leaf values comes from two (three in 3D) GEMMs on the device; each leaf then
+gathers its p^d block in its point order.
+
+From the HPS grid: the evaluation points are assigned on the host to the first
+leaf, in leaf order, whose closed box holds them (the reference's argmax over
+containment), and the HIP kernel ``hps_interp_points`` evaluates each point's
+barycentric interpolant on its leaf.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._grid_creation import _subdivide_2D, _subdivide_3D, rearrange_indices_ext_int_2D, rearrange_indices_ext_int_3D
+from ._lib import HPS_C128, HPS_F64, check, load_library, ptr
+from .quadrature import _bary_factor_eps, affine_transform, chebyshev_points
+
+
+def _device(device):
+    return torch.device(device) if device is not None else torch.device("cuda", 0)
+
+
+def _factors(samples, intervals, p, flip):
+    """Stacked 1D barycentric factors (n_side p, n_samples): block i maps the
+    samples to cell column i's Chebyshev nodes (reversed when flip)."""
+    c = chebyshev_points(p)
+    blocks = []
+    for lo_hi in intervals:
+        nodes = affine_transform(c, np.asarray(lo_hi))
+        if flip:
+            nodes = np.flip(nodes)
+        blocks.append(_bary_factor_eps(samples, nodes))
+    return np.concatenate(blocks, axis=0)
+
+
+def _leaf_boxes(domain):
+    """(n_leaves, 2 d) [lo, hi] per coordinate of the leaves, in leaf order,
+    from the same recursive bisection the tree uses."""
+    if domain.bool_2D:
+        b = _subdivide_2D([domain.root.xmin, domain.root.xmax, domain.root.ymin, domain.root.ymax], domain.L)
+    else:
+        r = domain.root
+        b = _subdivide_3D([r.xmin, r.xmax, r.ymin, r.ymax, r.zmin, r.zmax], domain.L)
+    return np.asarray(b, dtype=np.float64).reshape(-1, 4 if domain.bool_2D else 6)
+
+
+def _leaf_cells(boxes, n_side):
+    """(n_leaves, d) integer cell coordinates of the leaves."""
+    d = boxes.shape[1] // 2
+    lo = boxes[:, 0::2].min(axis=0)
+    hi = boxes[:, 1::2].max(axis=0)
+    centers = 0.5 * (boxes[:, 0::2] + boxes[:, 1::2])
+    return np.clip(np.floor((centers - lo) / ((hi - lo) / n_side)).astype(np.int64), 0, n_side - 1).reshape(-1, d)
+
+
+def _intervals(boxes, cells, axis, n_side):
+    """[lo, hi] of each cell column along axis, taken from the leaves' boxes."""
+    out = np.empty((n_side, 2))
+    out[cells[:, axis]] = boxes[:, 2 * axis:2 * axis + 2]
+    return out
+
+
+def interp_to_interior_points(domain, values, sample_points_x, sample_points_y, sample_points_z=None, device=None):
+    """Values on meshgrid(x, y[, z], indexing="ij") -> (n_leaves, p^d) on the device."""
+    dev = _device(device)
+    x = np.asarray(sample_points_x, dtype=np.float64)
+    y = np.asarray(sample_points_y, dtype=np.float64)
+    v = torch.as_tensor(values)
+    dt = torch.complex128 if v.is_complex() else torch.float64
+    v = v.to(device=dev, dtype=dt).contiguous()
+    p, L = domain.p, domain.L
+    n_side = 2**L
+    boxes = _leaf_boxes(domain)
+    cells = _leaf_cells(boxes, n_side)
+    if domain.bool_2D:
+        if sample_points_z is not None:
+            raise ValueError("2D domain: sample_points_z must be None")
+        if tuple(v.shape) != (x.size, y.size):
+            raise ValueError(f"values must have shape {(x.size, y.size)}")
+        Fx = torch.as_tensor(_factors(x, _intervals(boxes, cells, 0, n_side), p, False)).to(dev, dt).contiguous()
+        Fy = torch.as_tensor(_factors(y, _intervals(boxes, cells, 1, n_side), p, True)).to(dev, dt)
+        T1 = _lib.gemm(Fx, v)[0]  # (n_side p, n_y)
+        G = _lib.gemm(T1, Fy.t().contiguous())[0]  # (n_side p, n_side p)
+        r = rearrange_indices_ext_int_2D(p)
+        a, b = np.divmod(r, p)
+        idx = (cells[:, :1] * p + a[None, :]) * (n_side * p) + (cells[:, 1:2] * p + b[None, :])
+        return G.reshape(-1)[torch.as_tensor(idx, device=dev)]
+    z = np.asarray(sample_points_z, dtype=np.float64)
+    if tuple(v.shape) != (x.size, y.size, z.size):
+        raise ValueError(f"values must have shape {(x.size, y.size, z.size)}")
+    Fx = torch.as_tensor(_factors(x, _intervals(boxes, cells, 0, n_side), p, False)).to(dev, dt).contiguous()
+    Fy = torch.as_tensor(_factors(y, _intervals(boxes, cells, 1, n_side), p, False)).to(dev, dt).contiguous()
+    Fz = torch.as_tensor(_factors(z, _intervals(boxes, cells, 2, n_side), p, False)).to(dev, dt)
+    P = n_side * p
+    T1 = _lib.gemm(Fx, v.reshape(x.size, -1))[0].reshape(P, y.size, z.size)  # x contracted
+    T2 = _lib.gemm(Fy, T1.permute(1, 0, 2).reshape(y.size, -1).contiguous())[0]  # (P_y, P_x n_z)
+    T2 = T2.reshape(P, P, z.size).permute(1, 0, 2).reshape(P * P, z.size).contiguous()  # (P_x P_y, n_z)
+    G = _lib.gemm(T2, Fz.t().contiguous())[0]  # (P_x P_y, P_z)
+    r = rearrange_indices_ext_int_3D(p)
+    a, rem = np.divmod(r, p * p)
+    b, c = np.divmod(rem, p)
+    idx = ((cells[:, :1] * p + a[None, :]) * P + (cells[:, 1:2] * p + b[None, :])) * P + (cells[:, 2:3] * p + c[None, :])
+    return G.reshape(-1)[torch.as_tensor(idx, device=dev)]
+
+
+def _first_leaf(boxes, n_side, pts):
+    """Leaf index (leaf order) of the first leaf whose closed box holds each
+    point (pts (npts, d)); points outside every box take leaf 0, the
+    reference's argmax over an all-false row."""
+    d = pts.shape[1]
+    cells = _leaf_cells(boxes, n_side)
+    order = np.full((n_side,) * d, -1, dtype=np.int64)
+    order[tuple(cells[:, a] for a in range(d))] = np.arange(cells.shape[0])
+    lo = boxes[:, 0::2].min(axis=0)
+    hi = boxes[:, 1::2].max(axis=0)
+    base = np.floor((pts - lo) / ((hi - lo) / n_side)).astype(np.int64)
+    none = np.iinfo(np.int64).max
+    best = np.full(pts.shape[0], none, dtype=np.int64)
+    for shift in np.ndindex(*(2,) * d):
+        ii = base - np.asarray(shift)[None, :]
+        ok = np.all((ii >= 0) & (ii < n_side), axis=1)
+        leaf = order[tuple(np.clip(ii[:, a], 0, n_side - 1) for a in range(d))]
+        bx = boxes[leaf]
+        ok &= np.all((pts >= bx[:, 0::2]) & (pts <= bx[:, 1::2]), axis=1)
+        best = np.minimum(best, np.where(ok, leaf, none))
+    best[best == none] = 0
+    return best
+
+
+def interp_from_interior_points(domain, samples, eval_points_x, eval_points_y, eval_points_z=None, device=None):
+    """(vals, target_pts) as the reference returns them: vals (n_x, n_y[, nsrc])
+    (3D: (n_x, n_y, n_z)) from the points of meshgrid(x, y[, z]) in its default
+    ("xy") order, and target_pts ((n_y, n_x, 2) or (n_y, n_x, n_z, 3))."""
+    dev = _device(device)
+    s = torch.as_tensor(samples)
+    dt = torch.complex128 if s.is_complex() else torch.float64
+    s = s.to(device=dev, dtype=dt)
+    nl, npl = domain.interior_points.shape[:2]
+    if tuple(s.shape[:2]) != (nl, npl):
+        raise ValueError(f"samples must have leading shape {(nl, npl)}")
+    multi = s.dim() == 3
+    s3 = (s if multi else s.unsqueeze(-1)).contiguous()
+    nsrc = s3.shape[-1]
+    x = np.asarray(eval_points_x, dtype=np.float64)
+    y = np.asarray(eval_points_y, dtype=np.float64)
+    p = domain.p
+    boxes = _leaf_boxes(domain)
+    if domain.bool_2D:
+        if eval_points_z is not None:
+            raise ValueError("2D domain: eval_points_z must be None")
+        X, Y = np.meshgrid(x, y)
+        target = np.stack([X, Y], axis=2)
+        flat = target.reshape(-1, 2)
+        leaf = _first_leaf(boxes, 2**domain.L, flat)
+        r = rearrange_indices_ext_int_2D(p)
+        out_shape = (x.size, y.size)
+    else:
+        z = np.asarray(eval_points_z, dtype=np.float64)
+        X, Y, Z = np.meshgrid(x, y, z)
+        target = np.concatenate([X[..., None], Y[..., None], Z[..., None]], axis=3)
+        flat = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=-1)
+        leaf = _first_leaf(boxes, 2**domain.L, flat)
+        r = rearrange_indices_ext_int_3D(p)
+        out_shape = (x.size, y.size, z.size)
+    dim = flat.shape[1]
+    grid_to_leaf = np.empty(r.size, dtype=np.int32)
+    grid_to_leaf[r] = np.arange(r.size, dtype=np.int32)
+    npts = flat.shape[0]
+
+    def d(a, t):
+        return torch.as_tensor(np.ascontiguousarray(a)).to(device=dev, dtype=t)
+
+    pts_d = d(flat, torch.float64)
+    leaf_d = d(leaf, torch.int32)
+    boxes_d = d(boxes, torch.float64)
+    cheb_d = d(chebyshev_points(p), torch.float64)
+    g2l_d = d(grid_to_leaf, torch.int32)
+    out = torch.empty((npts, nsrc), dtype=dt, device=dev)
+    lib = load_library()
+    hd = _lib.handle()
+    rc = lib.hps_interp_points(hd, HPS_C128 if dt == torch.complex128 else HPS_F64, npts, dim, p, nsrc, ptr(pts_d),
+                               ptr(leaf_d), ptr(boxes_d), ptr(cheb_d), 1, ptr(g2l_d), ptr(s3), ptr(out))
+    check(rc, hd, "hps_interp_points")
+    vals = out.reshape(out_shape + (nsrc,)) if multi else out[:, 0].reshape(out_shape)
+    return vals, target
+
+
+__all__ = ["interp_to_interior_points", "interp_from_interior_points"]

@@ -71,6 +71,7 @@ SIGNATURES = {
     "hps_adjoint_up_workspace": (_S, [_I, _I, _I, _I, _I]),
     "hps_adjoint_up_level": (_I, [_P] + [_I] * 8 + [_P] * 12 + [_S]),
     "hps_coef_gradient": (_I, [_P] + [_I] * 7 + [_P, _P, _I, _P, _P, _P]),
+    "hps_interp_points": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _I, _P, _P, _P]),
     "hps_mlp_gemm_bf16": (_I, [_P, _I, _I, _I, _I, _I, _P, _I, _P, _I, ctypes.c_float, _P, _I, _P, _I]),
     "hps_mlp_tanh_backward": (_I, [_P, _L, _P, _P]),
     "hps_mlp_bias_grad": (_I, [_P, _I, _I, _P, _P]),

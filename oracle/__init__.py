@@ -14,6 +14,7 @@ hps2d          stage-by-stage restatement of the reference's 2D DtN / ItI
                leaf solve, merges, down pass and ItI up pass
 hps3d          3D DtN leaf/merge/down pass restatement
 global_system  the global HPS discrete system solved densely (cross-check)
+interp         HPS grid <-> rectangular grid by explicit Lagrange products
 
 Pinning (what checks each function beyond re-typing it):
 * spectral: textbook D-matrices n=2,3,4 and Chebyshev/Gauss points from the
@@ -26,6 +27,9 @@ Pinning (what checks each function beyond re-typing it):
 * hps3d: leaf operators exact on polynomials (derivatives, P, Q); a
   manufactured cubic solution reproduced to rounding through the oct merges
   and down pass; the 3D global system (coordinates only) on tiny inputs.
+* interp: exactness on polynomials of degree below the node counts (2D and
+  3D, both directions); the first-leaf rule on shared edges by a leaf-wise
+  constant field.
 """
 
 from . import spectral, hps2d, hps3d, global_system  # noqa: F401
