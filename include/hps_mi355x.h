@@ -260,6 +260,42 @@ int hps_interp_points(hps_handle_t h, int dtype, int npts, int dim, int p, int n
                       const int* leaf, const double* boxes, const double* cheb, int flip_y, const int* grid_to_leaf,
                       const void* F, void* out);
 
+/* ---- exterior-scattering coupling of the top-level operator ---------------
+ * Dense complex128 work on the n root-boundary Gauss points (S, E, N, W sides
+ * of n/4 points each).  Device pointers; info is one device int (0, or j+1
+ * for the first exactly singular pivot column of the dense LU).
+ *
+ * u_in[i, s] = exp(i k <pts_i, (cos dirs_s, sin dirs_s)>), (npts, nsrc); with
+ * uin_dn, the outward normal derivative on boundary points in sides of
+ * n_per_side points (the last side takes the remainder).  Replaces get_uin
+ * and get_uin_and_normals (examples/wave_scattering_utils.py:139-197). */
+int hps_plane_wave(hps_handle_t h, long npts, int nsrc, const double* pts, const double* dirs, double k,
+                   long n_per_side, void* uin, void* uin_dn);
+
+/* T = -i eta (R - I)^{-1} (R + I) for the n x n top-level ItI map R (T may
+ * alias R).  Replaces get_DtN_from_ItI (examples/wave_scattering_utils.py:
+ * 31-49). */
+size_t hps_iti_to_dtn_workspace(int n);
+int hps_iti_to_dtn(hps_handle_t h, int dtype, int n, const void* R, long ldr, double eta, void* T, long ldt,
+                   int* info, void* ws, size_t ws_bytes);
+
+/* A = I/2 - D + S T (n x n) and b = S (du_in/dn - T u_in) (n x nsrc) for the
+ * boundary-integral equation of the exterior problem.  Replaces
+ * setup_scattering_lin_system (examples/wave_scattering_utils.py:96-136). */
+size_t hps_scattering_workspace(int n, int nsrc);
+int hps_scattering_system(hps_handle_t h, int dtype, int n, int nsrc, const void* S, const void* D, const void* T,
+                          const double* bdry_pts, const double* source_dirs, double k, void* A, void* b, void* ws,
+                          size_t ws_bytes);
+
+/* imp (n x nsrc) = du_scat/dn + i eta u_scat with u_scat = A^{-1} b and
+ * du_scat/dn = T (u_scat + u_in) - du_in/dn: the incoming impedance data of
+ * the scattered field for the down pass.  Workspace as
+ * hps_scattering_workspace.  Replaces get_scattering_uscat_impedance
+ * (examples/wave_scattering_utils.py:200-244). */
+int hps_scattering_impedance(hps_handle_t h, int dtype, int n, int nsrc, const void* S, const void* D, const void* T,
+                             const double* bdry_pts, const double* source_dirs, double k, double eta, void* imp,
+                             int* info, void* ws, size_t ws_bytes);
+
 /* ---- learned leaf-ItI surrogate (no counterpart in the reference) ---------
  * A small MLP maps a leaf's coefficient samples to a rank-r complex
  * correction U V^H of the q = 0 leaf ItI map.  FP32 tensors in memory, BF16

@@ -72,6 +72,12 @@ SIGNATURES = {
     "hps_adjoint_up_level": (_I, [_P] + [_I] * 8 + [_P] * 12 + [_S]),
     "hps_coef_gradient": (_I, [_P] + [_I] * 7 + [_P, _P, _I, _P, _P, _P]),
     "hps_interp_points": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _I, _P, _P, _P]),
+    "hps_plane_wave": (_I, [_P, _L, _I, _P, _P, _D, _L, _P, _P]),
+    "hps_iti_to_dtn_workspace": (_S, [_I]),
+    "hps_iti_to_dtn": (_I, [_P, _I, _I, _P, _L, _D, _P, _L, _P, _P, _S]),
+    "hps_scattering_workspace": (_S, [_I, _I]),
+    "hps_scattering_system": (_I, [_P, _I, _I, _I, _P, _P, _P, _P, _P, _D, _P, _P, _P, _S]),
+    "hps_scattering_impedance": (_I, [_P, _I, _I, _I, _P, _P, _P, _P, _P, _D, _D, _P, _P, _P, _S]),
     "hps_mlp_gemm_bf16": (_I, [_P, _I, _I, _I, _I, _I, _P, _I, _P, _I, ctypes.c_float, _P, _I, _P, _I]),
     "hps_mlp_tanh_backward": (_I, [_P, _L, _P, _P]),
     "hps_mlp_bias_grad": (_I, [_P, _I, _I, _P, _P]),

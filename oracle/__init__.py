@@ -15,6 +15,7 @@ hps2d          stage-by-stage restatement of the reference's 2D DtN / ItI
 hps3d          3D DtN leaf/merge/down pass restatement
 global_system  the global HPS discrete system solved densely (cross-check)
 interp         HPS grid <-> rectangular grid by explicit Lagrange products
+scattering     ItI -> DtN, plane waves, exterior boundary-integral coupling
 
 Pinning (what checks each function beyond re-typing it):
 * spectral: textbook D-matrices n=2,3,4 and Chebyshev/Gauss points from the
@@ -30,6 +31,9 @@ Pinning (what checks each function beyond re-typing it):
 * interp: exactness on polynomials of degree below the node counts (2D and
   3D, both directions); the first-leaf rule on shared edges by a leaf-wise
   constant field.
+* scattering: the Cayley map of a known DtN inverted; normal derivatives
+  against finite differences; a q = 0 Helmholtz DtN reproducing plane-wave
+  normal derivatives; the D = I/2 closed form; no scattered field at q = 0.
 """
 
 from . import spectral, hps2d, hps3d, global_system  # noqa: F401
