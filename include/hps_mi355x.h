@@ -260,6 +260,20 @@ int hps_interp_points(hps_handle_t h, int dtype, int npts, int dim, int p, int n
                       const int* leaf, const double* boxes, const double* cheb, int flip_y, const int* grid_to_leaf,
                       const void* F, void* out);
 
+/* ---- adaptive (non-uniform) trees ------------------------------------------
+ * X[b*stride + i] *= scale[b] (i < n), scale a device double array: the
+ * per-leaf side-length scaling of coefficient fields and of DtN maps and
+ * fluxes that replaces rebuilding the operators for every leaf size
+ * (local_solve/_adaptive_2D_DtN.py:130-172). */
+int hps_scale_batched(hps_handle_t h, int dtype, int batch, long n, long stride, const double* scale, void* X);
+
+/* dst[b][r][c] = alpha * src[b][r][c] (src == NULL writes zeros): packs
+ * children of different sizes into the padded blocks a merge level reads and
+ * unpacks the down pass's results; replaces the reference's per-node
+ * slicing and concatenation (merge/_adaptive_2D_DtN.py:110-206). */
+int hps_copy2d_batched(hps_handle_t h, int dtype, int rows, int cols, double alpha, const void* src, long lds,
+                       long ss, void* dst, long ldd, long sd, int batch);
+
 /* ---- exterior-scattering coupling of the top-level operator ---------------
  * Dense complex128 work on the n root-boundary Gauss points (S, E, N, W sides
  * of n/4 points each).  Device pointers; info is one device int (0, or j+1

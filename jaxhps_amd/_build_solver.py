@@ -8,6 +8,8 @@ iterative refinement and the adjoint without refactorising.
 
 from __future__ import annotations
 
+from ._discretization_tree import get_all_leaves
+
 from ._lib import HPS_KIND_DTN, HPS_KIND_ITI, raise_on_info
 from ._stages import cast
 from .local_solve import _leaf
@@ -80,7 +82,9 @@ def build_solver(pde_problem, return_top_T: bool = False, compute_device=None, h
 
     dom = pde_problem.domain
     if not dom.bool_uniform:
-        raise NotImplementedError("adaptive trees are outside this package's scope")
+        if precision != "fp64":
+            raise NotImplementedError("adaptive trees are built in FP64")
+        return _adaptive_build_solver(pde_problem, return_top_T, compute_device)
     leaf_dtype, store_dtype, hi, lo = _compute_dtypes(pde_problem, precision, low_precision_leaves)
     if precision == "mixed" and not dom.bool_2D:
         raise NotImplementedError("the mixed-precision build covers 2D uniform trees")
@@ -108,6 +112,22 @@ def build_solver(pde_problem, return_top_T: bool = False, compute_device=None, h
     pde_problem._factors = {"leaf": res, "levels": levels}
     if return_top_T:
         return levels[-1].T[0]
+    return None
+
+
+def _adaptive_build_solver(pde_problem, return_top_T, compute_device):
+    """Adaptive 2D DtN build (``_build_solver.py:177-208``): leaf maps, then
+    the merges, whose outputs live in the tree's nodes."""
+    from ._adaptive import local_solve_stage_adaptive_2D_DtN, merge_stage_adaptive_2D_DtN
+
+    Y, T, v, h = local_solve_stage_adaptive_2D_DtN(pde_problem, device=compute_device)
+    pde_problem.Y, pde_problem.v = Y, v
+    for i, leaf in enumerate(get_all_leaves(pde_problem.domain.root)):
+        leaf.data.Y, leaf.data.v = Y[i], v[i]
+    merge_stage_adaptive_2D_DtN(pde_problem, T, h)
+    pde_problem._precision = "fp64"
+    if return_top_T:
+        return pde_problem.domain.root.data.T
     return None
 
 

@@ -1,10 +1,10 @@
-"""``Domain``: uniform quadtree/octree discretization (``_domain.py:38-97``)."""
+"""``Domain``: uniform quadtree/octree or adaptive quadtree discretization (``_domain.py:38-97``)."""
 
 from __future__ import annotations
 
 import numpy as np
 
-from ._discretization_tree import DiscretizationNode2D, DiscretizationNode3D
+from ._discretization_tree import DiscretizationNode2D, DiscretizationNode3D, get_all_leaves
 from ._grid_creation import (
     compute_boundary_Gauss_points_uniform_2D,
     compute_boundary_Gauss_points_uniform_3D,
@@ -14,13 +14,15 @@ from ._grid_creation import (
 
 
 class Domain:
-    """Uniform HPS discretization of a box.
+    """HPS discretization of a box.
 
     Args mirror the reference: p Chebyshev points per leaf side, q Gauss
     points per leaf side, the root node, and L levels of uniform refinement.
     Point arrays are float64 numpy arrays on the host:
     ``interior_points`` (n_leaves, p^d, d) and ``boundary_points``
-    (n_bdry, d).  Adaptive trees (L=None) are not supported by this package.
+    (n_bdry, d).  With L=None the tree is adaptive (2D only): the root's
+    existing leaves are the leaves, and the side counts n_0..n_3 of every
+    node are set.
     """
 
     def __init__(self, p: int, q: int, root, L: int | None = None):
@@ -32,9 +34,21 @@ class Domain:
         if not (self.bool_2D or isinstance(root, DiscretizationNode3D)):
             raise TypeError("root must be a DiscretizationNode2D or DiscretizationNode3D")
         if L is None:
-            raise NotImplementedError(
-                "adaptive discretizations are outside this package's scope; pass L for a uniform tree"
+            # adaptive tree: the leaves are the given tree's leaves
+            from ._adaptive import (
+                assign_side_counts,
+                compute_boundary_Gauss_points_adaptive_2D,
+                compute_interior_Chebyshev_points_adaptive_2D,
             )
+
+            if not self.bool_2D:
+                raise NotImplementedError("adaptive 3D trees are not supported by this package")
+            self.bool_uniform = False
+            assign_side_counts(root, q)
+            self.n_leaves = len(get_all_leaves(root))
+            self.interior_points = compute_interior_Chebyshev_points_adaptive_2D(root, p)
+            self.boundary_points = compute_boundary_Gauss_points_adaptive_2D(root, q)
+            return
         self.bool_uniform = True
         if self.bool_2D:
             self.interior_points = compute_interior_Chebyshev_points_uniform_2D(root, L, p)
@@ -62,6 +76,34 @@ class Domain:
         from ._interp import interp_from_interior_points
 
         return interp_from_interior_points(self, samples, eval_points_x, eval_points_y, eval_points_z)
+
+    def get_adaptive_boundary_data_lst(self, f):
+        """f evaluated on the root boundary points of each side S, E, N, W
+        (``_domain.py:294-357``): a list of four arrays."""
+        bp = self.boundary_points
+        if not self.bool_2D:
+            raise NotImplementedError("adaptive 3D trees are not supported by this package")
+        r = self.root
+        return [
+            f(bp[bp[:, 1] == r.ymin]),
+            f(bp[bp[:, 0] == r.xmax]),
+            f(bp[bp[:, 1] == r.ymax]),
+            f(bp[bp[:, 0] == r.xmin]),
+        ]
+
+    @classmethod
+    def from_adaptive_discretization(cls, p: int, q: int, root, f, tol: float, use_level_restriction: bool = True,
+                                     use_l_2_norm: bool = False) -> "Domain":
+        """Refine root until each function in f is resolved to tol, then build
+        the adaptive Domain (``_domain.py:359-434``)."""
+        from ._adaptive import generate_adaptive_mesh_level_restriction_2D
+
+        if not isinstance(root, DiscretizationNode2D):
+            raise NotImplementedError("adaptive 3D trees are not supported by this package")
+        for fn in (f if isinstance(f, list) else [f]):
+            generate_adaptive_mesh_level_restriction_2D(root=root, f_fn=fn, tol=tol, p=p, q=q,
+                                                        restrict_bool=use_level_restriction, l2_norm=use_l_2_norm)
+        return cls(p=p, q=q, root=root)
 
     def __repr__(self) -> str:
         return f"Domain(p={self.p}, q={self.q}, L={self.L}, {'2D' if self.bool_2D else '3D'}, n_leaves={self.n_leaves})"

@@ -14,7 +14,7 @@ from typing import List
 import numpy as np
 import torch
 
-from ._discretization_tree import DiscretizationNode3D
+from ._discretization_tree import DiscretizationNode3D, get_all_leaves
 from ._domain import Domain
 from ._precompute_operators import (
     precompute_diff_operators_2D,
@@ -111,7 +111,14 @@ class PDEProblem:
         self.use_ItI = use_ItI
         self.eta = eta
 
-        half_side_len = (domain.root.xmax - domain.root.xmin) / (2 ** (domain.L + 1))
+        if domain.bool_uniform:
+            half_side_len = (domain.root.xmax - domain.root.xmin) / (2 ** (domain.L + 1))
+        else:
+            if not bool_2D:
+                raise NotImplementedError("adaptive 3D trees are not supported by this package")
+            # operators for half side 1; each leaf's size is applied to its data
+            half_side_len = 1.0
+            self.sidelens = np.array([lf.xmax - lf.xmin for lf in get_all_leaves(domain.root)])
         self.half_side_len = half_side_len
         p, q = domain.p, domain.q
         if bool_2D:
@@ -119,6 +126,10 @@ class PDEProblem:
             if not use_ItI:
                 self.P = precompute_P_2D_DtN(p, q)
                 self.Q = precompute_Q_2D_DtN(p, q, self.D_x, self.D_y)
+                if not domain.bool_uniform:
+                    from ._adaptive import precompute_projection_ops_2D
+
+                    self.L_2f1, self.L_1f2 = precompute_projection_ops_2D(q)
             else:
                 self.P = precompute_P_2D_ItI(p, q)
                 N_tilde = precompute_N_tilde_matrix_2D(self.D_x, self.D_y, p)

@@ -54,7 +54,9 @@ def solve(pde_problem, boundary_data, source=None, compute_device=None, host_dev
             )
         return _up_then_down_pass(pde_problem, boundary_data, source)
     if not dom.bool_uniform:
-        raise NotImplementedError("adaptive trees are outside this package's scope")
+        from ._adaptive import down_pass_adaptive_2D_DtN
+
+        return down_pass_adaptive_2D_DtN(pde_problem, boundary_data)
     if not dom.bool_2D:
         fn = down_pass_uniform_3D_DtN
     else:

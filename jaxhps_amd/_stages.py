@@ -69,11 +69,25 @@ class LeafResult:
     kind: int
 
 
-def leaf_solve(pde, kind: int, dtype, device, source=None, nsrc_zero=False) -> LeafResult:
+def scale_batched(X, scale):
+    """X[b] *= scale[b] in place (X contiguous, scale a float64 device vector)."""
+    lib = load_library()
+    h = _lib.handle()
+    b = X.shape[0]
+    n = X[0].numel()
+    rc = lib.hps_scale_batched(h, dtype_code(X.dtype), b, n, n, ptr(scale), ptr(X))
+    check(rc, h, "hps_scale_batched")
+
+
+def leaf_solve(pde, kind: int, dtype, device, source=None, nsrc_zero=False, leaf_half_sides=None) -> LeafResult:
     """Run hps_leaf_solve over all leaves of `pde`.
 
     source: (n_leaves, n, nsrc) tensor in `dtype`, or None for a zero source
-    (nsrc_zero=True gives the no-source leaf stage, nsrc = 0)."""
+    (nsrc_zero=True gives the no-source leaf stage, nsrc = 0).
+    leaf_half_sides (n_leaves,) float64 numpy array: leaves of different
+    sizes with operators built for half side 1 -- each coefficient of a
+    k-th order term is scaled by half^-k and T, h by 1/half (the operators
+    are homogeneous in the half side)."""
     lib = load_library()
     h = _lib.handle()
     dom = pde.domain
@@ -88,6 +102,14 @@ def leaf_solve(pde, kind: int, dtype, device, source=None, nsrc_zero=False) -> L
         raise ValueError("PDEProblem has no coefficients")
     coef_ops = np.array([-1 if nm == "I" else names.index(nm) for nm, _ in clist], dtype=np.int32)
     coeffs = torch.stack([to_device(a, device, dtype) for _, a in clist]).contiguous()
+    if leaf_half_sides is not None:
+        hs = np.asarray(leaf_half_sides, dtype=np.float64)
+        inv = torch.as_tensor(1.0 / hs).to(device)
+        inv2 = torch.as_tensor(1.0 / (hs * hs)).to(device)
+        for k, (nm, _) in enumerate(clist):
+            order = 0 if nm == "I" else len(nm) - 2
+            if order:
+                scale_batched(coeffs[k], inv if order == 1 else inv2)
     if nsrc_zero:
         nsrc = 0
         src = None
@@ -115,6 +137,10 @@ def leaf_solve(pde, kind: int, dtype, device, source=None, nsrc_zero=False) -> L
         ptr(Y), ptr(T), ptr(v), ptr(hv), ptr(lu), ptr(perm), ptr(dinv), ptr(info), ptr(ws), ws.numel(),
     )
     check(rc, h, "hps_leaf_solve")
+    if leaf_half_sides is not None:
+        scale_batched(T, inv)
+        if hv is not None:
+            scale_batched(hv, inv)
     return LeafResult(Y, T, v, hv, lu, perm, dinv, info, kind)
 
 

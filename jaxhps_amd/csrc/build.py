@@ -17,7 +17,7 @@ PKG = HERE.parent
 REPO = PKG.parent
 LIB = PKG / "libhps_mi355x.so"
 BUILD = REPO / "_build"
-SOURCES = ["gemm.hip", "lu.hip", "hps_capi.hip", "refine.hip", "mlp.hip", "interp.hip", "scatter.hip", "hps_prof.hip"]
+SOURCES = ["gemm.hip", "lu.hip", "hps_capi.hip", "refine.hip", "mlp.hip", "interp.hip", "scatter.hip", "adaptive.hip", "hps_prof.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

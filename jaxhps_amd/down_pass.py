@@ -92,3 +92,10 @@ def down_pass_uniform_3D_DtN(boundary_data, S_lst, g_tilde_lst, Y_arr, v_arr, de
         if g_tilde_lst:
             g_tilde_lst[-1] = g_tilde_lst[-1].unsqueeze(0)
     return _down(boundary_data, S_lst, g_tilde_lst, Y_arr, v_arr, table_fn=lambda ne: oct_merge_tables(ne // 24))
+
+
+def down_pass_adaptive_2D_DtN(pde_problem, boundary_data):
+    """Adaptive 2D down pass (``down_pass/_adaptive_2D_DtN.py:16-98``)."""
+    from ._adaptive import down_pass_adaptive_2D_DtN as fn
+
+    return fn(pde_problem, boundary_data)

@@ -75,3 +75,10 @@ def local_solve_stage_uniform_2D_ItI(pde_problem, device=None, host_device=None)
     Returns (Y, T, v, h) in the reference's order; complex128."""
     res, v, h = _leaf(pde_problem, HPS_KIND_ITI, device)
     return res.Y, res.T, v, h
+
+
+def local_solve_stage_adaptive_2D_DtN(pde_problem, device=None, host_device=None):
+    """Adaptive 2D leaf stage (``local_solve/_adaptive_2D_DtN.py:13-117``)."""
+    from ._adaptive import local_solve_stage_adaptive_2D_DtN as fn
+
+    return fn(pde_problem, device=device, host_device=host_device)

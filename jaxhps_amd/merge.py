@@ -121,3 +121,11 @@ def merge_stage_uniform_3D_DtN(T_arr, h_arr, l: int, device=None, host_device=No
     if return_T:
         return S_lst, g_lst, out[2]
     return S_lst, g_lst
+
+
+def merge_stage_adaptive_2D_DtN(pde_problem) -> None:
+    """Adaptive 2D merges (``merge/_adaptive_2D_DtN.py:25-110``); expects
+    each leaf's data.T and data.h to be set."""
+    from ._adaptive import merge_stage_adaptive_2D_DtN as fn
+
+    fn(pde_problem)

@@ -16,6 +16,7 @@ hps3d          3D DtN leaf/merge/down pass restatement
 global_system  the global HPS discrete system solved densely (cross-check)
 interp         HPS grid <-> rectangular grid by explicit Lagrange products
 scattering     ItI -> DtN, plane waves, exterior boundary-integral coupling
+adaptive2d     adaptive 2D DtN: refinement, compressed merges, down pass
 
 Pinning (what checks each function beyond re-typing it):
 * spectral: textbook D-matrices n=2,3,4 and Chebyshev/Gauss points from the
@@ -34,6 +35,9 @@ Pinning (what checks each function beyond re-typing it):
 * scattering: the Cayley map of a known DtN inverted; normal derivatives
   against finite differences; a q = 0 Helmholtz DtN reproducing plane-wave
   normal derivatives; the D = I/2 closed form; no scattered field at q = 0.
+* adaptive2d: identity with hps2d on uniform trees; exact polynomial
+  solutions on non-uniform trees; the reference's panel-norm closed forms;
+  level restriction of refined trees.
 """
 
 from . import spectral, hps2d, hps3d, global_system  # noqa: F401
