@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--refine-tol", type=float, default=1e-11)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint grads/s measurement")
     ap.add_argument("--adjoint-L", type=int, default=6, help="tree depth of the adjoint workload (testing only)")
+    ap.add_argument("--workload", default="hps", choices=["hps", "scattering", "interp", "adaptive"],
+                    help="hps: the headline build+solve; the others time the exterior coupling, the grid "
+                         "interpolation and the adaptive 2D build+solve")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (hps_convert, complex128 -> complex128, "
                          "256 MiB, 10 times): the reference for scaling HBM byte counters")
@@ -251,6 +254,185 @@ def adjoint_bench(L=6, warmup=1, reps=2):
             "dtype": "c128", "grad_norm": float(grads.norm())}
 
 
+def _line(metric, unit, value, ms, steps, warmup, dtype, config, roofline, kernels, cpu):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+            "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": dtype,
+            "data": "synthetic", "config": config, "roofline": roofline, "kernels": kernels, "cpu_baseline": cpu}
+
+
+def _timed_steps(step, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps * 1e3
+
+
+def bench_scattering(args, n=7168, nsrc=16):
+    """Exterior coupling of a top-level operator: ItI -> DtN and the
+    boundary-integral solve for nsrc incident waves (n = 7168 is the root
+    boundary of the 2D p = 16, L = 7 tree).  Step = get_DtN_from_ItI +
+    get_scattering_uscat_impedance on resident inputs."""
+    import numpy as np
+    import torch
+
+    from jaxhps_amd import profiling
+    from jaxhps_amd.scattering import get_DtN_from_ItI, get_scattering_uscat_impedance
+
+    dev = torch.device("cuda", 0)
+    g = torch.Generator(device=dev).manual_seed(0)
+
+    def crand(scale):
+        return torch.complex(torch.randn(n, n, generator=g, device=dev, dtype=torch.float64),
+                             torch.randn(n, n, generator=g, device=dev, dtype=torch.float64)) * scale
+
+    R, S, D = crand(0.5 / np.sqrt(n)), crand(1.0 / n), crand(1.0 / n)
+    bdry = torch.as_tensor(np.random.default_rng(1).uniform(-1, 1, (n, 2))).to(dev)
+    dirs = torch.linspace(0, 2 * np.pi, nsrc + 1, dtype=torch.float64, device=dev)[:-1]
+    k = 100.0
+
+    def step():
+        T = get_DtN_from_ItI(R, k)
+        return get_scattering_uscat_impedance(S, D, T, dirs, bdry, k, k)
+
+    ms = _timed_steps(step, args.steps, args.warmup)
+    _, kernels, roof = profiling.instrumented(step)
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import scattering as osc
+
+        m = 1792
+        rng = np.random.default_rng(2)
+        Rh = (rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m))) * 0.5 / np.sqrt(m)
+        Sh = (rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m))) / m
+        Dh = (rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m))) / m
+        bh = rng.uniform(-1, 1, (m, 2))
+        dh = np.linspace(0, 2 * np.pi, nsrc + 1)[:-1]
+        t, reps = _time_bounded(lambda: osc.uscat_impedance(Sh, Dh, osc.dtn_from_iti(Rh, k), dh, bh, k, k), 20.0)
+        cores = len(os.sched_getaffinity(0))
+        cpu = {"value": (1.0 / t) * (m / n) ** 3, "unit": "couplings/s", "cores": cores, "kind": "port",
+               "sample": f"numpy oracle at n = {m} (median of {reps}), scaled by (n/{m})^3 to n = {n}"}
+    return _line(f"exterior coupling (ItI->DtN + BIE solve) per second at n_bdry={n}", "couplings/s",
+                 1e3 / ms, ms, args.steps, args.warmup, "c128",
+                 {"workload": "exterior scattering coupling", "n_bdry": n, "nsrc": nsrc}, roof, kernels, cpu)
+
+
+def bench_interp(args, ngrid=1024, nsamp=64):
+    """Both interpolation directions on the 2D p = 16, L = 7 HPS grid (16384
+    leaves, 4.2M points): from nsamp x nsamp samples at Chebyshev abscissae
+    (one global interpolant per axis, as the reference builds it) to the HPS
+    grid, and from the HPS grid to an ngrid x ngrid uniform grid.  Step =
+    interp_to_interior_points + interp_from_interior_points."""
+    import numpy as np
+    import torch
+
+    import jaxhps_amd as jh
+    from jaxhps_amd import profiling
+
+    dev = torch.device("cuda", 0)
+    dom = jh.Domain(p=16, q=14, root=jh.DiscretizationNode2D(-1.0, 1.0, -1.0, 1.0), L=7)
+    xs = np.linspace(-1, 1, ngrid)
+    cs = -np.cos(np.pi * np.arange(nsamp) / (nsamp - 1))
+    vals = torch.as_tensor(np.sin(3 * cs)[:, None] * np.cos(2 * cs)[None, :]).to(dev)
+    samples = torch.as_tensor(np.sin(dom.interior_points[..., 0]) * dom.interior_points[..., 1]).to(dev)
+
+    def step():
+        a = dom.interp_to_interior_points(vals, cs, cs)
+        b, _ = dom.interp_from_interior_points(samples, xs, xs)
+        return a, b
+
+    ms = _timed_steps(step, args.steps, args.warmup)
+    _, kernels, roof = profiling.instrumented(step)
+    npts = dom.n_leaves * 256 + ngrid * ngrid
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import interp as oi
+
+        m = 64
+        t, reps = _time_bounded(lambda: oi.interp_from_interior_2D((-1, 1, -1, 1), 7, 16, np.asarray(samples.cpu()),
+                                                                  np.linspace(-1, 1, m), np.linspace(-1, 1, m)), 20.0)
+        cores = len(os.sched_getaffinity(0))
+        cpu = {"value": m * m / t, "unit": "points/s", "cores": cores, "kind": "port",
+               "sample": f"oracle interp_from_interior_2D to a {m}x{m} grid (median of {reps}); "
+                         "the oracle's to-interior direction is not timed"}
+    return _line("HPS<->grid interpolation points/s (both directions)", "points/s", npts / (ms / 1e3), ms,
+                 args.steps, args.warmup, "f64",
+                 {"workload": "interpolation 2D p=16 L=7: 64x64 Chebyshev samples -> HPS grid -> 1024x1024 grid",
+                  "n_leaves": dom.n_leaves, "grid": ngrid, "samples": nsamp}, roof, kernels, cpu)
+
+
+def adaptive_workload(p=16, q=14, tol=1e-12, nbumps=16):
+    """Adaptive 2D Poisson problem: a tree refined around nbumps seeded sharp
+    Gaussian bumps (1468 leaves, depth 7 at the defaults), source and
+    boundary data of their sum."""
+    import numpy as np
+
+    import jaxhps_amd as jh
+
+    rng = np.random.default_rng(0)
+    cs = list(zip(rng.uniform(-0.8, 0.8, nbumps), rng.uniform(-0.8, 0.8, nbumps), rng.uniform(1000, 5000, nbumps)))
+
+    def u(x):
+        return sum(np.exp(-a * ((x[..., 0] - cx) ** 2 + (x[..., 1] - cy) ** 2)) for cx, cy, a in cs)
+
+    def lap(x):
+        out = 0.0
+        for cx, cy, a in cs:
+            r2 = (x[..., 0] - cx) ** 2 + (x[..., 1] - cy) ** 2
+            out = out + (4 * a * a * r2 - 4 * a) * np.exp(-a * r2)
+        return out
+
+    dom = jh.Domain.from_adaptive_discretization(p, q, jh.DiscretizationNode2D(-1.0, 1.0, -1.0, 1.0), u, tol)
+    return dom, u, lap
+
+
+def bench_adaptive(args):
+    """Adaptive 2D DtN build + solve (FP64).  Step = build_solver + solve."""
+    import numpy as np
+    import torch
+
+    import jaxhps_amd as jh
+    from jaxhps_amd import profiling
+
+    dev = torch.device("cuda", 0)
+    dom, u, lap = adaptive_workload()
+    pts = dom.interior_points
+    ones = torch.ones(pts.shape[:2], dtype=torch.float64, device=dev)
+    f = torch.as_tensor(lap(pts)).to(dev)
+    g = torch.as_tensor(np.concatenate(dom.get_adaptive_boundary_data_lst(u))).to(dev)
+    pde = jh.PDEProblem(dom, source=f, D_xx_coefficients=ones, D_yy_coefficients=ones)
+
+    def step():
+        jh.build_solver(pde)
+        return jh.solve(pde, g)
+
+    ms = _timed_steps(step, args.steps, args.warmup)
+    sol, kernels, roof = profiling.instrumented(step)
+    err = float(np.abs(sol.cpu().numpy() - u(pts)).max())
+    dof = dom.n_leaves * dom.p**2
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import adaptive2d as ad
+
+        one = np.ones(pts.shape[:2])
+        fh, gh = lap(pts), g.cpu().numpy()
+        t, reps = _time_bounded(lambda: ad.solve(dom.root, dom.p, dom.q, {"D_xx": one, "D_yy": one}, fh, gh), 20.0)
+        cores = len(os.sched_getaffinity(0))
+        cpu = {"value": dof / t, "unit": "DOF/s", "cores": cores, "kind": "port",
+               "sample": f"adaptive2d oracle, the full build+solve of this tree (median of {reps})"}
+    out = _line("adaptive 2D DtN build+solve DOF/s", "DOF/s", dof / (ms / 1e3), ms, args.steps, args.warmup, "f64",
+                {"workload": "adaptive 2D Poisson, 16 Gaussian bumps, p=16 q=14 tol=1e-12",
+                 "n_leaves": dom.n_leaves, "max_depth": int(max(lf.depth for lf in jh.get_all_leaves(dom.root)))},
+                roof, kernels, cpu)
+    out["max_abs_err_vs_exact"] = err
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -281,6 +463,10 @@ def main():
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
+    if args.workload != "hps":
+        fn = {"scattering": bench_scattering, "interp": bench_interp, "adaptive": bench_adaptive}[args.workload]
+        print(json.dumps(fn(args)), flush=True)
+        return
     dom, kw, g, desc = make_workload(args.config, args.L)
     # inputs resident in HBM before the timed region
     dkw = {}

@@ -245,20 +245,6 @@ int hps_adjoint_up_level(hps_handle_t h, int dtype, int kind, int nm, int nc, in
 int hps_coef_gradient(hps_handle_t h, int dtype, int nl, int n, int row0, int nsrc, int nops, int ncoef,
                       const int* coef_ops, const void* R, int nrow_R, const void* u, const void* lam, double* grad);
 
-/* ---- HPS grid -> points ----------------------------------------------------
- * out (npts, nsrc) = the interpolant of F (n_leaves, p^dim, nsrc) on leaf
- * leaf[n] at pts[n] (dim coordinates, host-chosen leaf: the first in leaf
- * order whose closed box holds the point).  boxes (n_leaves, 2 dim) =
- * [lo, hi] per coordinate; cheb (p) the Chebyshev nodes on [-1, 1], ascending;
- * flip_y (2D) runs the leaf's y nodes top down; grid_to_leaf (p^dim) maps the
- * tensor-grid position (x slowest) to the leaf's point order.  Weights are
- * the barycentric factors with exact coincidences replaced by EPS.  FP64 /
- * complex128.  Device pointers.  Replaces interp_from_hps_2D / _3D
- * (_interpolation_methods.py:24-219) behind Domain.interp_from_interior_points
- * (_domain.py:209-294). */
-int hps_interp_points(hps_handle_t h, int dtype, int npts, int dim, int p, int nsrc, const double* pts,
-                      const int* leaf, const double* boxes, const double* cheb, int flip_y, const int* grid_to_leaf,
-                      const void* F, void* out);
 
 /* ---- adaptive (non-uniform) trees ------------------------------------------
  * X[b*stride + i] *= scale[b] (i < n), scale a device double array: the
@@ -309,6 +295,25 @@ int hps_scattering_system(hps_handle_t h, int dtype, int n, int nsrc, const void
 int hps_scattering_impedance(hps_handle_t h, int dtype, int n, int nsrc, const void* S, const void* D, const void* T,
                              const double* bdry_pts, const double* source_dirs, double k, double eta, void* imp,
                              int* info, void* ws, size_t ws_bytes);
+
+/* Interpolation from the HPS leaf grids: F (n_leaves, p^dim, nsrc) in leaf
+ * point order, boxes (n_leaves, 2 dim) = [lo, hi] per coordinate, cheb (p)
+ * the Chebyshev nodes on [-1, 1] ascending, flip_y (2D) runs the leaf's y
+ * nodes top down, grid_to_leaf (p^dim) maps the tensor-grid position (x
+ * slowest) to the leaf's point order; barycentric weights with exact
+ * coincidences replaced by EPS; FP64 / complex128.  Evaluated on the points
+ * of meshgrid(xs, ys[, zs]) in its default
+ * ("xy") order, flat index (iy * nx + ix) * nz + iz (nz = 1 in 2D), with
+ * each point's leaf found on the device for a uniform tree: root (2 dim) =
+ * [lo, hi] per coordinate, n_side leaves per axis, leaf_of_cell
+ * (n_side^dim, x slowest) the leaf of each cell; a point on shared edges
+ * takes the first leaf in leaf order, one outside every leaf takes leaf 0.
+ * out (n_points, nsrc).  Replaces interp_from_hps_2D / _3D
+ * (_interpolation_methods.py:24-219) for uniform trees. */
+int hps_interp_grid(hps_handle_t h, int dtype, int nx, int ny, int nz, int dim, int p, int nsrc, const double* xs,
+                    const double* ys, const double* zs, int n_side, const double* root, const int* leaf_of_cell,
+                    const double* boxes, const double* cheb, int flip_y, const int* grid_to_leaf, const void* F,
+                    void* out);
 
 /* ---- learned leaf-ItI surrogate (no counterpart in the reference) ---------
  * A small MLP maps a leaf's coefficient samples to a rank-r complex

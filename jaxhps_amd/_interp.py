@@ -8,10 +8,10 @@ column's x factor (p x n_x) is stacked into one matrix and the whole grid of
 leaf values comes from two (three in 3D) GEMMs on the device; each leaf then
 gathers its p^d block in its point order.
 
-From the HPS grid: the evaluation points are assigned on the host to the first
-leaf, in leaf order, whose closed box holds them (the reference's argmax over
-containment), and the HIP kernel ``hps_interp_points`` evaluates each point's
-barycentric interpolant on its leaf.
+From the HPS grid: the HIP kernel ``hps_interp_grid`` takes the grid's axes,
+finds each point's leaf -- the first in leaf order whose closed box holds it
+(the reference's argmax over containment) -- and evaluates the point's
+barycentric interpolant on that leaf.
 """
 
 from __future__ import annotations
@@ -69,6 +69,26 @@ def _intervals(boxes, cells, axis, n_side):
     return out
 
 
+def _leaf_gather(domain, cells, dev):
+    """(n_leaves, p^d) flat indices into the tensor grid of all leaves'
+    points (x slowest) in leaf point order; cached on the domain."""
+    cache = domain.__dict__.setdefault("_interp_cache", {})
+    key = ("gather", str(dev))
+    if key not in cache:
+        p = domain.p
+        P = (2**domain.L) * p
+        if domain.bool_2D:
+            a, b = np.divmod(rearrange_indices_ext_int_2D(p), p)
+            idx = (cells[:, :1] * p + a[None, :]) * P + (cells[:, 1:2] * p + b[None, :])
+        else:
+            a, rem = np.divmod(rearrange_indices_ext_int_3D(p), p * p)
+            b, c = np.divmod(rem, p)
+            idx = ((cells[:, :1] * p + a[None, :]) * P + (cells[:, 1:2] * p + b[None, :])) * P + (
+                cells[:, 2:3] * p + c[None, :])
+        cache[key] = torch.as_tensor(idx, device=dev)
+    return cache[key]
+
+
 def interp_to_interior_points(domain, values, sample_points_x, sample_points_y, sample_points_z=None, device=None):
     """Values on meshgrid(x, y[, z], indexing="ij") -> (n_leaves, p^d) on the device."""
     dev = _device(device)
@@ -90,10 +110,7 @@ def interp_to_interior_points(domain, values, sample_points_x, sample_points_y, 
         Fy = torch.as_tensor(_factors(y, _intervals(boxes, cells, 1, n_side), p, True)).to(dev, dt)
         T1 = _lib.gemm(Fx, v)[0]  # (n_side p, n_y)
         G = _lib.gemm(T1, Fy.t().contiguous())[0]  # (n_side p, n_side p)
-        r = rearrange_indices_ext_int_2D(p)
-        a, b = np.divmod(r, p)
-        idx = (cells[:, :1] * p + a[None, :]) * (n_side * p) + (cells[:, 1:2] * p + b[None, :])
-        return G.reshape(-1)[torch.as_tensor(idx, device=dev)]
+        return G.reshape(-1)[_leaf_gather(domain, cells, dev)]
     z = np.asarray(sample_points_z, dtype=np.float64)
     if tuple(v.shape) != (x.size, y.size, z.size):
         raise ValueError(f"values must have shape {(x.size, y.size, z.size)}")
@@ -105,41 +122,41 @@ def interp_to_interior_points(domain, values, sample_points_x, sample_points_y, 
     T2 = _lib.gemm(Fy, T1.permute(1, 0, 2).reshape(y.size, -1).contiguous())[0]  # (P_y, P_x n_z)
     T2 = T2.reshape(P, P, z.size).permute(1, 0, 2).reshape(P * P, z.size).contiguous()  # (P_x P_y, n_z)
     G = _lib.gemm(T2, Fz.t().contiguous())[0]  # (P_x P_y, P_z)
-    r = rearrange_indices_ext_int_3D(p)
-    a, rem = np.divmod(r, p * p)
-    b, c = np.divmod(rem, p)
-    idx = ((cells[:, :1] * p + a[None, :]) * P + (cells[:, 1:2] * p + b[None, :])) * P + (cells[:, 2:3] * p + c[None, :])
-    return G.reshape(-1)[torch.as_tensor(idx, device=dev)]
+    return G.reshape(-1)[_leaf_gather(domain, cells, dev)]
 
 
-def _first_leaf(boxes, n_side, pts):
-    """Leaf index (leaf order) of the first leaf whose closed box holds each
-    point (pts (npts, d)); points outside every box take leaf 0, the
-    reference's argmax over an all-false row."""
-    d = pts.shape[1]
-    cells = _leaf_cells(boxes, n_side)
-    order = np.full((n_side,) * d, -1, dtype=np.int64)
-    order[tuple(cells[:, a] for a in range(d))] = np.arange(cells.shape[0])
-    lo = boxes[:, 0::2].min(axis=0)
-    hi = boxes[:, 1::2].max(axis=0)
-    base = np.floor((pts - lo) / ((hi - lo) / n_side)).astype(np.int64)
-    none = np.iinfo(np.int64).max
-    best = np.full(pts.shape[0], none, dtype=np.int64)
-    for shift in np.ndindex(*(2,) * d):
-        ii = base - np.asarray(shift)[None, :]
-        ok = np.all((ii >= 0) & (ii < n_side), axis=1)
-        leaf = order[tuple(np.clip(ii[:, a], 0, n_side - 1) for a in range(d))]
-        bx = boxes[leaf]
-        ok &= np.all((pts >= bx[:, 0::2]) & (pts <= bx[:, 1::2]), axis=1)
-        best = np.minimum(best, np.where(ok, leaf, none))
-    best[best == none] = 0
-    return best
+def _grid_consts(domain, dev):
+    """Device tables of a uniform domain for hps_interp_grid, cached on it."""
+    cache = domain.__dict__.setdefault("_interp_cache", {})
+    key = str(dev)
+    if key not in cache:
+        p = domain.p
+        n_side = 2**domain.L
+        boxes = _leaf_boxes(domain)
+        d = boxes.shape[1] // 2
+        cells = _leaf_cells(boxes, n_side)
+        leaf_of_cell = np.empty((n_side,) * d, dtype=np.int32)
+        leaf_of_cell[tuple(cells[:, a] for a in range(d))] = np.arange(cells.shape[0], dtype=np.int32)
+        root = np.stack([boxes[:, 0::2].min(axis=0), boxes[:, 1::2].max(axis=0)], axis=1).reshape(-1)
+        r = rearrange_indices_ext_int_2D(p) if d == 2 else rearrange_indices_ext_int_3D(p)
+        grid_to_leaf = np.empty(r.size, dtype=np.int32)
+        grid_to_leaf[r] = np.arange(r.size, dtype=np.int32)
+
+        def put(a, t):
+            return torch.as_tensor(np.ascontiguousarray(a)).to(device=dev, dtype=t)
+
+        cache[key] = dict(n_side=n_side, d=d, boxes=put(boxes, torch.float64), root=put(root, torch.float64),
+                          leaf_of_cell=put(leaf_of_cell, torch.int32), cheb=put(chebyshev_points(p), torch.float64),
+                          grid_to_leaf=put(grid_to_leaf, torch.int32))
+    return cache[key]
 
 
 def interp_from_interior_points(domain, samples, eval_points_x, eval_points_y, eval_points_z=None, device=None):
     """(vals, target_pts) as the reference returns them: vals (n_x, n_y[, nsrc])
     (3D: (n_x, n_y, n_z)) from the points of meshgrid(x, y[, z]) in its default
-    ("xy") order, and target_pts ((n_y, n_x, 2) or (n_y, n_x, n_z, 3))."""
+    ("xy") order, and target_pts ((n_y, n_x, 2) or (n_y, n_x, n_z, 3)).  The
+    grid points, their leaves and the interpolants are all computed on the
+    device (``hps_interp_grid``)."""
     dev = _device(device)
     s = torch.as_tensor(samples)
     dt = torch.complex128 if s.is_complex() else torch.float64
@@ -147,49 +164,37 @@ def interp_from_interior_points(domain, samples, eval_points_x, eval_points_y, e
     nl, npl = domain.interior_points.shape[:2]
     if tuple(s.shape[:2]) != (nl, npl):
         raise ValueError(f"samples must have leading shape {(nl, npl)}")
+    if not domain.bool_uniform:
+        raise NotImplementedError("interpolation from adaptive trees is not supported by this package")
     multi = s.dim() == 3
     s3 = (s if multi else s.unsqueeze(-1)).contiguous()
     nsrc = s3.shape[-1]
-    x = np.asarray(eval_points_x, dtype=np.float64)
-    y = np.asarray(eval_points_y, dtype=np.float64)
-    p = domain.p
-    boxes = _leaf_boxes(domain)
+    x = np.asarray(eval_points_x, dtype=np.float64).reshape(-1)
+    y = np.asarray(eval_points_y, dtype=np.float64).reshape(-1)
+    c = _grid_consts(domain, dev)
     if domain.bool_2D:
         if eval_points_z is not None:
             raise ValueError("2D domain: eval_points_z must be None")
+        z = np.zeros(1)
         X, Y = np.meshgrid(x, y)
         target = np.stack([X, Y], axis=2)
-        flat = target.reshape(-1, 2)
-        leaf = _first_leaf(boxes, 2**domain.L, flat)
-        r = rearrange_indices_ext_int_2D(p)
         out_shape = (x.size, y.size)
     else:
-        z = np.asarray(eval_points_z, dtype=np.float64)
+        z = np.asarray(eval_points_z, dtype=np.float64).reshape(-1)
         X, Y, Z = np.meshgrid(x, y, z)
         target = np.concatenate([X[..., None], Y[..., None], Z[..., None]], axis=3)
-        flat = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=-1)
-        leaf = _first_leaf(boxes, 2**domain.L, flat)
-        r = rearrange_indices_ext_int_3D(p)
         out_shape = (x.size, y.size, z.size)
-    dim = flat.shape[1]
-    grid_to_leaf = np.empty(r.size, dtype=np.int32)
-    grid_to_leaf[r] = np.arange(r.size, dtype=np.int32)
-    npts = flat.shape[0]
-
-    def d(a, t):
-        return torch.as_tensor(np.ascontiguousarray(a)).to(device=dev, dtype=t)
-
-    pts_d = d(flat, torch.float64)
-    leaf_d = d(leaf, torch.int32)
-    boxes_d = d(boxes, torch.float64)
-    cheb_d = d(chebyshev_points(p), torch.float64)
-    g2l_d = d(grid_to_leaf, torch.int32)
+    nz = 1 if domain.bool_2D else z.size
+    npts = x.size * y.size * nz
+    xs, ys, zs = (torch.as_tensor(a).to(dev) for a in (x, y, z))
     out = torch.empty((npts, nsrc), dtype=dt, device=dev)
     lib = load_library()
     hd = _lib.handle()
-    rc = lib.hps_interp_points(hd, HPS_C128 if dt == torch.complex128 else HPS_F64, npts, dim, p, nsrc, ptr(pts_d),
-                               ptr(leaf_d), ptr(boxes_d), ptr(cheb_d), 1, ptr(g2l_d), ptr(s3), ptr(out))
-    check(rc, hd, "hps_interp_points")
+    rc = lib.hps_interp_grid(hd, HPS_C128 if dt == torch.complex128 else HPS_F64, x.size, y.size, nz, c["d"],
+                             domain.p, nsrc, ptr(xs), ptr(ys), ptr(zs), c["n_side"], ptr(c["root"]),
+                             ptr(c["leaf_of_cell"]), ptr(c["boxes"]), ptr(c["cheb"]), 1, ptr(c["grid_to_leaf"]),
+                             ptr(s3), ptr(out))
+    check(rc, hd, "hps_interp_grid")
     vals = out.reshape(out_shape + (nsrc,)) if multi else out[:, 0].reshape(out_shape)
     return vals, target
 

@@ -71,7 +71,7 @@ SIGNATURES = {
     "hps_adjoint_up_workspace": (_S, [_I, _I, _I, _I, _I]),
     "hps_adjoint_up_level": (_I, [_P] + [_I] * 8 + [_P] * 12 + [_S]),
     "hps_coef_gradient": (_I, [_P] + [_I] * 7 + [_P, _P, _I, _P, _P, _P]),
-    "hps_interp_points": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P, _I, _P, _P, _P]),
+    "hps_interp_grid": (_I, [_P, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _I, _P, _P, _P, _P, _I, _P, _P, _P]),
     "hps_plane_wave": (_I, [_P, _L, _I, _P, _P, _D, _L, _P, _P]),
     "hps_scale_batched": (_I, [_P, _I, _I, _L, _L, _P, _P]),
     "hps_copy2d_batched": (_I, [_P, _I, _I, _I, _D, _P, _L, _L, _P, _L, _L, _I]),

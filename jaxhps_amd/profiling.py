@@ -115,3 +115,26 @@ def instrumented_step(pde, g_dev, build=None, solve=None):
             for r in table[:12]
         ],
     }
+
+
+def instrumented(fn):
+    """Run fn() once with per-kernel instrumentation.  Returns (result,
+    kernels, roofline of the dominant kernel)."""
+    lib = _lib.load_library()
+    torch.cuda.synchronize()
+    lib.hps_prof_reset()
+    lib.hps_prof_enable(1)
+    try:
+        out = fn()
+        torch.cuda.synchronize()
+    finally:
+        lib.hps_prof_enable(0)
+    table = kernel_table()
+    lib.hps_prof_reset()
+    kernels = [
+        dict(name=r["name"], launches=r["launches"], ms=round(r["ms"], 3),
+             tflops=(r["flops"] / (r["ms"] * 1e-3) / 1e12) if r["ms"] > 0 and r["flops"] else None,
+             gbs=(r["bytes"] / (r["ms"] * 1e-3) / 1e9) if r["ms"] > 0 else None)
+        for r in table[:12]
+    ]
+    return out, kernels, (roofline_of(table[0]) if table else None)

@@ -279,8 +279,12 @@ def leaf_stage(root, p, q, coeffs, source):
     lv = leaves(root)
     f = source if source.ndim == 3 else source[..., None]
     Ys, Ts, vs, hs = [], [], [], []
+    ops_of = {}
     for i, lf in enumerate(lv):
-        ops = osp.leaf_ops_2D(p, q, (lf.xmax - lf.xmin) / 2)
+        half = (lf.xmax - lf.xmin) / 2
+        if half not in ops_of:
+            ops_of[half] = osp.leaf_ops_2D(p, q, half)
+        ops = ops_of[half]
         A = hps2d.assemble_diff_operator({k: v[i:i + 1] for k, v in coeffs.items()}, ops)
         Y, T, v, h = hps2d.leaf_dtn(A, f[i:i + 1], ops["P"], ops["Q"])
         Ys.append(Y[0])

@@ -247,29 +247,3 @@ def test_product_fails_loudly_without_gpu():
     pde = jh.PDEProblem(dom, source=ones, D_xx_coefficients=ones, D_yy_coefficients=ones)
     with pytest.raises(jh.HPSLibraryError):
         jh.build_solver(pde)
-
-
-@pytest.mark.parametrize("L,dim", [(1, 2), (3, 2), (2, 3)])
-def test_interp_first_leaf_matches_oracle_rule(L, dim):
-    """Host-side leaf assignment for interpolation from the HPS grid: the first
-    leaf in leaf order whose closed box holds the point (leaf 0 outside)."""
-    import jaxhps_amd as jh
-    from jaxhps_amd._interp import _first_leaf, _leaf_boxes
-    from oracle.interp import _first_containing
-
-    if dim == 2:
-        b = (-1.0, 1.0, -0.5, 0.5)
-        dom = jh.Domain(p=4, q=2, root=jh.DiscretizationNode2D(*b), L=L)
-    else:
-        b = (0.0, 1.0, -1.0, 1.0, 0.0, 2.0)
-        dom = jh.Domain(p=4, q=2, root=jh.DiscretizationNode3D(*b), L=L)
-    boxes = _leaf_boxes(dom)
-    rng = np.random.default_rng(L)
-    lo, hi = np.array(b[0::2]), np.array(b[1::2])
-    grid = [np.linspace(lo[a], hi[a], 2**L + 1) for a in range(dim)]
-    edge = np.stack([g.ravel() for g in np.meshgrid(*grid, indexing="ij")], -1)
-    inner = lo + (hi - lo) * rng.uniform(size=(40, dim))
-    outside = np.array([hi + 0.1, lo - 0.3])
-    pts = np.concatenate([edge, inner, outside])
-    want = [_first_containing(boxes, pt) for pt in pts]
-    assert list(_first_leaf(boxes, 2**L, pts)) == want
