@@ -260,6 +260,14 @@ int hps_scale_batched(hps_handle_t h, int dtype, int batch, long n, long stride,
 int hps_copy2d_batched(hps_handle_t h, int dtype, int rows, int cols, double alpha, const void* src, long lds,
                        long ss, void* dst, long ldd, long sd, int batch);
 
+/* count block copies in one launch: desc (device, count x 6 int64) =
+ * (src byte address, dst byte address, rows, cols, lds, ldd) per block, in
+ * elements of dtype; max_elems >= rows * cols of every block (sizes the
+ * grid).  Packs the children of a group of adaptive merges, and their
+ * boundary data, without a launch per node (merge/_adaptive_2D_DtN.py:
+ * 110-206, down_pass/_adaptive_2D_DtN.py:44-98). */
+int hps_pack_blocks(hps_handle_t h, int dtype, int count, long max_elems, const long long* desc);
+
 /* ---- exterior-scattering coupling of the top-level operator ---------------
  * Dense complex128 work on the n root-boundary Gauss points (S, E, N, W sides
  * of n/4 points each).  Device pointers; info is one device int (0, or j+1

@@ -508,24 +508,40 @@ def _signature(node, q):
 # ---------------------------------------------------------------------------
 # device helpers
 # ---------------------------------------------------------------------------
-def _gemm_raw(M, N, K, A, lda, B, ldb, D_ptr, ldd, dt):
+def _gemm_strided(M, N, K, A, lda, sA, B, ldb, sB, D_ptr, ldd, sD, batch, dt):
+    """D[b] = A[b] @ B[b] with explicit strides (sA = 0 broadcasts A)."""
     lib = load_library()
     hd = _lib.handle()
-    rc = lib.hps_gemm_batched(hd, dtype_code(dt), M, N, K, 1.0, ptr(A), lda, M * lda, ptr(B), ldb, K * ldb, 0.0,
-                              None, ldd, 0, D_ptr, ldd, 0, 1)
+    rc = lib.hps_gemm_batched(hd, dtype_code(dt), M, N, K, 1.0, A, lda, sA, B, ldb, sB, 0.0, None, ldd, 0, D_ptr,
+                              ldd, sD, batch)
     check(rc, hd, "hps_gemm_batched")
 
 
-def _copy_into(src, rows, cols, D_ptr, ldd):
-    lib = load_library()
-    hd = _lib.handle()
-    rc = lib.hps_copy2d_batched(hd, dtype_code(src.dtype), rows, cols, 1.0, ptr(src), cols, rows * cols, D_ptr, ldd,
-                                rows * ldd, 1)
-    check(rc, hd, "hps_copy2d_batched")
+class _Packer:
+    """Collects block copies (device addresses) and issues them as one
+    hps_pack_blocks launch."""
+
+    def __init__(self, dt, dev):
+        self.dt, self.dev, self.rows, self.esz = dt, dev, [], torch.empty((), dtype=dt).element_size()
+
+    def add(self, src, dst_addr, rows, cols, lds, ldd):
+        self.rows.append((src.data_ptr(), dst_addr, rows, cols, lds, ldd))
+        self._keep = getattr(self, "_keep", []) + [src]
+
+    def run(self):
+        if not self.rows:
+            return
+        desc = torch.as_tensor(np.array(self.rows, dtype=np.int64)).to(self.dev)
+        lib = load_library()
+        hd = _lib.handle()
+        rc = lib.hps_pack_blocks(hd, dtype_code(self.dt), len(self.rows), max(r[2] * r[3] for r in self.rows),
+                                 ptr(desc))
+        check(rc, hd, "hps_pack_blocks")
+        self.rows = []
 
 
-def _vp(t, offset_elems=0):
-    return ctypes.c_void_p(t.data_ptr() + offset_elems * t.element_size())
+def _addr(t, offset_elems=0):
+    return t.data_ptr() + offset_elems * t.element_size()
 
 
 # ---------------------------------------------------------------------------
@@ -565,6 +581,7 @@ def merge_stage_adaptive_2D_DtN(pde_problem, T_leaf=None, h_leaf=None) -> None:
             lf.data.h = h_leaf[i] if h_leaf[i].dim() == 2 else h_leaf[i][:, None]
     L21, L12 = pde_problem.L_2f1, pde_problem.L_1f2
     cache = pde_problem.__dict__.setdefault("_adaptive_plans", {})
+    levels = []
     depth = get_depth(root)
     for level in range(depth, 0, -1):
         parents = [nd for nd in get_nodes_at_level(root, level - 1) if len(nd.children)]
@@ -581,27 +598,41 @@ def merge_stage_adaptive_2D_DtN(pde_problem, T_leaf=None, h_leaf=None) -> None:
             nm, nmax = len(plist), plan.nmax
             Tin = torch.empty((nm * 4, nmax, nmax), dtype=dt, device=dev)
             Hin = torch.empty((nm * 4, nmax, nsrc), dtype=dt, device=dev)
+            # compressed positions are staged per position, then compressed by
+            # two batched products (R T) P and one R h
+            stage = {ch: (torch.empty((nm, plan.n[ch], plan.n[ch]), dtype=dt, device=dev),
+                          torch.empty((nm, plan.n[ch], nsrc), dtype=dt, device=dev))
+                     for ch in range(4) if plan.R[ch] is not None}
+            pk = _Packer(dt, dev)
             for k, nd in enumerate(plist):
                 for ch, child in enumerate(nd.children):
-                    slot = (4 * k + ch) * nmax
-                    Tc = child.data.T.contiguous()
-                    hc = child.data.h.contiguous()
-                    n_c, m_c = plan.n[ch], plan.m[ch]
-                    if plan.R[ch] is None:
-                        _copy_into(Tc, n_c, n_c, _vp(Tin, slot * nmax), nmax)
-                        _copy_into(hc, n_c, nsrc, _vp(Hin, slot * nsrc), nsrc)
+                    n_c = plan.n[ch]
+                    Tc, hc = child.data.T, child.data.h
+                    if ch in stage:
+                        pk.add(Tc, _addr(stage[ch][0], k * n_c * n_c), n_c, n_c, n_c, n_c)
+                        pk.add(hc, _addr(stage[ch][1], k * n_c * nsrc), n_c, nsrc, nsrc, nsrc)
                     else:
-                        R, P = plan.ops(ch, dev, dt)
-                        tmp = torch.empty((m_c, n_c), dtype=dt, device=dev)
-                        _gemm_raw(m_c, n_c, n_c, R, n_c, Tc, n_c, ptr(tmp), n_c, dt)
-                        _gemm_raw(m_c, m_c, n_c, tmp, n_c, P, m_c, _vp(Tin, slot * nmax), nmax, dt)
-                        _gemm_raw(m_c, nsrc, n_c, R, n_c, hc, nsrc, _vp(Hin, slot * nsrc), nsrc, dt)
+                        slot = 4 * k + ch
+                        pk.add(Tc, _addr(Tin, slot * nmax * nmax), n_c, n_c, n_c, nmax)
+                        pk.add(hc, _addr(Hin, slot * nmax * nsrc), n_c, nsrc, nsrc, nsrc)
+            pk.run()
+            for ch, (Ts, Hs) in stage.items():
+                n_c, m_c = plan.n[ch], plan.m[ch]
+                R, P = plan.ops(ch, dev, dt)
+                tmp = torch.empty((nm, m_c, n_c), dtype=dt, device=dev)
+                _gemm_strided(m_c, n_c, n_c, ptr(R), n_c, 0, ptr(Ts), n_c, n_c * n_c, ptr(tmp), n_c, m_c * n_c, nm,
+                              dt)
+                _gemm_strided(m_c, m_c, n_c, ptr(tmp), n_c, m_c * n_c, ptr(P), m_c, 0,
+                              ctypes.c_void_p(_addr(Tin, ch * nmax * nmax)), nmax, 4 * nmax * nmax, nm, dt)
+                _gemm_strided(m_c, nsrc, n_c, ptr(R), n_c, 0, ptr(Hs), nsrc, n_c * nsrc,
+                              ctypes.c_void_p(_addr(Hin, ch * nmax * nsrc)), nsrc, 4 * nmax * nsrc, nm, dt)
             out = merge_level(Tin, Hin, plan)
             _lib.raise_on_info(out.info, "adaptive merge")
             for k, nd in enumerate(plist):
                 nd.data.T, nd.data.h = out.T[k], out.h[k]
                 nd.data.S, nd.data.g_tilde = out.S[k], out.gt[k]
-                nd.data._plan = plan
+            levels.append((plan, plist, out))
+    pde_problem._adaptive_levels = levels
 
 
 def down_pass_adaptive_2D_DtN(pde_problem, boundary_data):
@@ -626,31 +657,38 @@ def down_pass_adaptive_2D_DtN(pde_problem, boundary_data):
         raise ValueError(f"boundary data has {g.shape[0]} points, the root boundary {dom.boundary_points.shape[0]}")
     root.data.g = g.contiguous()
     nsrc = g.shape[1]
-    depth = get_depth(root)
-    for level in range(0, depth):
-        parents = [nd for nd in get_nodes_at_level(root, level) if len(nd.children)]
-        groups = OrderedDict()
-        for nd in parents:
-            groups.setdefault(id(nd.data._plan), []).append(nd)
-        for plist in groups.values():
-            plan = plist[0].data._plan
-            S = torch.stack([nd.data.S for nd in plist])
-            gt = torch.stack([nd.data.g_tilde for nd in plist])
-            gin = torch.stack([nd.data.g for nd in plist])
-            if gt.shape[-1] != nsrc:
-                if gt.shape[-1] != 1:
-                    raise ValueError("boundary data and the build's sources disagree in count")
-                gt = gt.expand(-1, -1, nsrc).contiguous()
-            gout = down_level(S.contiguous(), gt.contiguous(), gin.contiguous(), plan)
-            for k, nd in enumerate(plist):
-                for ch, child in enumerate(nd.children):
-                    gc = gout[4 * k + ch, :plan.m[ch]]
-                    if plan.P[ch] is not None:
-                        _, P = plan.ops(ch, dev, dt)
-                        gc = _lib.gemm(P, gc.contiguous())[0]
-                    child.data.g = gc.contiguous()
+    for plan, plist, out in reversed(pde_problem._adaptive_levels):
+        nm, nmax, ne = len(plist), plan.nmax, plan.ne
+        gt = out.gt
+        if gt.shape[-1] != nsrc:
+            if gt.shape[-1] != 1:
+                raise ValueError("boundary data and the build's sources disagree in count")
+            gt = gt.expand(-1, -1, nsrc).contiguous()
+        gin = torch.empty((nm, ne, nsrc), dtype=dt, device=dev)
+        pk = _Packer(dt, dev)
+        for k, nd in enumerate(plist):
+            pk.add(nd.data.g, _addr(gin, k * ne * nsrc), ne, nsrc, nsrc, nsrc)
+        pk.run()
+        gout = down_level(out.S, gt, gin, plan)
+        for ch in range(4):
+            n_c, m_c = plan.n[ch], plan.m[ch]
+            if plan.P[ch] is None:
+                for k, nd in enumerate(plist):
+                    nd.children[ch].data.g = gout[4 * k + ch, :n_c]
+            else:
+                _, P = plan.ops(ch, dev, dt)
+                gc = torch.empty((nm, n_c, nsrc), dtype=dt, device=dev)
+                _gemm_strided(n_c, nsrc, m_c, ptr(P), m_c, 0, ctypes.c_void_p(_addr(gout, ch * nmax * nsrc)), nsrc,
+                              4 * nmax * nsrc, ptr(gc), nsrc, n_c * nsrc, nm, dt)
+                for k, nd in enumerate(plist):
+                    nd.children[ch].data.g = gc[k]
     leaves = get_all_leaves(root)
-    G = torch.stack([lf.data.g for lf in leaves]).contiguous()
+    nb = leaves[0].data.g.shape[0]
+    G = torch.empty((len(leaves), nb, nsrc), dtype=dt, device=dev)
+    pk = _Packer(dt, dev)
+    for i, lf in enumerate(leaves):
+        pk.add(lf.data.g, _addr(G, i * nb * nsrc), nb, nsrc, nsrc, nsrc)
+    pk.run()
     vv = v if v.dim() == 3 else v[..., None]
     if vv.shape[-1] != nsrc:
         vv = vv.expand(-1, -1, nsrc)

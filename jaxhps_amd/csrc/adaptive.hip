@@ -38,9 +38,48 @@ static int copy_t(hps_handle_t h, int rows, int cols, double alpha, const void* 
   return HPS_OK;
 }
 
+// One launch copying `count` blocks between arbitrary device addresses:
+// desc[6*e .. 6*e+5] = (src, dst, rows, cols, lds, ldd) with src/dst byte
+// addresses.  blockIdx.y walks the blocks, blockIdx.x the elements.
+template <typename T>
+__global__ __launch_bounds__(256) void pack_blocks_kernel(int count, const long long* desc) {
+  for (int e = blockIdx.y; e < count; e += gridDim.y) {
+    const long long* d = desc + 6 * (long)e;
+    const T* src = reinterpret_cast<const T*>(d[0]);
+    T* dst = reinterpret_cast<T*>(d[1]);
+    const long rows = d[2], cols = d[3], lds = d[4], ldd = d[5];
+    const long total = rows * cols;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+      const long r = t / cols, c = t - r * cols;
+      dst[r * ldd + c] = src[r * lds + c];
+    }
+  }
+}
+
+template <typename T>
+static int pack_t(hps_handle_t h, int count, long max_elems, const long long* desc) {
+  hipStream_t s = h->stream;
+  unsigned gx = ceil_div(max_elems, 256);
+  if (gx > 64) gx = 64;
+  if (gx == 0) gx = 1;
+  unsigned gy = count < 65535 ? count : 65535;
+  HPS_PROF("pack_blocks", T, 0.0, 2.0 * sizeof(T) * (double)max_elems * count);
+  hipLaunchKernelGGL((pack_blocks_kernel<T>), dim3(gx, gy), dim3(256), 0, s, count, desc);
+  HPS_CHECK_HIP(h, hipGetLastError());
+  return HPS_OK;
+}
+
 }  // namespace hps
 
 using namespace hps;
+
+int hps_pack_blocks(hps_handle_t h, int dtype, int count, long max_elems, const long long* desc) {
+  if (!h) return HPS_ERR_ARG;
+  HPS_ARG(h, count >= 0 && max_elems >= 0, "bad size");
+  if (count == 0 || max_elems == 0) return HPS_OK;
+  HPS_ARG(h, desc, "null pointer");
+  DISPATCH(dtype, pack_t, h, count, max_elems, desc);
+}
 
 int hps_scale_batched(hps_handle_t h, int dtype, int batch, long n, long stride, const double* scale, void* X) {
   if (!h) return HPS_ERR_ARG;
