@@ -636,7 +636,8 @@ def merge_stage_adaptive_2D_DtN(pde_problem, T_leaf=None, h_leaf=None) -> None:
 
 
 def down_pass_adaptive_2D_DtN(pde_problem, boundary_data):
-    """Root Dirichlet data (a list of the four sides' samples, or one array)
+    """Root Dirichlet data (a list of the four sides' -- six faces' in 3D --
+    samples, or one array)
     down to the leaves (``down_pass/_adaptive_2D_DtN.py:16-98``).  Returns u
     (n_leaves, p^2) (with a trailing source axis for multi-source data)."""
     from ._stages import down_level, leaf_apply
@@ -670,16 +671,17 @@ def down_pass_adaptive_2D_DtN(pde_problem, boundary_data):
             pk.add(nd.data.g, _addr(gin, k * ne * nsrc), ne, nsrc, nsrc, nsrc)
         pk.run()
         gout = down_level(out.S, gt, gin, plan)
-        for ch in range(4):
+        nc = plan.nc
+        for ch in range(nc):
             n_c, m_c = plan.n[ch], plan.m[ch]
             if plan.P[ch] is None:
                 for k, nd in enumerate(plist):
-                    nd.children[ch].data.g = gout[4 * k + ch, :n_c]
+                    nd.children[ch].data.g = gout[nc * k + ch, :n_c]
             else:
                 _, P = plan.ops(ch, dev, dt)
                 gc = torch.empty((nm, n_c, nsrc), dtype=dt, device=dev)
                 _gemm_strided(n_c, nsrc, m_c, ptr(P), m_c, 0, ctypes.c_void_p(_addr(gout, ch * nmax * nsrc)), nsrc,
-                              4 * nmax * nsrc, ptr(gc), nsrc, n_c * nsrc, nm, dt)
+                              nc * nmax * nsrc, ptr(gc), nsrc, n_c * nsrc, nm, dt)
                 for k, nd in enumerate(plist):
                     nd.children[ch].data.g = gc[k]
     leaves = get_all_leaves(root)

@@ -116,15 +116,19 @@ def build_solver(pde_problem, return_top_T: bool = False, compute_device=None, h
 
 
 def _adaptive_build_solver(pde_problem, return_top_T, compute_device):
-    """Adaptive 2D DtN build (``_build_solver.py:177-208``): leaf maps, then
+    """Adaptive DtN build (``_build_solver.py:177-258``): leaf maps, then
     the merges, whose outputs live in the tree's nodes."""
     from ._adaptive import local_solve_stage_adaptive_2D_DtN, merge_stage_adaptive_2D_DtN
+    from ._adaptive3d import merge_stage_adaptive_3D_DtN
 
     Y, T, v, h = local_solve_stage_adaptive_2D_DtN(pde_problem, device=compute_device)
     pde_problem.Y, pde_problem.v = Y, v
     for i, leaf in enumerate(get_all_leaves(pde_problem.domain.root)):
         leaf.data.Y, leaf.data.v = Y[i], v[i]
-    merge_stage_adaptive_2D_DtN(pde_problem, T, h)
+    if pde_problem.domain.bool_2D:
+        merge_stage_adaptive_2D_DtN(pde_problem, T, h)
+    else:
+        merge_stage_adaptive_3D_DtN(pde_problem, T, h)
     pde_problem._precision = "fp64"
     if return_top_T:
         return pde_problem.domain.root.data.T

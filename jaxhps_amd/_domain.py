@@ -20,7 +20,7 @@ class Domain:
     points per leaf side, the root node, and L levels of uniform refinement.
     Point arrays are float64 numpy arrays on the host:
     ``interior_points`` (n_leaves, p^d, d) and ``boundary_points``
-    (n_bdry, d).  With L=None the tree is adaptive (2D only): the root's
+    (n_bdry, d).  With L=None the tree is adaptive: the root's
     existing leaves are the leaves, and the side counts n_0..n_3 of every
     node are set.
     """
@@ -41,13 +41,22 @@ class Domain:
                 compute_interior_Chebyshev_points_adaptive_2D,
             )
 
-            if not self.bool_2D:
-                raise NotImplementedError("adaptive 3D trees are not supported by this package")
             self.bool_uniform = False
-            assign_side_counts(root, q)
             self.n_leaves = len(get_all_leaves(root))
-            self.interior_points = compute_interior_Chebyshev_points_adaptive_2D(root, p)
-            self.boundary_points = compute_boundary_Gauss_points_adaptive_2D(root, q)
+            if self.bool_2D:
+                assign_side_counts(root, q)
+                self.interior_points = compute_interior_Chebyshev_points_adaptive_2D(root, p)
+                self.boundary_points = compute_boundary_Gauss_points_adaptive_2D(root, q)
+            else:
+                from ._adaptive3d import (
+                    assign_face_counts,
+                    compute_boundary_Gauss_points_adaptive_3D,
+                    compute_interior_Chebyshev_points_adaptive_3D,
+                )
+
+                assign_face_counts(root, q)
+                self.interior_points = compute_interior_Chebyshev_points_adaptive_3D(root, p)
+                self.boundary_points = compute_boundary_Gauss_points_adaptive_3D(root, q)
             return
         self.bool_uniform = True
         if self.bool_2D:
@@ -81,9 +90,10 @@ class Domain:
         """f evaluated on the root boundary points of each side S, E, N, W
         (``_domain.py:294-357``): a list of four arrays."""
         bp = self.boundary_points
-        if not self.bool_2D:
-            raise NotImplementedError("adaptive 3D trees are not supported by this package")
         r = self.root
+        if not self.bool_2D:
+            return [f(bp[bp[:, 0] == r.xmin]), f(bp[bp[:, 0] == r.xmax]), f(bp[bp[:, 1] == r.ymin]),
+                    f(bp[bp[:, 1] == r.ymax]), f(bp[bp[:, 2] == r.zmin]), f(bp[bp[:, 2] == r.zmax])]
         return [
             f(bp[bp[:, 1] == r.ymin]),
             f(bp[bp[:, 0] == r.xmax]),
@@ -97,12 +107,12 @@ class Domain:
         """Refine root until each function in f is resolved to tol, then build
         the adaptive Domain (``_domain.py:359-434``)."""
         from ._adaptive import generate_adaptive_mesh_level_restriction_2D
+        from ._adaptive3d import generate_adaptive_mesh_level_restriction
 
-        if not isinstance(root, DiscretizationNode2D):
-            raise NotImplementedError("adaptive 3D trees are not supported by this package")
+        gen = (generate_adaptive_mesh_level_restriction_2D if isinstance(root, DiscretizationNode2D)
+               else generate_adaptive_mesh_level_restriction)
         for fn in (f if isinstance(f, list) else [f]):
-            generate_adaptive_mesh_level_restriction_2D(root=root, f_fn=fn, tol=tol, p=p, q=q,
-                                                        restrict_bool=use_level_restriction, l2_norm=use_l_2_norm)
+            gen(root=root, f_fn=fn, tol=tol, p=p, q=q, restrict_bool=use_level_restriction, l2_norm=use_l_2_norm)
         return cls(p=p, q=q, root=root)
 
     def __repr__(self) -> str:

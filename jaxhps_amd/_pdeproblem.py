@@ -114,8 +114,6 @@ class PDEProblem:
         if domain.bool_uniform:
             half_side_len = (domain.root.xmax - domain.root.xmin) / (2 ** (domain.L + 1))
         else:
-            if not bool_2D:
-                raise NotImplementedError("adaptive 3D trees are not supported by this package")
             # operators for half side 1; each leaf's size is applied to its data
             half_side_len = 1.0
             self.sidelens = np.array([lf.xmax - lf.xmin for lf in get_all_leaves(domain.root)])
@@ -150,6 +148,10 @@ class PDEProblem:
             ) = precompute_diff_operators_3D(p, half_side_len)
             self.P = precompute_P_3D_DtN(p, q)
             self.Q = precompute_Q_3D_DtN(p, q, self.D_x, self.D_y, self.D_z)
+            if not domain.bool_uniform:
+                from ._adaptive3d import precompute_projection_ops_3D
+
+                self.L_4f1, self.L_1f4 = precompute_projection_ops_3D(q)
 
         self._dev_cache = {}
         self.reset()

@@ -99,3 +99,10 @@ def down_pass_adaptive_2D_DtN(pde_problem, boundary_data):
     from ._adaptive import down_pass_adaptive_2D_DtN as fn
 
     return fn(pde_problem, boundary_data)
+
+
+def down_pass_adaptive_3D_DtN(pde_problem, boundary_data):
+    """Adaptive 3D down pass (``down_pass/_adaptive_3D_DtN.py:15-142``)."""
+    from ._adaptive3d import down_pass_adaptive_3D_DtN as fn
+
+    return fn(pde_problem, boundary_data)

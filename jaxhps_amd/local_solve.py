@@ -82,3 +82,10 @@ def local_solve_stage_adaptive_2D_DtN(pde_problem, device=None, host_device=None
     from ._adaptive import local_solve_stage_adaptive_2D_DtN as fn
 
     return fn(pde_problem, device=device, host_device=host_device)
+
+
+def local_solve_stage_adaptive_3D_DtN(pde_problem, device=None, host_device=None):
+    """Adaptive 3D leaf stage (``local_solve/_adaptive_3D_DtN.py:13-128``)."""
+    from ._adaptive3d import local_solve_stage_adaptive_3D_DtN as fn
+
+    return fn(pde_problem, device=device, host_device=host_device)

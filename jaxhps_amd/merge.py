@@ -129,3 +129,11 @@ def merge_stage_adaptive_2D_DtN(pde_problem) -> None:
     from ._adaptive import merge_stage_adaptive_2D_DtN as fn
 
     fn(pde_problem)
+
+
+def merge_stage_adaptive_3D_DtN(pde_problem) -> None:
+    """Adaptive 3D merges (``merge/_adaptive_3D_DtN.py:30-149``); expects each
+    leaf's data.T and data.h to be set."""
+    from ._adaptive3d import merge_stage_adaptive_3D_DtN as fn
+
+    fn(pde_problem)

@@ -17,6 +17,7 @@ global_system  the global HPS discrete system solved densely (cross-check)
 interp         HPS grid <-> rectangular grid by explicit Lagrange products
 scattering     ItI -> DtN, plane waves, exterior boundary-integral coupling
 adaptive2d     adaptive 2D DtN: refinement, compressed merges, down pass
+adaptive3d     adaptive 3D DtN: face-panel compression, 8:1 merges, down pass
 
 Pinning (what checks each function beyond re-typing it):
 * spectral: textbook D-matrices n=2,3,4 and Chebyshev/Gauss points from the
@@ -38,6 +39,8 @@ Pinning (what checks each function beyond re-typing it):
 * adaptive2d: identity with hps2d on uniform trees; exact polynomial
   solutions on non-uniform trees; the reference's panel-norm closed forms;
   level restriction of refined trees.
+* adaptive3d: identity with hps3d on uniform octrees; a cubic solution exact
+  on a non-uniform octree; exact face projections.
 """
 
 from . import spectral, hps2d, hps3d, global_system  # noqa: F401
