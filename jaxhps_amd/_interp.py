@@ -20,6 +20,7 @@ import numpy as np
 import torch
 
 from . import _lib
+from ._discretization_tree import get_all_leaves
 from ._grid_creation import _subdivide_2D, _subdivide_3D, rearrange_indices_ext_int_2D, rearrange_indices_ext_int_3D
 from ._lib import HPS_C128, HPS_F64, check, load_library, ptr
 from .quadrature import _bary_factor_eps, affine_transform, chebyshev_points
@@ -43,48 +44,46 @@ def _factors(samples, intervals, p, flip):
 
 
 def _leaf_boxes(domain):
-    """(n_leaves, 2 d) [lo, hi] per coordinate of the leaves, in leaf order,
-    from the same recursive bisection the tree uses."""
-    if domain.bool_2D:
-        b = _subdivide_2D([domain.root.xmin, domain.root.xmax, domain.root.ymin, domain.root.ymax], domain.L)
-    else:
+    """(n_leaves, 2 d) [lo, hi] per coordinate of the leaves, in leaf order."""
+    cache = domain.__dict__.setdefault("_interp_cache", {})
+    if "boxes" not in cache:
         r = domain.root
-        b = _subdivide_3D([r.xmin, r.xmax, r.ymin, r.ymax, r.zmin, r.zmax], domain.L)
-    return np.asarray(b, dtype=np.float64).reshape(-1, 4 if domain.bool_2D else 6)
+        if not domain.bool_uniform:
+            keys = ("xmin", "xmax", "ymin", "ymax") + (() if domain.bool_2D else ("zmin", "zmax"))
+            b = [[getattr(lf, k) for k in keys] for lf in get_all_leaves(r)]
+        elif domain.bool_2D:
+            b = _subdivide_2D([r.xmin, r.xmax, r.ymin, r.ymax], domain.L)
+        else:
+            b = _subdivide_3D([r.xmin, r.xmax, r.ymin, r.ymax, r.zmin, r.zmax], domain.L)
+        cache["boxes"] = np.asarray(b, dtype=np.float64).reshape(-1, 4 if domain.bool_2D else 6)
+    return cache["boxes"]
 
 
-def _leaf_cells(boxes, n_side):
-    """(n_leaves, d) integer cell coordinates of the leaves."""
-    d = boxes.shape[1] // 2
-    lo = boxes[:, 0::2].min(axis=0)
-    hi = boxes[:, 1::2].max(axis=0)
-    centers = 0.5 * (boxes[:, 0::2] + boxes[:, 1::2])
-    return np.clip(np.floor((centers - lo) / ((hi - lo) / n_side)).astype(np.int64), 0, n_side - 1).reshape(-1, d)
+def _axis_intervals(boxes, axis):
+    """The distinct leaf intervals [lo, hi] along axis (sorted) and each
+    leaf's index into them."""
+    uniq, inv = np.unique(boxes[:, 2 * axis:2 * axis + 2], axis=0, return_inverse=True)
+    return uniq, np.asarray(inv).reshape(-1)
 
 
-def _intervals(boxes, cells, axis, n_side):
-    """[lo, hi] of each cell column along axis, taken from the leaves' boxes."""
-    out = np.empty((n_side, 2))
-    out[cells[:, axis]] = boxes[:, 2 * axis:2 * axis + 2]
-    return out
-
-
-def _leaf_gather(domain, cells, dev):
-    """(n_leaves, p^d) flat indices into the tensor grid of all leaves'
-    points (x slowest) in leaf point order; cached on the domain."""
+def _leaf_gather(domain, boxes, dev):
+    """(n_leaves, p^d) flat indices into the tensor grid of every distinct
+    interval's points (x slowest) in leaf point order; cached on the domain."""
     cache = domain.__dict__.setdefault("_interp_cache", {})
     key = ("gather", str(dev))
     if key not in cache:
         p = domain.p
-        P = (2**domain.L) * p
-        if domain.bool_2D:
+        d = boxes.shape[1] // 2
+        ivs = [_axis_intervals(boxes, a) for a in range(d)]
+        sizes = [len(u) * p for u, _ in ivs]
+        if d == 2:
             a, b = np.divmod(rearrange_indices_ext_int_2D(p), p)
-            idx = (cells[:, :1] * p + a[None, :]) * P + (cells[:, 1:2] * p + b[None, :])
+            idx = (ivs[0][1][:, None] * p + a[None, :]) * sizes[1] + (ivs[1][1][:, None] * p + b[None, :])
         else:
             a, rem = np.divmod(rearrange_indices_ext_int_3D(p), p * p)
             b, c = np.divmod(rem, p)
-            idx = ((cells[:, :1] * p + a[None, :]) * P + (cells[:, 1:2] * p + b[None, :])) * P + (
-                cells[:, 2:3] * p + c[None, :])
+            idx = ((ivs[0][1][:, None] * p + a[None, :]) * sizes[1] + (ivs[1][1][:, None] * p + b[None, :])) * sizes[
+                2] + (ivs[2][1][:, None] * p + c[None, :])
         cache[key] = torch.as_tensor(idx, device=dev)
     return cache[key]
 
@@ -97,47 +96,52 @@ def interp_to_interior_points(domain, values, sample_points_x, sample_points_y, 
     v = torch.as_tensor(values)
     dt = torch.complex128 if v.is_complex() else torch.float64
     v = v.to(device=dev, dtype=dt).contiguous()
-    p, L = domain.p, domain.L
-    n_side = 2**L
+    p = domain.p
     boxes = _leaf_boxes(domain)
-    cells = _leaf_cells(boxes, n_side)
     if domain.bool_2D:
         if sample_points_z is not None:
             raise ValueError("2D domain: sample_points_z must be None")
         if tuple(v.shape) != (x.size, y.size):
             raise ValueError(f"values must have shape {(x.size, y.size)}")
-        Fx = torch.as_tensor(_factors(x, _intervals(boxes, cells, 0, n_side), p, False)).to(dev, dt).contiguous()
-        Fy = torch.as_tensor(_factors(y, _intervals(boxes, cells, 1, n_side), p, True)).to(dev, dt)
-        T1 = _lib.gemm(Fx, v)[0]  # (n_side p, n_y)
-        G = _lib.gemm(T1, Fy.t().contiguous())[0]  # (n_side p, n_side p)
-        return G.reshape(-1)[_leaf_gather(domain, cells, dev)]
+        Fx = torch.as_tensor(_factors(x, _axis_intervals(boxes, 0)[0], p, False)).to(dev, dt).contiguous()
+        Fy = torch.as_tensor(_factors(y, _axis_intervals(boxes, 1)[0], p, True)).to(dev, dt)
+        T1 = _lib.gemm(Fx, v)[0]  # (n_xint p, n_y)
+        G = _lib.gemm(T1, Fy.t().contiguous())[0]  # (n_xint p, n_yint p)
+        return G.reshape(-1)[_leaf_gather(domain, boxes, dev)]
     z = np.asarray(sample_points_z, dtype=np.float64)
     if tuple(v.shape) != (x.size, y.size, z.size):
         raise ValueError(f"values must have shape {(x.size, y.size, z.size)}")
-    Fx = torch.as_tensor(_factors(x, _intervals(boxes, cells, 0, n_side), p, False)).to(dev, dt).contiguous()
-    Fy = torch.as_tensor(_factors(y, _intervals(boxes, cells, 1, n_side), p, False)).to(dev, dt).contiguous()
-    Fz = torch.as_tensor(_factors(z, _intervals(boxes, cells, 2, n_side), p, False)).to(dev, dt)
-    P = n_side * p
-    T1 = _lib.gemm(Fx, v.reshape(x.size, -1))[0].reshape(P, y.size, z.size)  # x contracted
-    T2 = _lib.gemm(Fy, T1.permute(1, 0, 2).reshape(y.size, -1).contiguous())[0]  # (P_y, P_x n_z)
-    T2 = T2.reshape(P, P, z.size).permute(1, 0, 2).reshape(P * P, z.size).contiguous()  # (P_x P_y, n_z)
-    G = _lib.gemm(T2, Fz.t().contiguous())[0]  # (P_x P_y, P_z)
-    return G.reshape(-1)[_leaf_gather(domain, cells, dev)]
+    Fx = torch.as_tensor(_factors(x, _axis_intervals(boxes, 0)[0], p, False)).to(dev, dt).contiguous()
+    Fy = torch.as_tensor(_factors(y, _axis_intervals(boxes, 1)[0], p, False)).to(dev, dt).contiguous()
+    Fz = torch.as_tensor(_factors(z, _axis_intervals(boxes, 2)[0], p, False)).to(dev, dt)
+    Px, Py, Pz = Fx.shape[0], Fy.shape[0], Fz.shape[0]
+    T1 = _lib.gemm(Fx, v.reshape(x.size, -1))[0].reshape(Px, y.size, z.size)  # x contracted
+    T2 = _lib.gemm(Fy, T1.permute(1, 0, 2).reshape(y.size, -1).contiguous())[0]  # (Py, Px n_z)
+    T2 = T2.reshape(Py, Px, z.size).permute(1, 0, 2).reshape(Px * Py, z.size).contiguous()  # (Px Py, n_z)
+    G = _lib.gemm(T2, Fz.t().contiguous())[0]  # (Px Py, Pz)
+    return G.reshape(-1)[_leaf_gather(domain, boxes, dev)]
 
 
 def _grid_consts(domain, dev):
-    """Device tables of a uniform domain for hps_interp_grid, cached on it."""
+    """Device tables of a domain for hps_interp_grid, cached on it: the leaf
+    of every cell of the uniform grid at the finest leaf size."""
     cache = domain.__dict__.setdefault("_interp_cache", {})
     key = str(dev)
     if key not in cache:
         p = domain.p
-        n_side = 2**domain.L
         boxes = _leaf_boxes(domain)
         d = boxes.shape[1] // 2
-        cells = _leaf_cells(boxes, n_side)
+        lo = boxes[:, 0::2].min(axis=0)
+        hi = boxes[:, 1::2].max(axis=0)
+        width = (boxes[:, 1] - boxes[:, 0]).min()
+        n_side = int(round((hi[0] - lo[0]) / width))
+        h = (hi - lo) / n_side
         leaf_of_cell = np.empty((n_side,) * d, dtype=np.int32)
-        leaf_of_cell[tuple(cells[:, a] for a in range(d))] = np.arange(cells.shape[0], dtype=np.int32)
-        root = np.stack([boxes[:, 0::2].min(axis=0), boxes[:, 1::2].max(axis=0)], axis=1).reshape(-1)
+        c0 = np.rint((boxes[:, 0::2] - lo) / h).astype(np.int64)
+        c1 = np.rint((boxes[:, 1::2] - lo) / h).astype(np.int64)
+        for l in range(boxes.shape[0]):
+            leaf_of_cell[tuple(slice(c0[l, a], c1[l, a]) for a in range(d))] = l
+        root = np.stack([lo, hi], axis=1).reshape(-1)
         r = rearrange_indices_ext_int_2D(p) if d == 2 else rearrange_indices_ext_int_3D(p)
         grid_to_leaf = np.empty(r.size, dtype=np.int32)
         grid_to_leaf[r] = np.arange(r.size, dtype=np.int32)
@@ -164,8 +168,6 @@ def interp_from_interior_points(domain, samples, eval_points_x, eval_points_y, e
     nl, npl = domain.interior_points.shape[:2]
     if tuple(s.shape[:2]) != (nl, npl):
         raise ValueError(f"samples must have leading shape {(nl, npl)}")
-    if not domain.bool_uniform:
-        raise NotImplementedError("interpolation from adaptive trees is not supported by this package")
     multi = s.dim() == 3
     s3 = (s if multi else s.unsqueeze(-1)).contiguous()
     nsrc = s3.shape[-1]

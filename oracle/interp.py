@@ -66,6 +66,33 @@ def _leaf_interp_row(box, p, pt, order_pts):
     return w
 
 
+def to_points(pts, values, axes):
+    """values on meshgrid(*axes, indexing="ij") -> every point of pts
+    (n_leaves, n, d): tensor-product Lagrange interpolation."""
+    flat = pts.reshape(-1, pts.shape[-1])
+    Ls = [lagrange_matrix(np.asarray(a, float), flat[:, k]) for k, a in enumerate(axes)]
+    if len(axes) == 2:
+        out = np.einsum("ni,ij,nj->n", Ls[0], values, Ls[1])
+    else:
+        out = np.einsum("ni,ijk,nj,nk->n", Ls[0], values, Ls[1], Ls[2])
+    return out.reshape(pts.shape[:2])
+
+
+def from_leaves(boxes, pts, samples, axes):
+    """HPS samples on leaves with the given boxes (leaf order) and points ->
+    the points of meshgrid(*axes) (default "xy" order), flattened, with the
+    first leaf holding each point (any tree)."""
+    p = round(pts.shape[1] ** (1.0 / len(axes)))
+    grids = np.meshgrid(*axes)
+    flat = np.stack([g.ravel() for g in grids], -1)
+    out = []
+    for pt in flat:
+        i = _first_containing(boxes, pt)
+        w = _leaf_interp_row(boxes[i], p, pt, pts[i])
+        out.append(np.tensordot(w, samples[i], axes=(0, 0)))
+    return np.asarray(out)
+
+
 def interp_from_interior_2D(bounds, L, p, samples, x_vals, y_vals):
     """samples (n_leaves, p^2[, s]) -> (vals reshaped (n_x, n_y[, s]), target
     points (n_y, n_x, 2)) with the reference's meshgrid(x, y) ("xy") point

@@ -125,3 +125,44 @@ def test_round_trip_grid_to_hps_and_back():
     hps = dom.interp_to_interior_points(f(*np.meshgrid(xs, ys, indexing="ij")), xs, ys)
     vals, tgt = dom.interp_from_interior_points(hps, xs, ys)
     assert np.abs(_np(vals) - f(tgt[..., 0], tgt[..., 1]).reshape(5, 4)).max() < 1e-11
+
+
+def test_interp_on_adaptive_trees_2D_and_3D():
+    """Both directions on non-uniform trees (the leaf of a point is the first
+    adaptive leaf holding it)."""
+    import jaxhps_amd as jh
+    from jaxhps_amd import _adaptive as pa
+    from jaxhps_amd import _adaptive3d as pa3
+
+    rng = np.random.default_rng(11)
+    root = jh.DiscretizationNode2D(-1.0, 1.0, -0.5, 0.5)
+    pa.add_four_children(root, root=root, q=4)
+    pa.add_four_children(root.children[1], root=root, q=4)
+    pa.add_four_children(root.children[1].children[3], root=root, q=4)
+    dom = jh.Domain(p=7, q=4, root=root)
+    boxes = [(lf.xmin, lf.xmax, lf.ymin, lf.ymax) for lf in jh.get_all_leaves(root)]
+    x, y = np.linspace(-1, 1, 6), np.linspace(-0.5, 0.5, 5)
+    V = rng.standard_normal((6, 5))
+    got = _np(dom.interp_to_interior_points(torch.as_tensor(V), x, y))
+    assert np.abs(got - oi.to_points(dom.interior_points, V, (x, y))).max() < 1e-9 * np.abs(V).max() * 10
+    S = rng.standard_normal(dom.interior_points.shape[:2])
+    xv = np.concatenate([np.linspace(-1, 1, 9), rng.uniform(-1, 1, 4)])
+    yv = np.concatenate([np.linspace(-0.5, 0.5, 9), rng.uniform(-0.5, 0.5, 3)])
+    vals, _ = dom.interp_from_interior_points(torch.as_tensor(S), xv, yv)
+    want = oi.from_leaves(boxes, dom.interior_points, S, (xv, yv)).reshape(xv.size, yv.size)
+    assert np.abs(_np(vals) - want).max() <= 1e-9 * np.abs(S).max()
+
+    root3 = jh.DiscretizationNode3D(0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
+    pa3.add_eight_children(root3, root=root3, q=2)
+    pa3.add_eight_children(root3.children[2], root=root3, q=2)
+    dom3 = jh.Domain(p=4, q=2, root=root3)
+    boxes3 = [(lf.xmin, lf.xmax, lf.ymin, lf.ymax, lf.zmin, lf.zmax) for lf in jh.get_all_leaves(root3)]
+    S3 = rng.standard_normal(dom3.interior_points.shape[:2])
+    ax = [np.linspace(0, 1, 5), np.linspace(0, 1, 3), np.linspace(0, 1, 4)]
+    vals3, _ = dom3.interp_from_interior_points(torch.as_tensor(S3), *ax)
+    want3 = oi.from_leaves(boxes3, dom3.interior_points, S3, ax).reshape(5, 3, 4)
+    assert np.abs(_np(vals3) - want3).max() <= 1e-9 * np.abs(S3).max()
+    V3 = rng.standard_normal((3, 4, 2))
+    ax3 = [np.linspace(0, 1, 3), np.linspace(0, 1, 4), np.linspace(0, 1, 2)]
+    got3 = _np(dom3.interp_to_interior_points(torch.as_tensor(V3), *ax3))
+    assert np.abs(got3 - oi.to_points(dom3.interior_points, V3, ax3)).max() < 1e-9 * 10
