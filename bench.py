@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--refine-tol", type=float, default=1e-11)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint grads/s measurement")
     ap.add_argument("--adjoint-L", type=int, default=6, help="tree depth of the adjoint workload (testing only)")
-    ap.add_argument("--workload", default="hps", choices=["hps", "scattering", "interp", "adaptive"],
+    ap.add_argument("--workload", default="hps", choices=["hps", "scattering", "interp", "adaptive", "adaptive3d"],
                     help="hps: the headline build+solve; the others time the exterior coupling, the grid "
                          "interpolation and the adaptive 2D build+solve")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -433,6 +433,74 @@ def bench_adaptive(args):
     return out
 
 
+def bench_adaptive3d(args, p=8, q=6, tol=1e-6):
+    """Adaptive 3D DtN build + solve (FP64) of a Poisson problem refined
+    around four seeded Gaussian bumps (1331 leaves, depth 5 at the
+    defaults).  Step = build_solver + solve."""
+    import numpy as np
+    import torch
+
+    import jaxhps_amd as jh
+    from jaxhps_amd import profiling
+
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(0)
+    cs = list(zip(rng.uniform(0.2, 0.8, 4), rng.uniform(0.2, 0.8, 4), rng.uniform(0.2, 0.8, 4),
+                  rng.uniform(100, 300, 4)))
+
+    def u(x):
+        return sum(np.exp(-a * ((x[..., 0] - cx) ** 2 + (x[..., 1] - cy) ** 2 + (x[..., 2] - cz) ** 2))
+                   for cx, cy, cz, a in cs)
+
+    def lap(x):
+        out = 0.0
+        for cx, cy, cz, a in cs:
+            r2 = (x[..., 0] - cx) ** 2 + (x[..., 1] - cy) ** 2 + (x[..., 2] - cz) ** 2
+            out = out + (4 * a * a * r2 - 6 * a) * np.exp(-a * r2)
+        return out
+
+    dom = jh.Domain.from_adaptive_discretization(p, q, jh.DiscretizationNode3D(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), u, tol)
+    pts = dom.interior_points
+    ones = torch.ones(pts.shape[:2], dtype=torch.float64, device=dev)
+    f = torch.as_tensor(lap(pts)).to(dev)
+    g = torch.as_tensor(np.concatenate(dom.get_adaptive_boundary_data_lst(u))).to(dev)
+    pde = jh.PDEProblem(dom, source=f, D_xx_coefficients=ones, D_yy_coefficients=ones, D_zz_coefficients=ones)
+
+    def step():
+        jh.build_solver(pde)
+        return jh.solve(pde, g)
+
+    ms = _timed_steps(step, args.steps, args.warmup)
+    sol, kernels, roof = profiling.instrumented(step)
+    err = float(np.abs(sol.cpu().numpy() - u(pts)).max())
+    dof = dom.n_leaves * p**3
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import adaptive3d as ad
+
+        o = ad.Node((0.0, 1.0, 0.0, 1.0, 0.0, 1.0))
+
+        def mirror(nd, on):
+            if len(nd.children):
+                ad.split(on)
+                for c, oc in zip(nd.children, on.children):
+                    mirror(c, oc)
+
+        mirror(dom.root, o)
+        one = np.ones(pts.shape[:2])
+        fh, gh = lap(pts), g.cpu().numpy()
+        t, reps = _time_bounded(lambda: ad.solve(o, p, q, {"D_xx": one, "D_yy": one, "D_zz": one}, fh, gh), 20.0)
+        cores = len(os.sched_getaffinity(0))
+        cpu = {"value": dof / t, "unit": "DOF/s", "cores": cores, "kind": "port",
+               "sample": f"adaptive3d oracle, the full build+solve of this tree (median of {reps})"}
+    out = _line("adaptive 3D DtN build+solve DOF/s", "DOF/s", dof / (ms / 1e3), ms, args.steps, args.warmup, "f64",
+                {"workload": f"adaptive 3D Poisson, 4 Gaussian bumps, p={p} q={q} tol={tol:g}",
+                 "n_leaves": dom.n_leaves, "max_depth": int(max(lf.depth for lf in jh.get_all_leaves(dom.root)))},
+                roof, kernels, cpu)
+    out["max_abs_err_vs_exact"] = err
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -464,7 +532,8 @@ def main():
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
     if args.workload != "hps":
-        fn = {"scattering": bench_scattering, "interp": bench_interp, "adaptive": bench_adaptive}[args.workload]
+        fn = {"scattering": bench_scattering, "interp": bench_interp, "adaptive": bench_adaptive,
+              "adaptive3d": bench_adaptive3d}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
     dom, kw, g, desc = make_workload(args.config, args.L)
