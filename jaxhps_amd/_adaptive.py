@@ -522,11 +522,17 @@ class _Packer:
     hps_pack_blocks launch."""
 
     def __init__(self, dt, dev):
-        self.dt, self.dev, self.rows, self.esz = dt, dev, [], torch.empty((), dtype=dt).element_size()
+        self.dt, self.dev, self.rows, self.keep = dt, dev, [], []
 
     def add(self, src, dst_addr, rows, cols, lds, ldd):
+        """Copy the rows x cols block of src (row stride lds) to dst_addr
+        (row stride ldd); src must hold that block from its data pointer."""
+        if src.dtype != self.dt or src.device != self.dev:
+            raise ValueError("pack source has the wrong dtype or device")
+        if src.numel() < (rows - 1) * lds + cols:
+            raise ValueError("pack source is smaller than the block")
         self.rows.append((src.data_ptr(), dst_addr, rows, cols, lds, ldd))
-        self._keep = getattr(self, "_keep", []) + [src]
+        self.keep.append(src)
 
     def run(self):
         if not self.rows:
