@@ -572,73 +572,129 @@ def local_solve_stage_adaptive_2D_DtN(pde_problem, device=None, host_device=None
     return res.Y, res.T, v, h
 
 
+_STREAMS: dict = {}
+
+
+def _side_streams(dev, n=4):
+    """A small pool of default-priority streams on which the independent merge
+    groups of one tree level run concurrently (their kernels are small and
+    latency-bound)."""
+    key = str(dev)
+    if key not in _STREAMS:
+        _STREAMS[key] = [torch.cuda.Stream(device=dev) for _ in range(n)]
+    return _STREAMS[key]
+
+
+def _run_level(dev, items, body):
+    """body(*item) for every item, spread over the side streams; the current
+    stream waits for all of them before returning, and they wait for it
+    before starting."""
+    main = torch.cuda.current_stream(dev)
+    if len(items) <= 1:
+        for it in items:
+            body(*it)
+        return
+    ss = _side_streams(dev)
+    for st in ss:
+        st.wait_stream(main)
+    for i, it in enumerate(items):
+        with torch.cuda.stream(ss[i % len(ss)]):
+            body(*it)
+    for st in ss:
+        main.wait_stream(st)
+
+
+def _merge_group(plan, plist, out_list):
+    """Pack, compress and merge the children of the parents in plist (one
+    signature); appends (plan, plist, merge outputs)."""
+    from ._stages import merge_level
+
+    nc = plan.nc
+    T0 = plist[0].children[0].data.T
+    dev, dt = T0.device, T0.dtype
+    nsrc = plist[0].children[0].data.h.shape[-1]
+    nm, nmax = len(plist), plan.nmax
+    Tin = torch.empty((nm * nc, nmax, nmax), dtype=dt, device=dev)
+    Hin = torch.empty((nm * nc, nmax, nsrc), dtype=dt, device=dev)
+    # compressed positions are staged per position, then compressed by two
+    # batched products (R T) P and one R h
+    stage = {ch: (torch.empty((nm, plan.n[ch], plan.n[ch]), dtype=dt, device=dev),
+                  torch.empty((nm, plan.n[ch], nsrc), dtype=dt, device=dev))
+             for ch in range(nc) if plan.R[ch] is not None}
+    pk = _Packer(dt, dev)
+    for k, nd in enumerate(plist):
+        for ch, child in enumerate(nd.children):
+            n_c = plan.n[ch]
+            Tc, hc = child.data.T, child.data.h
+            if ch in stage:
+                pk.add(Tc, _addr(stage[ch][0], k * n_c * n_c), n_c, n_c, n_c, n_c)
+                pk.add(hc, _addr(stage[ch][1], k * n_c * nsrc), n_c, nsrc, nsrc, nsrc)
+            else:
+                slot = nc * k + ch
+                pk.add(Tc, _addr(Tin, slot * nmax * nmax), n_c, n_c, n_c, nmax)
+                pk.add(hc, _addr(Hin, slot * nmax * nsrc), n_c, nsrc, nsrc, nsrc)
+    pk.run()
+    for ch, (Ts, Hs) in stage.items():
+        n_c, m_c = plan.n[ch], plan.m[ch]
+        R, P = plan.ops(ch, dev, dt)
+        tmp = torch.empty((nm, m_c, n_c), dtype=dt, device=dev)
+        _gemm_strided(m_c, n_c, n_c, ptr(R), n_c, 0, ptr(Ts), n_c, n_c * n_c, ptr(tmp), n_c, m_c * n_c, nm, dt)
+        _gemm_strided(m_c, m_c, n_c, ptr(tmp), n_c, m_c * n_c, ptr(P), m_c, 0,
+                      ctypes.c_void_p(_addr(Tin, ch * nmax * nmax)), nmax, nc * nmax * nmax, nm, dt)
+        _gemm_strided(m_c, nsrc, n_c, ptr(R), n_c, 0, ptr(Hs), nsrc, n_c * nsrc,
+                      ctypes.c_void_p(_addr(Hin, ch * nmax * nsrc)), nsrc, nc * nmax * nsrc, nm, dt)
+    out = merge_level(Tin, Hin, plan)
+    for k, nd in enumerate(plist):
+        nd.data.T, nd.data.h = out.T[k], out.h[k]
+        nd.data.S, nd.data.g_tilde = out.S[k], out.gt[k]
+    out_list.append((plan, plist, out))
+
+
+def adaptive_merge_stage(pde_problem, T_leaf, h_leaf, signature, make_plan) -> None:
+    """Merges from the deepest level to the root for 2D or 3D adaptive trees.
+    Groups of one level (one per signature) run concurrently; the pivots are
+    checked once at the end."""
+    dom = pde_problem.domain
+    root, q = dom.root, dom.q
+    if T_leaf is not None:
+        for i, lf in enumerate(get_all_leaves(root)):
+            lf.data.T = T_leaf[i]
+            lf.data.h = h_leaf[i] if h_leaf[i].dim() == 2 else h_leaf[i][:, None]
+    cache = pde_problem.__dict__.setdefault("_adaptive_plans", {})
+    dev = get_all_leaves(root)[0].data.T.device
+    dt = get_all_leaves(root)[0].data.T.dtype
+    levels = []
+    for level in range(get_depth(root), 0, -1):
+        parents = [nd for nd in get_nodes_at_level(root, level - 1) if len(nd.children)]
+        groups = OrderedDict()
+        for nd in parents:
+            groups.setdefault(signature(nd, q), []).append(nd)
+        items = []
+        for sig, plist in groups.items():
+            plan = cache.get(sig)
+            if plan is None:
+                plan = cache[sig] = make_plan(sig)
+            # device tables and operators on the current stream, before the
+            # side streams read them
+            plan.device(dev)
+            for ch in range(plan.nc):
+                if plan.R[ch] is not None:
+                    plan.ops(ch, dev, dt)
+            items.append((plan, plist, levels))
+        _run_level(dev, items, _merge_group)
+    if levels:
+        bad = torch.cat([out.info for _, _, out in levels])
+        _lib.raise_on_info(bad, "adaptive merge")
+    pde_problem._adaptive_levels = levels
+
+
 def merge_stage_adaptive_2D_DtN(pde_problem, T_leaf=None, h_leaf=None) -> None:
     """Merges from the deepest level to the root (``merge/_adaptive_2D_DtN.py:25-110``);
     each merged node keeps data.T, data.h, data.S, data.g_tilde (device tensors,
     h and g_tilde with a source axis)."""
-    from ._stages import merge_level
-
-    dom = pde_problem.domain
-    root, q = dom.root, dom.q
-    leaves = get_all_leaves(root)
-    if T_leaf is not None:
-        for i, lf in enumerate(leaves):
-            lf.data.T = T_leaf[i]
-            lf.data.h = h_leaf[i] if h_leaf[i].dim() == 2 else h_leaf[i][:, None]
     L21, L12 = pde_problem.L_2f1, pde_problem.L_1f2
-    cache = pde_problem.__dict__.setdefault("_adaptive_plans", {})
-    levels = []
-    depth = get_depth(root)
-    for level in range(depth, 0, -1):
-        parents = [nd for nd in get_nodes_at_level(root, level - 1) if len(nd.children)]
-        groups = OrderedDict()
-        for nd in parents:
-            groups.setdefault(_signature(nd, q), []).append(nd)
-        for sig, plist in groups.items():
-            plan = cache.get(sig)
-            if plan is None:
-                plan = cache[sig] = _Plan(sig[0], sig[1], q, L21, L12)
-            T0 = plist[0].children[0].data.T
-            dev, dt = T0.device, T0.dtype
-            nsrc = plist[0].children[0].data.h.shape[-1]
-            nm, nmax = len(plist), plan.nmax
-            Tin = torch.empty((nm * 4, nmax, nmax), dtype=dt, device=dev)
-            Hin = torch.empty((nm * 4, nmax, nsrc), dtype=dt, device=dev)
-            # compressed positions are staged per position, then compressed by
-            # two batched products (R T) P and one R h
-            stage = {ch: (torch.empty((nm, plan.n[ch], plan.n[ch]), dtype=dt, device=dev),
-                          torch.empty((nm, plan.n[ch], nsrc), dtype=dt, device=dev))
-                     for ch in range(4) if plan.R[ch] is not None}
-            pk = _Packer(dt, dev)
-            for k, nd in enumerate(plist):
-                for ch, child in enumerate(nd.children):
-                    n_c = plan.n[ch]
-                    Tc, hc = child.data.T, child.data.h
-                    if ch in stage:
-                        pk.add(Tc, _addr(stage[ch][0], k * n_c * n_c), n_c, n_c, n_c, n_c)
-                        pk.add(hc, _addr(stage[ch][1], k * n_c * nsrc), n_c, nsrc, nsrc, nsrc)
-                    else:
-                        slot = 4 * k + ch
-                        pk.add(Tc, _addr(Tin, slot * nmax * nmax), n_c, n_c, n_c, nmax)
-                        pk.add(hc, _addr(Hin, slot * nmax * nsrc), n_c, nsrc, nsrc, nsrc)
-            pk.run()
-            for ch, (Ts, Hs) in stage.items():
-                n_c, m_c = plan.n[ch], plan.m[ch]
-                R, P = plan.ops(ch, dev, dt)
-                tmp = torch.empty((nm, m_c, n_c), dtype=dt, device=dev)
-                _gemm_strided(m_c, n_c, n_c, ptr(R), n_c, 0, ptr(Ts), n_c, n_c * n_c, ptr(tmp), n_c, m_c * n_c, nm,
-                              dt)
-                _gemm_strided(m_c, m_c, n_c, ptr(tmp), n_c, m_c * n_c, ptr(P), m_c, 0,
-                              ctypes.c_void_p(_addr(Tin, ch * nmax * nmax)), nmax, 4 * nmax * nmax, nm, dt)
-                _gemm_strided(m_c, nsrc, n_c, ptr(R), n_c, 0, ptr(Hs), nsrc, n_c * nsrc,
-                              ctypes.c_void_p(_addr(Hin, ch * nmax * nsrc)), nsrc, 4 * nmax * nsrc, nm, dt)
-            out = merge_level(Tin, Hin, plan)
-            _lib.raise_on_info(out.info, "adaptive merge")
-            for k, nd in enumerate(plist):
-                nd.data.T, nd.data.h = out.T[k], out.h[k]
-                nd.data.S, nd.data.g_tilde = out.S[k], out.gt[k]
-            levels.append((plan, plist, out))
-    pde_problem._adaptive_levels = levels
+    q = pde_problem.domain.q
+    adaptive_merge_stage(pde_problem, T_leaf, h_leaf, _signature, lambda sig: _Plan(sig[0], sig[1], q, L21, L12))
 
 
 def down_pass_adaptive_2D_DtN(pde_problem, boundary_data):
@@ -664,7 +720,11 @@ def down_pass_adaptive_2D_DtN(pde_problem, boundary_data):
         raise ValueError(f"boundary data has {g.shape[0]} points, the root boundary {dom.boundary_points.shape[0]}")
     root.data.g = g.contiguous()
     nsrc = g.shape[1]
+    by_level = OrderedDict()
     for plan, plist, out in reversed(pde_problem._adaptive_levels):
+        by_level.setdefault(plist[0].depth, []).append((plan, plist, out))
+
+    def group_down(plan, plist, out):
         nm, nmax, ne = len(plist), plan.nmax, plan.ne
         gt = out.gt
         if gt.shape[-1] != nsrc:
@@ -690,6 +750,9 @@ def down_pass_adaptive_2D_DtN(pde_problem, boundary_data):
                               nc * nmax * nsrc, ptr(gc), nsrc, n_c * nsrc, nm, dt)
                 for k, nd in enumerate(plist):
                     nd.children[ch].data.g = gc[k]
+
+    for items in by_level.values():
+        _run_level(dev, items, group_down)
     leaves = get_all_leaves(root)
     nb = leaves[0].data.g.shape[0]
     G = torch.empty((len(leaves), nb, nsrc), dtype=dt, device=dev)

@@ -19,17 +19,13 @@ face coordinate slowest.
 
 from __future__ import annotations
 
-import ctypes
-from collections import OrderedDict
 
 import numpy as np
 import torch
 
-from . import _lib
-from ._adaptive import _addr, _gemm_strided, _pack, _Packer
-from ._discretization_tree import DiscretizationNode3D, get_all_leaves, get_depth, get_nodes_at_level
+from ._adaptive import _pack
+from ._discretization_tree import DiscretizationNode3D, get_all_leaves
 from ._grid_creation import bounds_to_cheby_points_3D, bounds_to_gauss_face, rearrange_indices_ext_int_3D
-from ._lib import ptr
 from ._tables import _OCT_EXT, _OCT_IFACES
 from .quadrature import (
     barycentric_lagrange_interpolation_matrix_1D,
@@ -451,64 +447,11 @@ def local_solve_stage_adaptive_3D_DtN(pde_problem, device=None, host_device=None
 
 def merge_stage_adaptive_3D_DtN(pde_problem, T_leaf=None, h_leaf=None) -> None:
     """8:1 merges from the deepest level to the root (``merge/_adaptive_3D_DtN.py:30-149``)."""
-    from ._stages import merge_level
+    from ._adaptive import adaptive_merge_stage
 
-    dom = pde_problem.domain
-    root, q = dom.root, dom.q
-    if T_leaf is not None:
-        for i, lf in enumerate(get_all_leaves(root)):
-            lf.data.T = T_leaf[i]
-            lf.data.h = h_leaf[i] if h_leaf[i].dim() == 2 else h_leaf[i][:, None]
     L41, L14 = pde_problem.L_4f1, pde_problem.L_1f4
-    cache = pde_problem.__dict__.setdefault("_adaptive_plans", {})
-    levels = []
-    for level in range(get_depth(root), 0, -1):
-        parents = [nd for nd in get_nodes_at_level(root, level - 1) if len(nd.children)]
-        groups = OrderedDict()
-        for nd in parents:
-            groups.setdefault(_signature3(nd, q), []).append(nd)
-        for sig, plist in groups.items():
-            plan = cache.get(sig)
-            if plan is None:
-                plan = cache[sig] = _Plan3(sig[0], sig[1], q, L41, L14)
-            T0 = plist[0].children[0].data.T
-            dev, dt = T0.device, T0.dtype
-            nsrc = plist[0].children[0].data.h.shape[-1]
-            nm, nmax = len(plist), plan.nmax
-            Tin = torch.empty((nm * 8, nmax, nmax), dtype=dt, device=dev)
-            Hin = torch.empty((nm * 8, nmax, nsrc), dtype=dt, device=dev)
-            stage = {ch: (torch.empty((nm, plan.n[ch], plan.n[ch]), dtype=dt, device=dev),
-                          torch.empty((nm, plan.n[ch], nsrc), dtype=dt, device=dev))
-                     for ch in range(8) if plan.R[ch] is not None}
-            pk = _Packer(dt, dev)
-            for k, nd in enumerate(plist):
-                for ch, child in enumerate(nd.children):
-                    n_c = plan.n[ch]
-                    if ch in stage:
-                        pk.add(child.data.T, _addr(stage[ch][0], k * n_c * n_c), n_c, n_c, n_c, n_c)
-                        pk.add(child.data.h, _addr(stage[ch][1], k * n_c * nsrc), n_c, nsrc, nsrc, nsrc)
-                    else:
-                        slot = 8 * k + ch
-                        pk.add(child.data.T, _addr(Tin, slot * nmax * nmax), n_c, n_c, n_c, nmax)
-                        pk.add(child.data.h, _addr(Hin, slot * nmax * nsrc), n_c, nsrc, nsrc, nsrc)
-            pk.run()
-            for ch, (Ts, Hs) in stage.items():
-                n_c, m_c = plan.n[ch], plan.m[ch]
-                R, P = plan.ops(ch, dev, dt)
-                tmp = torch.empty((nm, m_c, n_c), dtype=dt, device=dev)
-                _gemm_strided(m_c, n_c, n_c, ptr(R), n_c, 0, ptr(Ts), n_c, n_c * n_c, ptr(tmp), n_c, m_c * n_c, nm,
-                              dt)
-                _gemm_strided(m_c, m_c, n_c, ptr(tmp), n_c, m_c * n_c, ptr(P), m_c, 0,
-                              ctypes.c_void_p(_addr(Tin, ch * nmax * nmax)), nmax, 8 * nmax * nmax, nm, dt)
-                _gemm_strided(m_c, nsrc, n_c, ptr(R), n_c, 0, ptr(Hs), nsrc, n_c * nsrc,
-                              ctypes.c_void_p(_addr(Hin, ch * nmax * nsrc)), nsrc, 8 * nmax * nsrc, nm, dt)
-            out = merge_level(Tin, Hin, plan)
-            _lib.raise_on_info(out.info, "adaptive merge")
-            for k, nd in enumerate(plist):
-                nd.data.T, nd.data.h = out.T[k], out.h[k]
-                nd.data.S, nd.data.g_tilde = out.S[k], out.gt[k]
-            levels.append((plan, plist, out))
-    pde_problem._adaptive_levels = levels
+    q = pde_problem.domain.q
+    adaptive_merge_stage(pde_problem, T_leaf, h_leaf, _signature3, lambda sig: _Plan3(sig[0], sig[1], q, L41, L14))
 
 
 def down_pass_adaptive_3D_DtN(pde_problem, boundary_data):
