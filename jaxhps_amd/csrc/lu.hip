@@ -1299,6 +1299,18 @@ __global__ void gather_rows_kernel(int n, int nrhs, const int* perm, const T* R,
 constexpr int LU_SMALL_MAX_N = 1024;
 constexpr int LU_SMALL_MAX_RHS = 16;
 
+// Sum over each aligned group of four lanes (every lane gets its group's sum).
+__device__ __forceinline__ double quad_lane_sum(double v) {
+  v += __shfl_xor(v, 1, 64);
+  return v + __shfl_xor(v, 2, 64);
+}
+__device__ __forceinline__ float quad_lane_sum(float v) {
+  v += __shfl_xor(v, 1, 64);
+  return v + __shfl_xor(v, 2, 64);
+}
+__device__ __forceinline__ c128 quad_lane_sum(c128 v) { return c128{quad_lane_sum(v.re), quad_lane_sum(v.im)}; }
+__device__ __forceinline__ c64 quad_lane_sum(c64 v) { return c64{quad_lane_sum(v.re), quad_lane_sum(v.im)}; }
+
 template <typename T, bool CT>
 __global__ __launch_bounds__(256) void getrs_small_kernel(int n, int nrhs, const T* LU, long lda, long sA,
                                                           const int* perm, const T* dinv, const T* R, long ldr,
@@ -1363,6 +1375,34 @@ __global__ __launch_bounds__(256) void getrs_small_kernel(int n, int nrhs, const
       // rows outside the block: below (pass 0) or above (pass 1)
       const int r0 = pass == 0 ? k + nb : 0;
       const int r1 = pass == 0 ? n : k;
+      if (!CT && nrhs <= 2) {
+        // four lanes per row, eight block columns each: a row's block segment
+        // is read contiguously by its lanes and their products summed across
+        // them (the row-per-thread loop below strides lda between lanes)
+        const int sub = tid & 3;
+        const int row0 = r0 + (tid >> 2);
+        const int iters = (r1 - r0 + 63) / 64;  // uniform across the workgroup
+        for (int it = 0; it < iters; ++it) {
+          const int i = row0 + 64 * it;
+          const bool live = i < r1;
+          T a[8];
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const int q = 8 * sub + j;
+            a[j] = live && q < nb ? A[(long)i * lda + k + q] : s_zero<T>();
+          }
+          for (int c = 0; c < nrhs; ++c) {
+            T prod = s_zero<T>();
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+              const int q = 8 * sub + j;
+              if (q < nb) prod = s_add(prod, s_mul(a[j], xs[(k + q) * nrhs + c]));
+            }
+            prod = quad_lane_sum(prod);
+            if (live && sub == 0) xs[i * nrhs + c] = s_sub(xs[i * nrhs + c], prod);
+          }
+        }
+      } else
       for (int i = r0 + tid; i < r1; i += 256) {
         for (int c = 0; c < nrhs; ++c) {
           T acc = xs[i * nrhs + c];
