@@ -1311,6 +1311,40 @@ __device__ __forceinline__ float quad_lane_sum(float v) {
 __device__ __forceinline__ c128 quad_lane_sum(c128 v) { return c128{quad_lane_sum(v.re), quad_lane_sum(v.im)}; }
 __device__ __forceinline__ c64 quad_lane_sum(c64 v) { return c64{quad_lane_sum(v.re), quad_lane_sum(v.im)}; }
 
+// xs[i][c] -= sum_q Ab[i * lda + q] xs[kx + q][c] for rows r0 <= i < r1 and
+// the nb <= 32 block columns q, by a 256-thread workgroup: four lanes per
+// row, eight block columns each, so a row's block segment is read
+// contiguously by its lanes and their products summed across them (a
+// row-per-thread loop strides lda between lanes).  The block rows of xs are
+// only read.  Every thread of the workgroup must call it.
+template <typename T>
+__device__ __forceinline__ void offblock_rows(const T* Ab, long lda, int r0, int r1, int nb, int nrhs, T* xs, int kx,
+                                              int tid) {
+  const int sub = tid & 3;
+  const int row0 = r0 + (tid >> 2);
+  const int iters = (r1 - r0 + 63) / 64;  // uniform across the workgroup
+  for (int it = 0; it < iters; ++it) {
+    const int i = row0 + 64 * it;
+    const bool live = i < r1;
+    T a[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int q = 8 * sub + j;
+      a[j] = live && q < nb ? Ab[(long)i * lda + q] : s_zero<T>();
+    }
+    for (int c = 0; c < nrhs; ++c) {
+      T prod = s_zero<T>();
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int q = 8 * sub + j;
+        if (q < nb) prod = s_add(prod, s_mul(a[j], xs[(kx + q) * nrhs + c]));
+      }
+      prod = quad_lane_sum(prod);
+      if (live && sub == 0) xs[i * nrhs + c] = s_sub(xs[i * nrhs + c], prod);
+    }
+  }
+}
+
 template <typename T, bool CT>
 __global__ __launch_bounds__(256) void getrs_small_kernel(int n, int nrhs, const T* LU, long lda, long sA,
                                                           const int* perm, const T* dinv, const T* R, long ldr,
@@ -1375,34 +1409,9 @@ __global__ __launch_bounds__(256) void getrs_small_kernel(int n, int nrhs, const
       // rows outside the block: below (pass 0) or above (pass 1)
       const int r0 = pass == 0 ? k + nb : 0;
       const int r1 = pass == 0 ? n : k;
-      if (!CT && nrhs <= 2) {
-        // four lanes per row, eight block columns each: a row's block segment
-        // is read contiguously by its lanes and their products summed across
-        // them (the row-per-thread loop below strides lda between lanes)
-        const int sub = tid & 3;
-        const int row0 = r0 + (tid >> 2);
-        const int iters = (r1 - r0 + 63) / 64;  // uniform across the workgroup
-        for (int it = 0; it < iters; ++it) {
-          const int i = row0 + 64 * it;
-          const bool live = i < r1;
-          T a[8];
-#pragma unroll
-          for (int j = 0; j < 8; ++j) {
-            const int q = 8 * sub + j;
-            a[j] = live && q < nb ? A[(long)i * lda + k + q] : s_zero<T>();
-          }
-          for (int c = 0; c < nrhs; ++c) {
-            T prod = s_zero<T>();
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-              const int q = 8 * sub + j;
-              if (q < nb) prod = s_add(prod, s_mul(a[j], xs[(k + q) * nrhs + c]));
-            }
-            prod = quad_lane_sum(prod);
-            if (live && sub == 0) xs[i * nrhs + c] = s_sub(xs[i * nrhs + c], prod);
-          }
-        }
-      } else
+      if (!CT && nrhs <= 2)
+        offblock_rows(A + k, lda, r0, r1, nb, nrhs, xs, k, tid);
+      else
       for (int i = r0 + tid; i < r1; i += 256) {
         for (int c = 0; c < nrhs; ++c) {
           T acc = xs[i * nrhs + c];
@@ -1483,6 +1492,9 @@ __global__ __launch_bounds__(256) void getrs_block_kernel(int n, int nrhs, const
     __syncthreads();
     const int r0 = UPPER ? 0 : kl + nb;
     const int r1 = UPPER ? kl : nbk;
+    if (nrhs <= 4)
+      offblock_rows(A + (long)k0 * lda + k0 + kl, lda, r0, r1, nb, nrhs, xs, kl, tid);
+    else
     for (int i = r0 + tid; i < r1; i += 256) {
       for (int c = 0; c < nrhs; ++c) {
         T acc = xs[i * nrhs + c];
