@@ -319,12 +319,15 @@ __device__ __forceinline__ void finish_panel_tab(T* D, int ldd, T* Out, int ldo,
     }
     __syncthreads();
   }
-  invert_diag_block<T>(D, ldd, Out, ldo, nb);
-  const int cnt = tab[0] < 2 * LU_NB ? tab[0] : 2 * LU_NB;
-  for (int i = threadIdx.x; i < 1 + 2 * cnt; i += blockDim.x) swtab[(long)b * SWTAB + i] = i == 0 ? cnt : tab[i];
-  __syncthreads();
   const int nblk = (n + LU_NB - 1) / LU_NB;
   T* db = dinv + ((long)b * nblk + k / LU_NB) * LU_NB * LU_NB;
+  // with D zero outside nb x nb the inverse is zero there too, so without
+  // LDS scratch (Out == nullptr) it is written straight to dinv
+  invert_diag_block<T>(D, ldd, Out ? Out : db, Out ? ldo : LU_NB, nb);
+  const int cnt = tab[0] < 2 * LU_NB ? tab[0] : 2 * LU_NB;
+  for (int i = threadIdx.x; i < 1 + 2 * cnt; i += blockDim.x) swtab[(long)b * SWTAB + i] = i == 0 ? cnt : tab[i];
+  if (!Out) return;
+  __syncthreads();
   for (int idx = threadIdx.x; idx < LU_NB * LU_NB; idx += blockDim.x) {
     int r = idx / LU_NB, c = idx - r * LU_NB;
     db[idx] = (r < nb && c < nb) ? Out[r * ldo + c] : s_zero<T>();
@@ -710,7 +713,6 @@ __global__ __launch_bounds__(THREADS) void getrf_panel_rows_kernel(int n, T* A, 
   using R = typename real_of<T>::type;
   __shared__ PanelLDS<T, THREADS / 64> sh;
   __shared__ T Dblk[LU_NB][LU_NB + 1];
-  __shared__ T Dout[LU_NB][LU_NB + 1];
   const int b = blockIdx.x;
   const int t = threadIdx.x;
   const int m = n - k;
@@ -754,7 +756,7 @@ __global__ __launch_bounds__(THREADS) void getrf_panel_rows_kernel(int n, T* A, 
 #pragma unroll
       for (int c = 0; c < LU_NB; ++c) Dblk[pos[r]][c] = c < nb ? x[r][c] : s_zero<T>();
   }
-  finish_panel_tab<T>(&Dblk[0][0], LU_NB + 1, &Dout[0][0], LU_NB + 1, k, nb, n, b, tab, dinv, swtab);
+  finish_panel_tab<T>(&Dblk[0][0], LU_NB + 1, nullptr, 0, k, nb, n, b, tab, dinv, swtab);
   // the panel's row permutation on every other column of the matrix, CW
   // columns at a time: all moved rows are read before any is written
   constexpr int CW = THREADS >= 128 ? 32 : 16, RG = THREADS / CW, NV = 2 * LU_NB / RG;
