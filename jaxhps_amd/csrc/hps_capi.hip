@@ -81,26 +81,36 @@ hipError_t add2d(int rows, int cols, double alpha, const T* src, long lds, long 
 // leaf operator assembly: out[l][r][c] = sum_i coef_i[l][row0+r] * D_{op_i}[row0+r][col0+c]
 // op index < 0 means the identity operator.
 // ---------------------------------------------------------------------------
+// Each thread assembles its entry for ASM_LEAVES consecutive leaves, so the
+// operator entries it reads (shared by all leaves) come from cache once per
+// group instead of once per leaf.
+constexpr int ASM_LEAVES = 4;
+
 template <typename T>
 __global__ __launch_bounds__(256) void assemble_kernel(int n, int nl, int ncoef, CoefOps ops_idx, const T* coeffs,
                                                        const T* diffops, int row0, int nrows, int col0, int ncols,
                                                        T* out, long ldo, long so) {
-  const int l = blockIdx.y;
   long total = (long)nrows * ncols;
+  for (int l0 = blockIdx.y * ASM_LEAVES; l0 < nl; l0 += gridDim.y * ASM_LEAVES) {
+  const int nlg = nl - l0 < ASM_LEAVES ? nl - l0 : ASM_LEAVES;
   for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
     int r = (int)(idx / ncols), c = (int)(idx - (long)r * ncols);
     int gr = row0 + r, gc = col0 + c;
-    T acc = s_zero<T>();
+    T acc[ASM_LEAVES];
+#pragma unroll
+    for (int j = 0; j < ASM_LEAVES; ++j) acc[j] = s_zero<T>();
     for (int i = 0; i < ncoef; ++i) {
-      T cf = coeffs[((long)i * nl + l) * n + gr];
-      int op = ops_idx.op[i];
-      if (op < 0) {
-        if (gr == gc) acc = s_add(acc, cf);
-      } else {
-        acc = s_add(acc, s_mul(cf, diffops[((long)op * n + gr) * n + gc]));
-      }
+      const int op = ops_idx.op[i];
+      const T d = op < 0 ? (gr == gc ? s_one<T>() : s_zero<T>()) : diffops[((long)op * n + gr) * n + gc];
+      const T* cfp = coeffs + ((long)i * nl + l0) * n + gr;
+#pragma unroll
+      for (int j = 0; j < ASM_LEAVES; ++j)
+        if (j < nlg) acc[j] = s_add(acc[j], s_mul(cfp[(long)j * n], d));
     }
-    out[(long)l * so + (long)r * ldo + c] = acc;
+#pragma unroll
+    for (int j = 0; j < ASM_LEAVES; ++j)
+      if (j < nlg) out[(long)(l0 + j) * so + (long)r * ldo + c] = acc[j];
+  }
   }
 }
 
@@ -330,7 +340,7 @@ static int leaf_t(hps_handle_t h, int kind, int nl, int n, int nb, int nq, int n
     HPS_PROF("leaf_assemble", T, flop_mult<T>() * 2.0 * ncoef * (double)an * n * nl,
              sizeof(T) * ((double)an * n * nl + (double)ncoef * n * nl));
     dim3 g = grid_for((long)an * n, 1);
-    g.y = nl;
+    g.y = ceil_div(nl, ASM_LEAVES) < 65535 ? ceil_div(nl, ASM_LEAVES) : 65535;
     hipLaunchKernelGGL((assemble_kernel<T>), g, dim3(256), 0, s, n, nl, ncoef, co, (const T*)coeffs,
                        (const T*)diffops, arow0, an, 0, n, aout, ld_lu, s_lu);
     HPS_CHECK_HIP(h, hipGetLastError());
