@@ -253,13 +253,6 @@ int hps_coef_gradient(hps_handle_t h, int dtype, int nl, int n, int row0, int ns
  * (local_solve/_adaptive_2D_DtN.py:130-172). */
 int hps_scale_batched(hps_handle_t h, int dtype, int batch, long n, long stride, const double* scale, void* X);
 
-/* dst[b][r][c] = alpha * src[b][r][c] (src == NULL writes zeros): packs
- * children of different sizes into the padded blocks a merge level reads and
- * unpacks the down pass's results; replaces the reference's per-node
- * slicing and concatenation (merge/_adaptive_2D_DtN.py:110-206). */
-int hps_copy2d_batched(hps_handle_t h, int dtype, int rows, int cols, double alpha, const void* src, long lds,
-                       long ss, void* dst, long ldd, long sd, int batch);
-
 /* count block copies in one launch: desc (device, count x 6 int64) =
  * (src byte address, dst byte address, rows, cols, lds, ldd) per block, in
  * elements of dtype; max_elems >= rows * cols of every block (sizes the

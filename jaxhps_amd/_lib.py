@@ -75,7 +75,6 @@ SIGNATURES = {
     "hps_plane_wave": (_I, [_P, _L, _I, _P, _P, _D, _L, _P, _P]),
     "hps_scale_batched": (_I, [_P, _I, _I, _L, _L, _P, _P]),
     "hps_pack_blocks": (_I, [_P, _I, _I, _L, _P]),
-    "hps_copy2d_batched": (_I, [_P, _I, _I, _I, _D, _P, _L, _L, _P, _L, _L, _I]),
     "hps_iti_to_dtn_workspace": (_S, [_I]),
     "hps_iti_to_dtn": (_I, [_P, _I, _I, _P, _L, _D, _P, _L, _P, _P, _S]),
     "hps_scattering_workspace": (_S, [_I, _I]),

@@ -1,7 +1,7 @@
 // Small batched kernels for adaptive (non-uniform) trees, where leaves and
 // merges come in different sizes: per-leaf scaling of coefficient fields and
-// maps by powers of the leaf's half side, and strided copies that pack
-// children of different sizes into the padded blocks a merge level reads.
+// maps by powers of the leaf's half side, and one-launch block copies that
+// pack children of different sizes into the padded blocks a merge level reads.
 #include <string>
 
 #include "hps_capi_internal.h"
@@ -28,13 +28,6 @@ static int scale_t(hps_handle_t h, int batch, long n, long stride, const double*
   hipLaunchKernelGGL((scale_batched_kernel<T>), grid_for((long)batch * n, 1), dim3(256), 0, s, batch, n, stride,
                      scale, (T*)X);
   HPS_CHECK_HIP(h, hipGetLastError());
-  return HPS_OK;
-}
-
-template <typename T>
-static int copy_t(hps_handle_t h, int rows, int cols, double alpha, const void* src, long lds, long ss, void* dst,
-                  long ldd, long sd, int batch) {
-  HPS_CHECK_HIP(h, copy2d<T>(rows, cols, alpha, (const T*)src, lds, ss, (T*)dst, ldd, sd, batch, h->stream));
   return HPS_OK;
 }
 
@@ -87,14 +80,4 @@ int hps_scale_batched(hps_handle_t h, int dtype, int batch, long n, long stride,
   if ((long)batch * n == 0) return HPS_OK;
   HPS_ARG(h, scale && X, "null pointer");
   DISPATCH(dtype, scale_t, h, batch, n, stride, scale, X);
-}
-
-int hps_copy2d_batched(hps_handle_t h, int dtype, int rows, int cols, double alpha, const void* src, long lds,
-                       long ss, void* dst, long ldd, long sd, int batch) {
-  if (!h) return HPS_ERR_ARG;
-  HPS_ARG(h, rows >= 0 && cols >= 0 && batch >= 0 && lds >= cols && ldd >= cols, "bad size");
-  if ((long)rows * cols * batch == 0) return HPS_OK;
-  HPS_ARG(h, dst, "null pointer");
-  HPS_ARG(h, src != dst, "copy is out of place");
-  DISPATCH(dtype, copy_t, h, rows, cols, alpha, src, lds, ss, dst, ldd, sd, batch);
 }
