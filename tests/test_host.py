@@ -247,3 +247,33 @@ def test_product_fails_loudly_without_gpu():
     pde = jh.PDEProblem(dom, source=ones, D_xx_coefficients=ones, D_yy_coefficients=ones)
     with pytest.raises(jh.HPSLibraryError):
         jh.build_solver(pde)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus above 1 exits with a message."""
+    import subprocess
+    import sys
+
+    root = Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
+
+
+def test_scattering_and_interp_fail_loudly_without_gpu():
+    """The exterior coupling and interpolation have no CPU path either."""
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("needs a host without a GPU")
+    import jaxhps_amd as jh
+    from jaxhps_amd import HPSLibraryError
+    from jaxhps_amd.scattering import get_DtN_from_ItI
+
+    with pytest.raises((HPSLibraryError, RuntimeError)):
+        get_DtN_from_ItI(np.eye(8, dtype=complex) * 0.5, 3.0)
+    dom = jh.Domain(p=4, q=2, root=jh.DiscretizationNode2D(0.0, 1.0, 0.0, 1.0), L=1)
+    with pytest.raises((HPSLibraryError, RuntimeError)):
+        dom.interp_from_interior_points(np.zeros(dom.interior_points.shape[:2]), np.linspace(0, 1, 3),
+                                        np.linspace(0, 1, 3))
