@@ -41,6 +41,22 @@ constexpr int GBK = 16;
 // square layout; WR = 4 stacks the waves along M for N <= 16 (the
 // multi-right-hand-side products).  The next K-tile is fetched from global
 // memory into registers while the current one is multiplied out of LDS.
+// Workgroups are handed to the 8 XCDs round-robin by linear id, and each
+// XCD has its own L2.  For a batch (grid y >= 2) xcd_block gives the
+// workgroup the (x, y) block of the linear id (id % 8) * (W / 8) + id / 8
+// instead (W = grid size, a multiple of 8; otherwise the identity): each XCD
+// then runs a contiguous run of tiles, so the tiles of one matrix (and their
+// shared A row and B column panels) stay in one L2.  A single product keeps
+// the round-robin order, which measured faster for it.  The map is a
+// permutation of the grid.
+__device__ __forceinline__ void xcd_block(int& bx, int& by) {
+  const unsigned long gx = gridDim.x, w = gx * gridDim.y;
+  unsigned long id = blockIdx.x + gx * blockIdx.y;
+  if (gridDim.y >= 2 && w % 8 == 0) id = (id % 8) * (w / 8) + id / 8;
+  bx = (int)(id % gx);
+  by = (int)(id / gx);
+}
+
 template <typename T> __device__ __forceinline__ T s_conj(T x) { return x; }
 template <> __device__ __forceinline__ c128 s_conj<c128>(c128 x) { return c128{x.re, -x.im}; }
 template <> __device__ __forceinline__ c64 s_conj<c64>(c64 x) { return c64{x.re, -x.im}; }
@@ -70,11 +86,13 @@ __global__ __launch_bounds__(256, MINB) void gemm_mfma_kernel(GemmArgs<T> p, int
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wr = wave / WC, wc = wave % WC;
-  const int tm = blockIdx.x / tiles_n, tn = blockIdx.x % tiles_n;
+  int bx, by;
+  xcd_block(bx, by);
+  const int tm = bx / tiles_n, tn = bx % tiles_n;
   const int m0 = tm * BM, n0 = tn * BN;
   const int li = lane & 15, lk = lane >> 4;
 
-  for (int b = blockIdx.y; b < p.batch; b += gridDim.y) {
+  for (int b = by; b < p.batch; b += gridDim.y) {
     const T* A = p.A + (long)b * p.sA;
     const T* B = p.B + (long)b * p.sB;
     static_assert(!K3M || CPLX, "three-product form is for complex types");
@@ -224,10 +242,12 @@ __global__ __launch_bounds__(256, 2) void gemm_k32_kernel(GemmArgs<T> p, int til
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int wr = wave >> 1, wc = wave & 1;
-  const int tm = blockIdx.x / tiles_n, tn = blockIdx.x % tiles_n;
+  int bx, by;
+  xcd_block(bx, by);
+  const int tm = bx / tiles_n, tn = bx % tiles_n;
   const int m0 = tm * BM, n0 = tn * BN;
   const int li = lane & 15, lk = lane >> 4;
-  for (int b = blockIdx.y; b < p.batch; b += gridDim.y) {
+  for (int b = by; b < p.batch; b += gridDim.y) {
     const T* A = p.A + (long)b * p.sA;
     const T* B = p.B + (long)b * p.sB;
     const T* C = p.C ? p.C + (long)b * p.sC : nullptr;
