@@ -67,6 +67,24 @@ __host__ __device__ inline c64 s_mul(c64 a, c64 b) {
 template <typename T> __host__ __device__ inline T s_fnms(T a, T b, T c) { return s_sub(a, s_mul(b, c)); }
 // a*b + c
 template <typename T> __host__ __device__ inline T s_fma(T a, T b, T c) { return s_add(s_mul(a, b), c); }
+// complex forms as four fused multiply-adds each (the generic form costs a
+// separate multiply and add per component)
+template <typename C, typename R>
+__host__ __device__ inline C c_fnms(C a, C b, C c) {
+  const R re = fma(-b.re, c.re, fma(b.im, c.im, a.re));
+  const R im = fma(-b.re, c.im, fma(-b.im, c.re, a.im));
+  return C{re, im};
+}
+template <typename C, typename R>
+__host__ __device__ inline C c_fma(C a, C b, C c) {
+  const R re = fma(a.re, b.re, fma(-a.im, b.im, c.re));
+  const R im = fma(a.re, b.im, fma(a.im, b.re, c.im));
+  return C{re, im};
+}
+template <> __host__ __device__ inline c128 s_fnms<c128>(c128 a, c128 b, c128 c) { return c_fnms<c128, double>(a, b, c); }
+template <> __host__ __device__ inline c64 s_fnms<c64>(c64 a, c64 b, c64 c) { return c_fnms<c64, float>(a, b, c); }
+template <> __host__ __device__ inline c128 s_fma<c128>(c128 a, c128 b, c128 c) { return c_fma<c128, double>(a, b, c); }
+template <> __host__ __device__ inline c64 s_fma<c64>(c64 a, c64 b, c64 c) { return c_fma<c64, float>(a, b, c); }
 
 __host__ __device__ inline double s_scale(double a, double s) { return a * s; }
 __host__ __device__ inline float s_scale(float a, double s) { return a * (float)s; }
