@@ -67,12 +67,13 @@ __global__ void iota_kernel(int* perm, int n, long total) {
   if (i < total) perm[i] = (int)(i % n);
 }
 
+// selects instead of a branch: inside the wave reductions a branch costs a
+// dozen exec-mask instructions per merge
 template <typename R>
 __device__ __forceinline__ void argmax_merge(R& bv, int& bi, R v, int i) {
-  if (v > bv || (v == bv && i < bi)) {
-    bv = v;
-    bi = i;
-  }
+  const bool take = v > bv || (v == bv && i < bi);
+  bv = take ? v : bv;
+  bi = take ? i : bi;
 }
 
 // ---------------------------------------------------------------------------
@@ -655,17 +656,18 @@ __device__ __forceinline__ void panel_step(T (&x)[RPT][LU_NB], int (&pos)[RPT], 
     sh.cp[par][wave] = bp;
   }
   __syncthreads();
+  // selects, not branches: every candidate's read is unconditional, so the
+  // reads are issued together
   R fv = sh.cv[par][0];
   int fp = sh.cp[par][0], fw = 0;
 #pragma unroll
   for (int w = 1; w < NW; ++w) {
     const R v = sh.cv[par][w];
     const int q = sh.cp[par][w];
-    if (v > fv || (v == fv && q < fp)) {
-      fv = v;
-      fp = q;
-      fw = w;
-    }
+    const bool take = v > fv || (v == fv && q < fp);
+    fv = take ? v : fv;
+    fp = take ? q : fp;
+    fw = take ? w : fw;
   }
   // without a candidate the row at position J stays; the column's updates
   // are then NaN whatever the pivot slot holds
