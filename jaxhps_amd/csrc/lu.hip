@@ -893,8 +893,8 @@ __global__ __launch_bounds__(256) void tslu_leaf_kernel(int n, const T* A, long 
 // block), moves the displaced top rows, and writes dinv and the swap table.
 constexpr int TS_GROUP = 16;
 
-template <typename T, bool LAST>
-__global__ __launch_bounds__(512) void tslu_reduce_kernel(int n, T* A, long lda, long sA, int k, int nb, int nsets,
+template <typename T, bool LAST, int THREADS = 512>
+__global__ __launch_bounds__(THREADS) void tslu_reduce_kernel(int n, T* A, long lda, long sA, int k, int nb, int nsets,
                                                           const int* cand_in, int ngroups, int* cand_out, int* ipiv,
                                                           int* info, T* dinv, int* swtab) {
   __shared__ int out[LU_NB];
@@ -907,7 +907,7 @@ __global__ __launch_bounds__(512) void tslu_reduce_kernel(int n, T* A, long lda,
   int pos[1];
 #pragma unroll
   for (int q = 0; q < LU_NB; ++q) x[0][q] = (r >= 0 && q < nb) ? Ab[(long)r * lda + k + q] : s_zero<T>();
-  tslu_select<T, 512>(x, pos, r, nb, out, LAST ? info : nullptr, b, k);
+  tslu_select<T, THREADS>(x, pos, r, nb, out, LAST ? info : nullptr, b, k);
   if constexpr (!LAST) {
     if (t < nb) cand_out[((long)b * ngroups + g) * LU_NB + t] = out[t];
   } else {
@@ -959,12 +959,12 @@ __global__ __launch_bounds__(512) void tslu_reduce_kernel(int n, T* A, long lda,
     const int cnt = tab[0];
     // rows displaced below the top block come from original top rows: read
     // their panel columns before the winners overwrite the top rows
-    for (int idx = t; idx < cnt * LU_NB; idx += 512) {
+    for (int idx = t; idx < cnt * LU_NB; idx += THREADS) {
       const int a = idx / LU_NB, c = idx - a * LU_NB;
       if (tab[1 + 2 * a] >= k + nb && c < nb) vals[a][c] = Ab[(long)tab[2 + 2 * a] * lda + k + c];
     }
     __syncthreads();
-    for (int idx = t; idx < cnt * LU_NB; idx += 512) {
+    for (int idx = t; idx < cnt * LU_NB; idx += THREADS) {
       const int a = idx / LU_NB, c = idx - a * LU_NB;
       if (tab[1 + 2 * a] >= k + nb && c < nb) Ab[(long)tab[1 + 2 * a] * lda + k + c] = vals[a][c];
     }
@@ -1020,8 +1020,13 @@ static hipError_t launch_tslu_panel(int n, T* A, long lda, long sA, int k, int n
   for (;;) {
     const int ngroups = (nsets + TS_GROUP - 1) / TS_GROUP;
     if (ngroups == 1) {
-      hipLaunchKernelGGL((tslu_reduce_kernel<T, true>), dim3(1, batch), dim3(512), 0, s, n, A, lda, sA, k, nb, nsets,
-                         cur, 1, nullptr, ipiv, info, dinv, swtab);
+      // up to 8 candidate sets fit 256 threads: one wave per SIMD instead of two
+      if (nsets * LU_NB <= 256)
+        hipLaunchKernelGGL((tslu_reduce_kernel<T, true, 256>), dim3(1, batch), dim3(256), 0, s, n, A, lda, sA, k, nb,
+                           nsets, cur, 1, nullptr, ipiv, info, dinv, swtab);
+      else
+        hipLaunchKernelGGL((tslu_reduce_kernel<T, true>), dim3(1, batch), dim3(512), 0, s, n, A, lda, sA, k, nb, nsets,
+                           cur, 1, nullptr, ipiv, info, dinv, swtab);
       break;
     }
     hipLaunchKernelGGL((tslu_reduce_kernel<T, false>), dim3(ngroups, batch), dim3(512), 0, s, n, A, lda, sA, k, nb,
