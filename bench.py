@@ -69,63 +69,98 @@ def make_workload(cfg, L=None):
     return dom, kw, g, desc
 
 
+def _oracle_setup_2d(dom, kw):
+    import numpy as np
+
+    from oracle import hps2d
+    from oracle import spectral as osp
+
+    names = {"D_xx_coefficients": "D_xx", "D_yy_coefficients": "D_yy", "D_x_coefficients": "D_x",
+             "D_y_coefficients": "D_y", "I_coefficients": "I"}
+    iti = kw.get("use_ItI", False)
+    ops = osp.leaf_ops_2D(dom.p, dom.q, 2.0 / 2 ** (dom.L + 1), eta=kw.get("eta") if iti else None)
+    coeffs = {names[k]: np.asarray(v) for k, v in kw.items() if k in names}
+    src = np.asarray(kw["source"])
+    src = src if src.ndim == 3 else src[..., None]
+    return ops, coeffs, (src.astype(complex) if iti else src), iti, hps2d
+
+
 def cpu_baseline(cfg):
-    """The FP64 oracle (numpy/LAPACK on the host cores) on a bounded sample of
-    the same workload: the same p and coefficients at L = 4 (256 leaves; 3D:
-    L = 2, 64 leaves)."""
+    """The FP64 oracle (numpy/LAPACK on the host cores) on the workload.
+
+    Config B (L = 6) runs whole.  Config C (L = 7, 16384 leaves) is timed by
+    parts at full size and the parts are summed: the leaf stage on 512 of
+    the leaves (x 32); at every merge level one 4:1 merge of the level's real
+    size (children's maps are synthetic at levels >= 2: a merge's cost does
+    not depend on the values) x the level's merge count; the down pass the
+    same way; the leaf apply on the sampled leaves (x 16).  Config D (3D)
+    runs whole at L = 2."""
     import numpy as np
 
     from jaxhps_amd import workloads as wl
-    from oracle import hps2d, hps3d
-    from oracle import spectral as osp
 
-    L = 4
     if cfg == "D":
         return _cpu_baseline_3d()
-    if cfg == "C":
-        dom, kw, g = wl.config_C(L=L)
-        kind = "iti"
-    else:
-        dom, kw, g = wl.config_B(L=L)
-        kind = "dtn"
-    p, q = dom.p, dom.q
-    half = 2.0 / 2 ** (L + 1)
-    ops = osp.leaf_ops_2D(p, q, half, eta=kw.get("eta"))
-    names = {"D_xx_coefficients": "D_xx", "D_yy_coefficients": "D_yy", "D_x_coefficients": "D_x",
-             "D_y_coefficients": "D_y", "I_coefficients": "I"}
-    coeffs = {names[k]: v for k, v in kw.items() if k in names}
-    src = kw["source"]
-    src = src if src.ndim == 3 else src[..., None]
-    gg = g if g.ndim == 2 else g[:, None]
+    cores = len(os.sched_getaffinity(0))
+    if cfg == "B":
+        dom, kw, g = wl.config_B(L=6)
+        ops, coeffs, src, _, hps2d = _oracle_setup_2d(dom, kw)
 
-    def run():
-        A = hps2d.assemble_diff_operator(coeffs, ops)
-        if kind == "dtn":
+        def run():
+            A = hps2d.assemble_diff_operator(coeffs, ops)
             Y, T, v, h = hps2d.leaf_dtn(A, src, ops["P"], ops["Q"])
-        else:
-            T, Y, h, v = hps2d.leaf_iti(A, src.astype(complex), ops["P_iti"], ops["QH"], ops["G"])
-        S, gl, _, _ = hps2d.merge_stage(T, h, L, kind)
-        u, _ = hps2d.down_pass(gg, S, gl, Y, v, kind)
-        return u
+            S, gl, _, _ = hps2d.merge_stage(T, h, dom.L, "dtn")
+            return hps2d.down_pass(g, S, gl, Y, v, "dtn")[0]
 
-    run()
-    times = []
-    t_end = time.time() + 20.0
-    while len(times) < 3 and (time.time() < t_end or not times):
         t0 = time.perf_counter()
         run()
-        times.append(time.perf_counter() - t0)
-    t = sorted(times)[len(times) // 2]
-    dof = dom.n_leaves * p * p
-    cores = len(os.sched_getaffinity(0))
-    return {
-        "value": dof / t,
-        "unit": "DOF/s",
-        "cores": cores,
-        "kind": "port",
-        "sample": f"FP64 numpy oracle, full build+solve at L={L} ({dom.n_leaves} leaves, {dof} DOF), "
-        f"median of {len(times)} runs, BLAS threads <= {cores}",
-    }
+        t = time.perf_counter() - t0
+        dof = dom.n_leaves * dom.p**2
+        return {"value": dof / t, "unit": "DOF/s", "cores": cores, "kind": "port",
+                "sample": f"FP64 numpy oracle, the whole build+solve of the workload (L=6, {dom.n_leaves} leaves, "
+                          f"{dof} DOF, 16 RHS), one run, BLAS threads <= {cores}"}
+    dom, kw, g = wl.config_C(L=7)
+    ops, coeffs, src, _, hps2d = _oracle_setup_2d(dom, kw)
+    nl, L, nq = dom.n_leaves, dom.L, 4 * dom.q
+    ns = 512
+    parts = {}
+    t0 = time.perf_counter()
+    A = hps2d.assemble_diff_operator({k: v[:ns] for k, v in coeffs.items()}, ops)
+    T, Y, h, v = hps2d.leaf_iti(A, src[:ns], ops["P_iti"], ops["QH"], ops["G"])
+    parts["leaf"] = (time.perf_counter() - t0) * nl / ns
+    rng = np.random.default_rng(0)
+    down = 0.0
+    for lvl in range(1, L + 1):
+        ne = nq * 2 ** (lvl - 1)
+        if lvl == 1:
+            kids = [T[i] for i in range(4)]
+            hs = [h[i] for i in range(4)]
+        else:
+            kids = [(rng.standard_normal((ne, ne)) + 1j * rng.standard_normal((ne, ne))) * (0.5 / np.sqrt(ne))
+                    for _ in range(4)]
+            hs = [rng.standard_normal((ne, 1)) + 0j for _ in range(4)]
+        reps = max(1, min(64, 2 ** (2 * (L - lvl)) // 64))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            S, Tn, hn, gt = hps2d.quad_merge_iti(*kids, *hs)
+        parts[f"merge_level_{lvl}"] = (time.perf_counter() - t0) / reps * nl / 4**lvl
+        gpar = rng.standard_normal((2 * ne, 1)) + 0j
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            hps2d.propagate_down_iti(S, gpar, gt)
+        down += (time.perf_counter() - t0) / reps * nl / 4**lvl
+        del kids, S, Tn
+    gl = rng.standard_normal((ns, nq, 1)) + 0j
+    t0 = time.perf_counter()
+    _ = Y @ gl + v
+    parts["down_pass"] = down + (time.perf_counter() - t0) * nl / ns
+    total = sum(parts.values())
+    dof = nl * dom.p**2
+    return {"value": dof / total, "unit": "DOF/s", "cores": cores, "kind": "port",
+            "seconds_extrapolated": total, "parts_s": {k: round(v, 3) for k, v in parts.items()},
+            "sample": f"FP64 numpy oracle timed by parts at the workload's full size (L=7, {nl} leaves, {dof} DOF): "
+                      f"leaf stage on {ns} leaves x {nl // ns}; one 4:1 merge and one down-pass step per level at "
+                      "its real size x the level's merge count; BLAS threads <= " + str(cores)}
 
 
 def measured_traffic(kernel):

@@ -185,22 +185,33 @@ def dtype_code(dt) -> int:
 def gemm(A, B, C=None, alpha=1.0, beta=0.0, out=None):
     """Batched D = alpha A @ B + beta C on 3D contiguous tensors (batch, rows, cols).
     A 2D operand broadcasts over the batch."""
-    lib = load_library()
-    h = handle()
     a3 = A if A.dim() == 3 else A.unsqueeze(0)
     b3 = B if B.dim() == 3 else B.unsqueeze(0)
     batch = max(a3.shape[0], b3.shape[0])
     M, K = a3.shape[1], a3.shape[2]
     N = b3.shape[2]
+    c3 = None if C is None else (C if C.dim() == 3 else C.unsqueeze(0))
+    # the kernel indexes every operand by the common batch: a mismatch would read past the end
+    if b3.shape[1] != K:
+        raise ValueError(f"gemm: inner dimensions differ ({tuple(A.shape)} @ {tuple(B.shape)})")
+    for name, t, shp in (("A", a3, (M, K)), ("B", b3, (K, N)), ("C", c3, (M, N)), ("out", out, (M, N))):
+        if t is None:
+            continue
+        t3 = t if t.dim() == 3 else t.unsqueeze(0)
+        if t3.shape[0] not in (1, batch) or tuple(t3.shape[1:]) != shp or not t3.is_contiguous():
+            raise ValueError(f"gemm: {name} has shape {tuple(t.shape)}, expected (1 or {batch}, {shp[0]}, {shp[1]}) "
+                             "contiguous")
+        if t.dtype != A.dtype:
+            raise ValueError(f"gemm: {name} has dtype {t.dtype}, A has {A.dtype}")
     if out is None:
         out = torch.empty((batch, M, N), dtype=A.dtype, device=A.device)
+    elif (out if out.dim() == 3 else out.unsqueeze(0)).shape[0] != batch:
+        raise ValueError(f"gemm: out must hold the whole batch of {batch}")
     sA = 0 if a3.shape[0] == 1 and batch > 1 else M * K
     sB = 0 if b3.shape[0] == 1 and batch > 1 else K * N
-    if C is not None:
-        c3 = C if C.dim() == 3 else C.unsqueeze(0)
-        sC = 0 if c3.shape[0] == 1 and batch > 1 else M * N
-    else:
-        c3, sC = None, 0
+    sC = 0 if c3 is None or (c3.shape[0] == 1 and batch > 1) else M * N
+    lib = load_library()
+    h = handle()
     rc = lib.hps_gemm_batched(
         h, dtype_code(A.dtype), M, N, K, float(alpha), ptr(a3), K, sA, ptr(b3), N, sB, float(beta),
         ptr(c3) if c3 is not None else None, N, sC, ptr(out), N, M * N, batch,

@@ -136,7 +136,10 @@ def adjoint_lambda(pde_problem, G):
     vbar = _gemm_conjt(Q, n, 0, n, nq, hbar.contiguous(), w, nq * w, w, nl, C=Gd, beta=1.0)
     _, nrows, _ = leaf.lu.shape
     if leaf.kind == HPS_KIND_ITI:
+        # the factors are of B' = diag(s, 1) B, so B^{-H} = diag(s, 1) B'^{-H}
         lam = getrs_conjt(n, leaf.lu, 0, n, n * n, leaf.perm, leaf.dinv, vbar)
+        if leaf.row_scale is not None:
+            lam[:, : leaf.nb, :] *= leaf.row_scale.to(lam.real.dtype)
     else:
         nb = n - nrows
         lam = torch.zeros_like(vbar)

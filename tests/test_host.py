@@ -277,3 +277,21 @@ def test_scattering_and_interp_fail_loudly_without_gpu():
     with pytest.raises((HPSLibraryError, RuntimeError)):
         dom.interp_from_interior_points(np.zeros(dom.interior_points.shape[:2]), np.linspace(0, 1, 3),
                                         np.linspace(0, 1, 3))
+
+
+def test_gemm_wrapper_rejects_mismatched_operands():
+    """The batched GEMM indexes every operand by the common batch: shapes that
+    would make it read past an operand's end are refused before any launch."""
+    import torch
+
+    from jaxhps_amd import _lib
+
+    A = torch.zeros(3, 4, 5, dtype=torch.float64)
+    with pytest.raises(ValueError):
+        _lib.gemm(A, torch.zeros(5, 5, 6, dtype=torch.float64))  # batch 3 vs 5
+    with pytest.raises(ValueError):
+        _lib.gemm(A, torch.zeros(3, 4, 6, dtype=torch.float64))  # inner 5 vs 4
+    with pytest.raises(ValueError):
+        _lib.gemm(A, torch.zeros(3, 5, 6, dtype=torch.float64), C=torch.zeros(2, 4, 6, dtype=torch.float64))
+    with pytest.raises(ValueError):
+        _lib.gemm(A, torch.zeros(3, 5, 6, dtype=torch.float32))
