@@ -6,10 +6,16 @@ sides), with coefficients, source and boundary data already in HBM.  DOF =
 n_leaves * p^d.  Prints ONE JSON line (rank 0).
 
 Two paths are timed on the same inputs: the mixed-precision build
-(complex64 / FP32 factors, FP64 top two merge levels) whose solve is refined
-to FP64 accuracy -- the reported value -- and the plain FP64 path; the line
-carries the relative error of the mixed result against the FP64 one and the
-speed-up.
+(complex64 / FP32 merge factors at every level by default, leaves as the
+library chooses) whose solve is refined to FP64 accuracy -- the reported
+value -- and the plain FP64 path.  The line carries the relative error of
+the mixed result against the FP64 one, the speed-up, the same mixed build
+with its top two merge levels in FP64 (variant_fp64_top2), and, with the CPU
+baseline, both paths' errors against the CPU oracle and the exact discrete
+solution at L = 4 of the same workload (oracle_check).  The process exits
+with status 3 after printing the line when the mixed result misses 1e-8
+against the FP64 path or the oracle, the FP64 path misses 1e-10 against the
+oracle, or the refinement did not converge.
 
 Default workload: 2D Helmholtz, ItI merges, k = 100, p = 16, L = 7 (16384
 leaves, 4,194,304 DOF).  --config B selects the 2D variable-coefficient DtN
@@ -40,12 +46,19 @@ def parse():
     ap.add_argument("--precision", default="both", choices=["both", "mixed", "fp64"],
                     help="which path(s) to time; the value is the mixed path's unless only fp64 runs")
     ap.add_argument("--fp64-top-levels", type=int, default=0,
-                    help="merge levels at the top kept in FP64 by the mixed path (complex64 merges suffice for "
-                         "<= 1e-8 here: the refined error is set by the FP64 residual)")
+                    help="merge levels at the top kept in FP64 by the mixed path (the library's default, 0: "
+                         "complex64 merges suffice for <= 1e-8, the refined error is set by the FP64 residual)")
+    ap.add_argument("--leaf-precision", default="auto", choices=["auto", "low", "high"],
+                    help="mixed path leaf factors: the library's default (auto), FP32/complex64 (low) or FP64 (high)")
+    ap.add_argument("--no-oracle-check", action="store_true",
+                    help="skip the accuracy check of both GPU paths against the CPU oracle at L = 4")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip timing the mixed path with the top two merge levels in FP64")
     ap.add_argument("--refine-tol", type=float, default=1e-11)
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint grads/s measurement")
     ap.add_argument("--adjoint-L", type=int, default=6, help="tree depth of the adjoint workload (testing only)")
-    ap.add_argument("--workload", default="hps", choices=["hps", "scattering", "interp", "adaptive", "adaptive3d"],
+    ap.add_argument("--workload", default="hps",
+                    choices=["hps", "scattering", "interp", "adaptive", "adaptive3d", "subtree"],
                     help="hps: the headline build+solve; the others time the exterior coupling, the grid "
                          "interpolation and the adaptive 2D build+solve")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -161,6 +174,40 @@ def cpu_baseline(cfg):
             "sample": f"FP64 numpy oracle timed by parts at the workload's full size (L=7, {nl} leaves, {dof} DOF): "
                       f"leaf stage on {ns} leaves x {nl // ns}; one 4:1 merge and one down-pass step per level at "
                       "its real size x the level's merge count; BLAS threads <= " + str(cores)}
+
+
+def oracle_check(cfg, mixed_kwargs, refine_tol, L=4):
+    """Both GPU paths on the workload's own generator at L = 4 against the
+    FP64 CPU oracle (stage by stage) and against the exact discrete solution
+    of the same global system (sparse LU + extended-precision refinement).
+    Relative 2-norm errors."""
+    import numpy as np
+    import torch
+
+    import jaxhps_amd as jh
+    from jaxhps_amd import workloads as wl
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import config_oracle as co
+
+    dom, kw, g = (wl.config_C if cfg == "C" else wl.config_B)(L=L)
+    if cfg == "B":
+        kw = dict(kw)
+        kw["source"] = np.random.default_rng(5).standard_normal(kw["source"].shape)
+    t0 = time.perf_counter()
+    u_stage = co.stage_solution(dom, kw, g)
+    u_exact = co.exact_solution(dom, kw, g)
+    t_cpu = time.perf_counter() - t0
+    out = {"L": L, "n_leaves": dom.n_leaves, "oracle_seconds": t_cpu}
+    for name, bkw in (("fp64", {}), ("mixed", mixed_kwargs)):
+        pde = jh.PDEProblem(dom, **kw)
+        jh.build_solver(pde, **bkw)
+        u = jh.solve(pde, g, refine_tol=refine_tol) if bkw else jh.solve(pde, g)
+        u = u.cpu().numpy().reshape(u_stage.shape)
+        out[f"{name}_vs_oracle"] = co.rel2(u, u_stage)
+        out[f"{name}_vs_exact"] = co.rel2(u, u_exact)
+    out["oracle_vs_exact"] = co.rel2(u_stage, u_exact)
+    return out
 
 
 def measured_traffic(kernel):
@@ -536,6 +583,57 @@ def bench_adaptive3d(args, p=8, q=6, tol=1e-6):
     return out
 
 
+def bench_subtree(args, L=8, height=6):
+    """Subtree recomputation at a depth past the headline: the 2D Helmholtz
+    workload (k = 100, p = 16) at L = 8 (65,536 leaves, 16.8M DOF), solved by
+    solve_subtree (subtrees of height 6 rebuilt on the way down, nothing below
+    their roots kept) and by build_solver + solve (every factor kept), FP64.
+    Step = one solve_subtree; the line records both paths' times, their peak
+    device memory and their relative difference."""
+    import numpy as np
+    import torch
+
+    import jaxhps_amd as jh
+    from jaxhps_amd import workloads as wl
+
+    dev = torch.device("cuda", 0)
+    dom, kw, g = wl.config_C(L=L)
+    dkw = {k: (torch.as_tensor(v).to(dev) if isinstance(v, np.ndarray) else v) for k, v in kw.items()}
+    g_dev = torch.as_tensor(g).to(dev)
+    dof = dom.n_leaves * dom.p**2
+    out = {}
+    sols = {}
+    for name in ("subtree", "stored"):
+        pde = jh.PDEProblem(dom, **dkw)
+
+        def step():
+            if name == "subtree":
+                return jh.solve_subtree(pde, g_dev, subtree_height=height)
+            jh.build_solver(pde)
+            return jh.solve(pde, g_dev)
+
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        torch.cuda.reset_peak_memory_stats(dev)
+        base = torch.cuda.memory_allocated(dev)
+        ms = _timed_steps(step, max(1, args.steps), args.warmup)
+        sols[name] = step()
+        torch.cuda.synchronize()
+        out[name] = {"ms_per_step": ms, "value": dof / (ms / 1e3),
+                     "peak_device_bytes": int(torch.cuda.max_memory_allocated(dev) - base)}
+        del pde
+        print(f"subtree bench: {name} {ms:.1f} ms", file=sys.stderr, flush=True)
+    rel = float(torch.linalg.vector_norm(sols["subtree"] - sols["stored"]) / torch.linalg.vector_norm(sols["stored"]))
+    line = _line(f"solve_subtree DOF/s at L={L} (subtree height {height})", "DOF/s", out["subtree"]["value"],
+                 out["subtree"]["ms_per_step"], max(1, args.steps), args.warmup, "c128",
+                 {"workload": "2D Helmholtz ItI k=100 p=16", "L": L, "n_leaves": dom.n_leaves,
+                  "subtree_height": height}, None, None, None)
+    line["paths"] = out
+    line["rel_diff_subtree_vs_stored"] = rel
+    line["memory_ratio_stored_over_subtree"] = out["stored"]["peak_device_bytes"] / out["subtree"]["peak_device_bytes"]
+    return line
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -568,7 +666,7 @@ def main():
     torch.cuda.set_device(dev)
     if args.workload != "hps":
         fn = {"scattering": bench_scattering, "interp": bench_interp, "adaptive": bench_adaptive,
-              "adaptive3d": bench_adaptive3d}[args.workload]
+              "adaptive3d": bench_adaptive3d, "subtree": bench_subtree}[args.workload]
         print(json.dumps(fn(args)), flush=True)
         return
     dom, kw, g, desc = make_workload(args.config, args.L)
@@ -582,17 +680,22 @@ def main():
     g_dev = torch.as_tensor(g).to(dev)
     dof = dom.n_leaves * dom.p ** (2 if dom.bool_2D else 3)
     paths = ["mixed", "fp64"] if args.precision == "both" else [args.precision]
-    if args.config == "D":
-        paths = ["fp64"]  # the mixed-precision build covers 2D trees
+    low_leaves = {"auto": None, "low": True, "high": False}[args.leaf_precision]
 
-    def timed(precision):
+    def build_kwargs(precision, top=None):
+        if precision == "fp64":
+            return {}
+        return dict(precision="mixed", fp64_top_levels=args.fp64_top_levels if top is None else top,
+                    low_precision_leaves=low_leaves)
+
+    def timed(precision, top=None):
         pde = jh.PDEProblem(dom, **dkw)
+        bkw = build_kwargs(precision, top)
 
         def step():
+            jh.build_solver(pde, **bkw)
             if precision == "mixed":
-                jh.build_solver(pde, precision="mixed", fp64_top_levels=args.fp64_top_levels)
                 return jh.solve(pde, g_dev, refine_tol=args.refine_tol)
-            jh.build_solver(pde)
             return jh.solve(pde, g_dev)
 
         for _ in range(args.warmup):
@@ -608,10 +711,15 @@ def main():
     res = {}
     for prec in paths:
         pde, u, ms = timed(prec)
-        res[prec] = dict(pde=pde, u=u, ms=ms, hist=list(getattr(pde, "refine_history", [])))
+        res[prec] = dict(pde=pde, u=u, ms=ms, hist=list(getattr(pde, "refine_history", [])),
+                         converged=getattr(pde, "refine_converged", None),
+                         leaf_dtype=str(pde._factors["leaf"].lu.dtype).replace("torch.", ""),
+                         level_dtypes=[str(S.dtype).replace("torch.", "") for S in pde.S_lst])
     head = res[paths[0]]
     ms = head["ms"]
     is_c = args.config == "C"
+    dtypes = sorted({head["leaf_dtype"], *head["level_dtypes"]}, key=lambda d: ("64" in d, d))
+    short = {"complex64": "c64", "complex128": "c128", "float32": "f32", "float64": "f64"}
     out = {
         "metric": "build+solve DOF/s, mixed vs FP64 GPU at <=1e-8 rel err; adjoint grads/s",
         "value": dof / (ms / 1e3),
@@ -623,26 +731,35 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("c64+c128" if is_c else "f32+f64") if paths[0] == "mixed" else ("c128" if is_c else "f64"),
+        "dtype": "+".join(short[d] for d in dtypes) + ("+c128 refine" if paths[0] == "mixed" and is_c else
+                                                       "+f64 refine" if paths[0] == "mixed" else ""),
         "data": "synthetic (seeded coefficient fields and boundary data of the config)",
-        "config": dict(desc, path=paths[0], fp64_top_levels=args.fp64_top_levels if paths[0] == "mixed" else None),
+        "config": dict(desc, path=paths[0], fp64_top_levels=args.fp64_top_levels if paths[0] == "mixed" else None,
+                       leaf_dtype=head["leaf_dtype"], merge_level_dtypes=head["level_dtypes"]),
     }
     if "mixed" in res:
         out["refine"] = {"tol": args.refine_tol, "iterations": len(res["mixed"]["hist"]),
-                         "history": res["mixed"]["hist"]}
+                         "converged": res["mixed"]["converged"], "history": res["mixed"]["hist"]}
     if "fp64" in res and "mixed" in res:
         um, u64 = res["mixed"]["u"], res["fp64"]["u"]
         rel = float(torch.linalg.vector_norm(um - u64) / torch.linalg.vector_norm(u64))
         out["fp64_path"] = {"ms_per_step": res["fp64"]["ms"], "value": dof / (res["fp64"]["ms"] / 1e3)}
         out["speedup_vs_fp64"] = res["fp64"]["ms"] / res["mixed"]["ms"]
         out["rel_err_vs_fp64"] = rel
+        if not args.no_variants and dom.L >= 2:
+            pde2, u2, ms2 = timed("mixed", top=2)
+            out["variant_fp64_top2"] = {
+                "ms_per_step": ms2, "value": dof / (ms2 / 1e3), "speedup_vs_fp64": res["fp64"]["ms"] / ms2,
+                "rel_err_vs_fp64": float(torch.linalg.vector_norm(u2 - u64) / torch.linalg.vector_norm(u64)),
+                "refine_iterations": len(pde2.refine_history), "converged": pde2.refine_converged}
+            del pde2, u2
     if not args.profile_only:
         # per-stage breakdown and dominant-kernel roofline from one instrumented step
         pde = head["pde"]
         if paths[0] == "mixed":
             prof = profiling.instrumented_step(
                 pde, g_dev,
-                build=lambda p: jh.build_solver(p, precision="mixed", fp64_top_levels=args.fp64_top_levels),
+                build=lambda p: jh.build_solver(p, **build_kwargs("mixed")),
                 solve=lambda p, gg: jh.solve(p, gg, refine_tol=args.refine_tol))
         else:
             prof = profiling.instrumented_step(pde, g_dev)
@@ -658,14 +775,30 @@ def main():
             out["fp64_path"]["stages_ms"] = p64["stages_ms"]
             out["fp64_path"]["roofline"] = p64["roofline"]
     if not args.no_cpu_baseline and not args.profile_only:
+        # the CPU leg: the oracle timed on the host cores, and (2D) both GPU
+        # paths checked against it at L = 4 of the same workload
         out["cpu_baseline"] = cpu_baseline(args.config)
+        if dom.bool_2D and not args.no_oracle_check:
+            out["oracle_check"] = oracle_check(args.config, build_kwargs("mixed"), args.refine_tol)
     if args.config == "C" and not args.no_adjoint and not args.profile_only:
         for prec in res.values():
             prec.pop("pde", None)
         res.clear()
         torch.cuda.empty_cache()
         out["adjoint"] = adjoint_bench(args.adjoint_L)
+    ok = True
+    if "mixed" in res:
+        ok = bool(res["mixed"]["converged"]) and out.get("rel_err_vs_fp64", 0.0) <= 1e-8
+    oc = out.get("oracle_check")
+    if oc:
+        ok = ok and oc["fp64_vs_oracle"] <= 1e-10 and oc["fp64_vs_exact"] <= 1e-10
+        ok = ok and oc["mixed_vs_oracle"] <= 1e-8 and oc["mixed_vs_exact"] <= 1e-8
+    out["accuracy_ok"] = ok
     print(json.dumps(out), flush=True)
+    if not ok:
+        print("bench.py: the accuracy bar (mixed <= 1e-8 vs FP64 and the oracle, FP64 <= 1e-10 vs the oracle, "
+              "refinement converged) is not met; the value above is not valid", file=sys.stderr)
+        sys.exit(3)
 
 
 if __name__ == "__main__":

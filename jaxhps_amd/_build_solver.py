@@ -66,18 +66,21 @@ def _merge_levels(pde_problem, T, h, return_top_T, dtypes=None):
 
 
 def build_solver(pde_problem, return_top_T: bool = False, compute_device=None, host_device=None,
-                 precision: str = "fp64", fp64_top_levels: int = 2, low_precision_leaves=None):
+                 precision: str = "fp64", fp64_top_levels: int = 0, low_precision_leaves=None):
     """Leaf solves and merges for the whole tree; stores Y, v, S_lst and
     g_tilde_lst in ``pde_problem``.  Returns the top-level T when
     ``return_top_T``.  ``compute_device`` may name the GPU (torch device);
     ``host_device`` is accepted for signature compatibility -- the solver
     state stays in HBM.
 
-    precision="mixed" (2D uniform trees) runs all but the top
+    precision="mixed" (uniform 2D and 3D trees) runs all but the top
     ``fp64_top_levels`` merge levels in FP32 (complex64 for ItI) and stores
     Y, v in that precision; DtN leaves are factored in FP32, ItI leaves in
     complex128 unless ``low_precision_leaves`` says otherwise.  solve() then
-    refines its solution to FP64 accuracy."""
+    refines its solution to FP64 accuracy.  The default keeps no merge level
+    in FP64: the refined error is set by the FP64 residual, and FP32 merges
+    leave the first solution accurate enough (about 1e-6 at p = 16, L = 7,
+    k = 100) for refinement to converge in one or two steps."""
     from . import profiling
 
     dom = pde_problem.domain
@@ -86,8 +89,6 @@ def build_solver(pde_problem, return_top_T: bool = False, compute_device=None, h
             raise NotImplementedError("adaptive trees are built in FP64")
         return _adaptive_build_solver(pde_problem, return_top_T, compute_device)
     leaf_dtype, store_dtype, hi, lo = _compute_dtypes(pde_problem, precision, low_precision_leaves)
-    if precision == "mixed" and not dom.bool_2D:
-        raise NotImplementedError("the mixed-precision build covers 2D uniform trees")
     pde_problem._precision = precision
     dtypes = _level_dtypes(dom.L, precision, hi, lo, fp64_top_levels)
     if pde_problem.source is None:

@@ -52,3 +52,36 @@ def gauss_neighbors_2D(L: int, q: int) -> np.ndarray:
         out[l, 3 * q:4 * q] = (-(3 * nside + (n - 1 - y) * q + k + 1) if x == 0
                                else cell_to_leaf[x - 1, y] * nq + 1 * q + rk)
     return out.astype(np.int32)
+
+
+def gauss_neighbors_3D(root, L: int, q: int) -> np.ndarray:
+    """(n_leaves, 6q^2) int32 for a uniform octree: the neighbour slot
+    l'*6q^2 + j' holding the same Gauss node across a shared face, or
+    -(root boundary index + 1) on the root boundary.  A leaf's slots are its
+    faces 1..6 (x-, x+, y-, y+, z-, z+) with a q x q Gauss grid each, the
+    order of compute_boundary_Gauss_points_uniform_3D on the leaf's box; the
+    root order is Domain.boundary_points.  Nodes are matched by coordinates."""
+    from ._discretization_tree import DiscretizationNode3D
+    from ._grid_creation import _subdivide_3D, compute_boundary_Gauss_points_uniform_3D
+
+    boxes = _subdivide_3D([root.xmin, root.xmax, root.ymin, root.ymax, root.zmin, root.zmax], L)
+    nl = boxes.shape[0]
+    nq = 6 * q * q
+    leaf_pts = np.concatenate(
+        [compute_boundary_Gauss_points_uniform_3D(DiscretizationNode3D(*b), 0, q) for b in boxes])
+    root_pts = compute_boundary_Gauss_points_uniform_3D(root, L, q)
+    scale = max(root.xmax - root.xmin, root.ymax - root.ymin, root.zmax - root.zmin)
+    keys = np.round(np.concatenate([leaf_pts, root_pts]) / scale * 2**40).astype(np.int64)
+    _, group = np.unique(keys, axis=0, return_inverse=True)
+    group = group.ravel()
+    ids = np.concatenate([np.arange(nl * nq), -(np.arange(root_pts.shape[0]) + 1)])
+    order = np.argsort(group, kind="stable")
+    g_sorted = group[order]
+    if np.any(np.bincount(g_sorted) != 2):
+        raise ValueError("Gauss nodes of a uniform octree must pair up (two leaves, or a leaf and the root boundary)")
+    a, b = ids[order[0::2]], ids[order[1::2]]
+    out = np.empty(nl * nq, dtype=np.int64)
+    # every pair holds at least one leaf slot (a root node is matched by exactly one leaf)
+    out[a[a >= 0]] = b[a >= 0]
+    out[b[b >= 0]] = a[b >= 0]
+    return out.reshape(nl, nq).astype(np.int32)

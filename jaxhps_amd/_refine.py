@@ -1,4 +1,4 @@
-"""Mixed-precision solve with FP64 iterative refinement (uniform 2D trees).
+"""Mixed-precision solve with FP64 iterative refinement (uniform 2D and 3D trees).
 
 ``build_solver(..., precision="mixed")`` factors the leaves and the lower merge
 levels in FP32 (complex64 for ItI) and the top ``fp64_top_levels`` merge
@@ -19,8 +19,8 @@ f_int - A_int u] with t_in the root data or minus the neighbour's outgoing
 impedance.  The correction is an up pass whose leaf right-hand side has both
 row blocks, and a down pass from zero root data.
 
-DtN state: the leaves' Gauss boundary data g_leaf (shared values agree on both
-sides of an interface) and u, whose boundary rows are P g_leaf.  Residuals:
+DtN state (2D and 3D): the leaves' Gauss boundary data g_leaf (shared values
+agree on both sides of an interface) and u, whose boundary rows are P g_leaf.  Residuals:
 interior rows f - A u, the flux mismatch Q u_a + Q u_b on interior Gauss nodes
 (added to one leaf's h in the up pass) and g_root - g_leaf on the root
 boundary (the down pass's root data).
@@ -29,6 +29,7 @@ boundary (the down pass's root data).
 from __future__ import annotations
 
 import ctypes
+import functools
 import warnings
 
 import numpy as np
@@ -36,7 +37,7 @@ import torch
 
 from . import _lib, profiling
 from ._lib import HPS_KIND_DTN, check, dtype_code, load_library, ptr
-from ._neighbors import gauss_neighbors_2D
+from ._neighbors import gauss_neighbors_2D, gauss_neighbors_3D
 from ._stages import cast, leaf_apply_inverse, to_device
 
 LOW = {torch.float64: torch.float32, torch.complex128: torch.complex64}
@@ -68,12 +69,12 @@ class RefineOps:
         self.MT = dev(M.T)
         self.P = dev(pde.P)
         self.coeffs = None  # set by refine_ops from the current coefficient fields
-        # leaf adjacency (2D trees; the refinement residuals need it)
-        self.nbr = torch.as_tensor(gauss_neighbors_2D(dom.L, dom.q)).to(device) if dom.bool_2D else None
+        # leaf adjacency on the Gauss nodes (the refinement residuals need it)
+        self.nbr = torch.as_tensor(_neighbors(dom)).to(device)
         self.nb, self.nq = pde.P.shape
-        self.n = dom.p**2
+        self.n = dom.p**2 if dom.bool_2D else dom.p**3
         self.nl = dom.n_leaves
-        self.nroot = 4 * dom.q * 2**dom.L
+        self.nroot = root_points(dom)
         self.dtype = dtype
         self.device = device
 
@@ -93,6 +94,45 @@ class RefineOps:
         if w == 1:
             return _lib.gemm(u.view(nl, n), self.MT).view(nl, self.nrow_R, 1)
         return _lib.gemm(self.M, u)
+
+
+def root_points(dom) -> int:
+    """Number of Gauss points on the root boundary of a uniform tree."""
+    return 4 * dom.q * 2**dom.L if dom.bool_2D else 6 * dom.q**2 * 4**dom.L
+
+
+@functools.lru_cache(maxsize=8)
+def _neighbors_cached(bool_2D, L, q, bounds):
+    if bool_2D:
+        return gauss_neighbors_2D(L, q)
+    from ._discretization_tree import DiscretizationNode3D
+
+    return gauss_neighbors_3D(DiscretizationNode3D(*bounds), L, q)
+
+
+def _neighbors(dom):
+    r = dom.root
+    bounds = (r.xmin, r.xmax, r.ymin, r.ymax) if dom.bool_2D else (r.xmin, r.xmax, r.ymin, r.ymax, r.zmin, r.zmax)
+    return _neighbors_cached(dom.bool_2D, dom.L, dom.q, tuple(float(b) for b in bounds))
+
+
+def tree_down(pde, g_root, g_lst, v, Y="Y"):
+    """Down pass through the build's stored S (each level in its own
+    precision) from root data (nroot, w); returns (u, leaf boundary data).
+    3D roots are stored without their batch axis and use the oct tables."""
+    from .down_pass import down_levels
+    from ._tables import oct_merge_tables
+
+    S_lst = list(pde.S_lst)
+    g_lst = list(g_lst) if g_lst else []
+    table_fn = None
+    if not pde.domain.bool_2D:
+        table_fn = lambda ne: oct_merge_tables(ne // 24)  # noqa: E731
+        if S_lst[-1].ndim == 2:
+            S_lst[-1] = S_lst[-1].unsqueeze(0)
+            if g_lst and g_lst[-1].ndim == S_lst[-1].ndim - 1:
+                g_lst[-1] = g_lst[-1].unsqueeze(0)
+    return down_levels(g_root, S_lst, g_lst, getattr(pde, Y) if isinstance(Y, str) else Y, v, table_fn=table_fn)
 
 
 def refine_ops(pde, device, dtype):
@@ -200,7 +240,6 @@ def refine_solve(pde, boundary_data, u0, source, g_leaf0=None, tol=1e-11, max_it
     evaluated; the next update is estimated as the last update times the
     residual's reduction and the loop ends when that estimate (or an actual
     update) is below tol relative to |u|."""
-    from .down_pass import down_levels
     from .up_pass import _up
 
     squeeze = u0.ndim == 2
@@ -218,7 +257,7 @@ def refine_solve(pde, boundary_data, u0, source, g_leaf0=None, tol=1e-11, max_it
             f = f.expand(-1, -1, w)
         f = f.contiguous()
     g_root = _root_data(boundary_data, dev, hi, w)
-    nroot = 4 * pde.domain.q * 2**pde.domain.L
+    nroot = root_points(pde.domain)
     if g_root.shape[0] != nroot:
         raise ValueError(f"boundary data has {g_root.shape[0]} points, the root boundary has {nroot}")
     iti = pde.use_ItI
@@ -252,14 +291,14 @@ def refine_solve(pde, boundary_data, u0, source, g_leaf0=None, tol=1e-11, max_it
             with profiling.stage("refine_up"):
                 v, g_lst = _up(res, pde, rhs_full=True)
             with profiling.stage("refine_down"):
-                du, _ = down_levels(torch.zeros_like(g_root), pde.S_lst, g_lst, pde.Y, v)
+                du, _ = tree_down(pde, torch.zeros_like(g_root), g_lst, v)
             with profiling.stage("refine_update"):
                 dn2, un2 = refine_update(u, du)
         else:
             with profiling.stage("refine_up"):
                 v, g_lst = _up(res, pde, rhs_full=True, hinj=hinj)
             with profiling.stage("refine_down"):
-                du, dg = down_levels(groot, pde.S_lst, g_lst, pde.Y, v)
+                du, dg = tree_down(pde, groot, g_lst, v)
             with profiling.stage("refine_update"):
                 dn2, un2 = refine_update(u, du)
                 refine_update(g_leaf, dg)

@@ -26,7 +26,15 @@ def _solve_mixed(pde_problem, boundary_data, source, refine_tol, max_refine):
     else:
         v, g_lst = pde_problem.v, pde_problem.g_tilde_lst
         f = pde_problem.source
-    u0, g_leaf = _down(boundary_data, pde_problem.S_lst, g_lst, pde_problem.Y, v, return_leaf_g=True)
+    S_lst, table_fn = pde_problem.S_lst, None
+    if not pde_problem.domain.bool_2D:
+        from ._tables import oct_merge_tables
+
+        # the 3D root level is stored without its batch axis
+        S_lst = [*S_lst[:-1], S_lst[-1].unsqueeze(0)]
+        g_lst = [*g_lst[:-1], g_lst[-1].unsqueeze(0)] if g_lst else g_lst
+        table_fn = lambda ne: oct_merge_tables(ne // 24)  # noqa: E731
+    u0, g_leaf = _down(boundary_data, S_lst, g_lst, pde_problem.Y, v, return_leaf_g=True, table_fn=table_fn)
     return refine_solve(pde_problem, boundary_data, u0, f, g_leaf0=g_leaf, tol=refine_tol, max_iter=max_refine)
 
 

@@ -35,8 +35,12 @@ FLAGS = [
 ]
 
 
-def _compile(src: str) -> Path:
+def _compile(src: str, incremental: bool = False) -> Path:
     obj = BUILD / (Path(src).stem + ".o")
+    if incremental and obj.exists():
+        deps = [HERE / src, *HERE.glob("*.h"), *(REPO / "include").glob("*.h")]
+        if obj.stat().st_mtime > max(d.stat().st_mtime for d in deps):
+            return obj
     cmd = [HIPCC, *FLAGS, "-c", str(HERE / src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -44,10 +48,12 @@ def _compile(src: str) -> Path:
     return obj
 
 
-def build(verbose: bool = True) -> Path:
+def build(verbose: bool = True, incremental: bool = False) -> Path:
+    """Compile every source (incremental=True keeps objects newer than their
+    source and every header) and link the library."""
     BUILD.mkdir(exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(lambda f: _compile(f, incremental), SOURCES))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -60,5 +66,5 @@ def build(verbose: bool = True) -> Path:
 
 
 if __name__ == "__main__":
-    build()
+    build(incremental="--incremental" in sys.argv)
     sys.exit(0)

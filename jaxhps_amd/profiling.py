@@ -112,7 +112,7 @@ def instrumented_step(pde, g_dev, build=None, solve=None):
             dict(name=r["name"], launches=r["launches"], ms=round(r["ms"], 3),
                  tflops=(r["flops"] / (r["ms"] * 1e-3) / 1e12) if r["ms"] > 0 and r["flops"] else None,
                  gbs=(r["bytes"] / (r["ms"] * 1e-3) / 1e9) if r["ms"] > 0 else None)
-            for r in table[:12]
+            for r in table[:24]
         ],
     }
 

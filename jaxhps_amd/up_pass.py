@@ -12,7 +12,7 @@ from __future__ import annotations
 
 import torch
 
-from . import _lib
+from . import _lib, profiling
 from ._stages import cast, leaf_apply_inverse, to_device, up_level
 
 
@@ -45,19 +45,22 @@ def _up(source, pde_problem, return_h_last=False, inj_lst=None, rhs_full=False, 
         rhs = src.clone()
         if pde_problem.use_ItI:
             rhs[:, :nb, :] = 0  # ItI: the boundary rows carry the incoming data, zero here
-    v = leaf_apply_inverse(leaf, rhs)
-    Q = pde_problem.device_ops(dev, dt)["Q"]
-    if hinj is not None:
-        h = _lib.gemm(Q, v, C=cast(hinj, dt), beta=1.0)
-    else:
-        h = _lib.gemm(Q, v)
+    with profiling.stage("up_leaf"):
+        v = leaf_apply_inverse(leaf, rhs)
+        Q = pde_problem.device_ops(dev, dt)["Q"]
+        if hinj is not None:
+            h = _lib.gemm(Q, v, C=cast(hinj, dt), beta=1.0)
+        else:
+            h = _lib.gemm(Q, v)
     g_lst = []
     levels = fac["levels"]
     for i, lvl in enumerate(levels):
         last = i == len(levels) - 1
         inj = inj_lst[i] if inj_lst is not None else None
-        h = cast(h, lvl.F_lu.dtype)
-        gt, h = up_level(h.reshape(-1, lvl.tables.nec, h.shape[-1]), lvl, inj=inj, want_h=(not last) or return_h_last)
+        with profiling.stage(f"up_level_{i + 1}"):
+            h = cast(h, lvl.F_lu.dtype)
+            gt, h = up_level(h.reshape(-1, lvl.tables.nec, h.shape[-1]), lvl, inj=inj,
+                             want_h=(not last) or return_h_last)
         g_lst.append(gt if multi else gt[..., 0])
     v_out = v if multi else v[..., 0]
     if return_h_last:

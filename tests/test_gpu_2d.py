@@ -241,7 +241,7 @@ def test_nosource_iti_build_then_source_solve(gpu, L):
 @pytest.mark.parametrize("kind", ["dtn", "iti"])
 def test_subtree_recomputation_matches_build_solve(gpu, kind):
     """solve_subtree and upward/downward_pass_subtree (_subtree_recomp.py)
-    give the build_solver + solve solution."""
+    give the oracle's solution and the build_solver + solve one."""
     import jaxhps_amd as jh
 
     print(f"subtree recomputation kind={kind}", flush=True)
@@ -249,15 +249,21 @@ def test_subtree_recomputation_matches_build_solve(gpu, kind):
     rng = np.random.default_rng(4)
     nb = dom.boundary_points.shape[0]
     g = rng.standard_normal(nb) + (1j * rng.standard_normal(nb) if kind == "iti" else 0)
+    # the oracle's stage-by-stage solution of the same problem
+    ref = _oracle(pde, kind)
+    u_oracle, _ = hps2d.down_pass(g[:, None], ref["S"], ref["g"], ref["Y"], ref["v"], kind)
     jh.build_solver(pde)
     u_ref = _np(jh.solve(pde, g))
+    assert _rel(u_ref, u_oracle[..., 0]) < 1e-10
     _, pde2 = _problem(8, 6, 4, kind, seed=2)
     u_sub = _np(jh.solve_subtree(pde2, g, subtree_height=2))
+    assert _rel(u_sub, u_oracle[..., 0]) < 1e-10
     assert _rel(u_sub, u_ref) < 1e-11
     _, pde3 = _problem(8, 6, 4, kind, seed=2)
     T = jh.upward_pass_subtree(pde3, subtree_height=2)
     assert T.shape == (nb, nb)
     u_ud = _np(jh.downward_pass_subtree(pde3, g, subtree_height=2))
+    assert _rel(u_ud, u_oracle[..., 0]) < 1e-10
     assert _rel(u_ud, u_ref) < 1e-11
     # all-together branch when the tree is no taller than the subtree height
     _, pde4 = _problem(8, 6, 4, kind, seed=2)

@@ -127,3 +127,31 @@ def test_config_d_workload_small(gpu):
     assert _rel(u, u0) < 1e-10
     # and it approximates the manufactured solution
     assert np.abs(u - wl.exact_D(dom)).max() < 1e-2
+
+
+@pytest.mark.parametrize("p,L", [(8, 1), (10, 2)])
+def test_config_d_mixed_precision_matches_oracle(gpu, p, L):
+    """Config D's workload (bumps coefficient, 4 right-hand sides) built in
+    FP32 (leaves and every merge level, the root included) and refined in
+    FP64: within 1e-8 of the oracle's stage-by-stage FP64 solution."""
+    import jaxhps_amd as jh
+    from jaxhps_amd import workloads as wl
+
+    print(f"config D mixed p={p} L={L}", flush=True)
+    dom, kw, g = wl.config_D(L=L, p=p, nsrc=4)
+    pde = jh.PDEProblem(dom, **kw)
+    jh.build_solver(pde, precision="mixed")
+    assert str(pde._factors["leaf"].lu.dtype) == "torch.float32"
+    assert all(str(S.dtype) == "torch.float32" for S in pde.S_lst)
+    u = jh.solve(pde, g, refine_tol=1e-12)
+    assert str(u.dtype) == "torch.float64"
+    assert pde.refine_converged, pde.refine_history
+    u = _np(u)
+    ops = hps3d.leaf_ops_3D(p, p - 2, 2.0 / 2 ** (L + 1))
+    names = {"D_xx_coefficients": "D_xx", "D_yy_coefficients": "D_yy", "D_zz_coefficients": "D_zz",
+             "D_x_coefficients": "D_x", "D_y_coefficients": "D_y", "D_z_coefficients": "D_z"}
+    A = hps2d.assemble_diff_operator({names[k]: v for k, v in kw.items() if k in names}, ops, order=ORDER3)
+    Y, T, v, h = hps2d.leaf_dtn(A, kw["source"], ops["P"], ops["Q"])
+    S, gl, _, _ = hps3d.merge_stage_3d(T, h, L)
+    u0, _ = hps3d.down_pass_3d(g, S, gl, Y, v)
+    assert np.linalg.norm(u - u0) / np.linalg.norm(u0) < 1e-8
