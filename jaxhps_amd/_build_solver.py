@@ -26,14 +26,15 @@ def _level_dtypes(n_levels, precision, hi, lo, fp64_top_levels):
 def _compute_dtypes(pde_problem, precision, low_leaf=None):
     """(leaf factor dtype, leaf output storage dtype, FP64 dtype, low dtype).
 
-    The mixed build factors ItI leaves in complex128 by default: a complex64
-    copy of the leaf operator cannot represent the k^2 term next to the
-    O((p^2/h)^2) differentiation entries (a relative error of about
-    eps_32 * (p^2/h)^2 / k^2 in the local wavenumber, which the global
-    Helmholtz problem amplifies by its phase sensitivity), and refinement
-    then fails to contract on fine trees.  Merges have no such cancellation
-    and run in complex64.  DtN leaves (elliptic, no oscillation) are
-    factored in FP32.  low_leaf overrides the choice."""
+    The mixed build factors the leaves in FP32 (complex64 for ItI) like the
+    merges.  An ItI leaf matrix B = [G; A_int] is factored with its impedance
+    rows scaled to the size of the collocation rows (_stages.iti_row_scale):
+    unscaled, B's condition number at p = 16, L = 7 is ~2e6 and a complex64
+    factorisation loses the k^2 (1 - q) term next to the O((p^2/h)^2)
+    differentiation entries badly enough that refinement stops contracting;
+    scaled, it is ~1e4 and the complex64 leaf maps are accurate to ~5e-6,
+    which refinement corrects in two or three steps.  low_leaf=False keeps
+    the leaves in FP64 / complex128."""
     from .local_solve import _compute_dtype
     from ._refine import LOW
 
@@ -44,8 +45,19 @@ def _compute_dtypes(pde_problem, precision, low_leaf=None):
         return hi, hi, hi, hi
     lo = LOW[hi]
     if low_leaf is None:
-        low_leaf = not pde_problem.use_ItI
+        low_leaf = True
     return (lo if low_leaf else hi), lo, hi, lo
+
+
+def _release_state(pde_problem):
+    """Drop a previous build's solver state before building again, so that a
+    rebuild never holds two sets of factors (the reference overwrites the
+    same attributes)."""
+    pde_problem.Y = None
+    pde_problem.v = None
+    pde_problem.S_lst = []
+    pde_problem.g_tilde_lst = []
+    pde_problem._factors = None
 
 
 def _merge_levels(pde_problem, T, h, return_top_T, dtypes=None):
@@ -75,8 +87,8 @@ def build_solver(pde_problem, return_top_T: bool = False, compute_device=None, h
 
     precision="mixed" (uniform 2D and 3D trees) runs all but the top
     ``fp64_top_levels`` merge levels in FP32 (complex64 for ItI) and stores
-    Y, v in that precision; DtN leaves are factored in FP32, ItI leaves in
-    complex128 unless ``low_precision_leaves`` says otherwise.  solve() then
+    Y, v in that precision; the leaves are factored in FP32 / complex64 unless
+    ``low_precision_leaves=False`` keeps them in FP64.  solve() then
     refines its solution to FP64 accuracy.  The default keeps no merge level
     in FP64: the refined error is set by the FP64 residual, and FP32 merges
     leave the first solution accurate enough (about 1e-6 at p = 16, L = 7,
@@ -84,6 +96,7 @@ def build_solver(pde_problem, return_top_T: bool = False, compute_device=None, h
     from . import profiling
 
     dom = pde_problem.domain
+    _release_state(pde_problem)
     if not dom.bool_uniform:
         if precision != "fp64":
             raise NotImplementedError("adaptive trees are built in FP64")

@@ -25,6 +25,27 @@ def gaussian_bumps(x, centers, widths, amps):
     return val, grad
 
 
+def config_A(L: int = 3, p: int = 8):
+    """2D Poisson -Lap u = f on [-1, 1]^2, DtN, with the manufactured solution
+    u = sin(pi x) sin(pi y): D_xx = D_yy = -1, f = 2 pi^2 u, Dirichlet data u
+    on the root boundary's Gauss points."""
+    root = DiscretizationNode2D(-1.0, 1.0, -1.0, 1.0)
+    dom = Domain(p=p, q=p - 2, root=root, L=L)
+    x = dom.interior_points
+    ones = np.ones(x.shape[:2])
+    u = np.sin(np.pi * x[..., 0]) * np.sin(np.pi * x[..., 1])
+    kwargs = dict(D_xx_coefficients=-ones, D_yy_coefficients=-ones, source=2 * np.pi**2 * u)
+    bp = dom.boundary_points
+    g = np.sin(np.pi * bp[:, 0]) * np.sin(np.pi * bp[:, 1])
+    return dom, kwargs, g
+
+
+def exact_A(dom):
+    """The manufactured solution of config_A at the leaf points."""
+    x = dom.interior_points
+    return np.sin(np.pi * x[..., 0]) * np.sin(np.pi * x[..., 1])
+
+
 def config_B(L: int = 6, p: int = 16, nsrc: int = 16, seed: int = 0):
     """2D variable-coefficient elliptic -div(a grad u) + c u = f, DtN.
 

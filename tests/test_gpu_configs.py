@@ -130,3 +130,18 @@ def test_inverse_scattering_gradient_matches_central_fd(gpu):
             e[j] = eps
             fd[j] = (float(J(theta.detach() + e)) - float(J(theta.detach() - e))) / (2 * eps)
     assert np.abs(grad - fd).max() <= 1e-6 * np.abs(fd).max(), (grad, fd)
+
+
+def test_poisson_config_a_matches_oracle_and_manufactured_solution(gpu):
+    """Config A (2D Poisson, p = 8, L = 3, DtN, u = sin(pi x) sin(pi y)):
+    FP64 GPU against the oracle (1e-10) and the manufactured solution (the
+    discretisation error at p = 8)."""
+    from jaxhps_amd import workloads as wl
+
+    print("config A", flush=True)
+    dom, kw, g = wl.config_A()
+    _, u = _run(dom, kw, g, "fp64")
+    u_stage = co.stage_solution(dom, kw, g)[..., 0]
+    assert co.rel2(u, u_stage) < 1e-10
+    assert co.rel2(u, co.exact_solution(dom, kw, g)[..., 0]) < 1e-10
+    assert np.abs(u - wl.exact_A(dom)).max() < 1e-5
