@@ -169,6 +169,20 @@ size_t hps_down_workspace(int dtype, int nm, int ni, int nsrc);
 int hps_down_level(hps_handle_t h, int dtype, int nm, int nc, int nec, int ni, int ne, int nsrc, const void* S,
                    const void* gt, const void* g_in, const int* child_src, void* g_out, void* ws, size_t ws_bytes);
 
+/* hps_down_level with no workspace argument and no allocation: g_int lives
+ * in a library-owned static device array of HPS_STATIC_SCRATCH_BYTES per GPU
+ * (hps_static_scratch_bytes()), held until the process ends.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops re-solving for new
+ * boundary data).  Sizes with nm * ni * nsrc * sizeof(element) above the
+ * array return HPS_ERR_WORKSPACE.  Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (several streams or threads) race.
+ * The handle must have been created on the GPU it runs on. */
+#define HPS_STATIC_SCRATCH_BYTES (32u << 20)
+size_t hps_static_scratch_bytes(void);
+int hps_down_level_static(hps_handle_t h, int dtype, int nm, int nc, int nec, int ni, int ne, int nsrc,
+                          const void* S, const void* gt, const void* g_in, const int* child_src, void* g_out);
+
 /* u = Y g + v per leaf (v may be NULL).  Replaces the leaf einsum of the down
  * pass (down_pass/_uniform_2D_DtN.py:110-120). */
 int hps_leaf_apply(hps_handle_t h, int dtype, int nl, int n, int nq, int nsrc, const void* Y, const void* g,

@@ -60,6 +60,8 @@ SIGNATURES = {
     ),
     "hps_down_workspace": (_S, [_I, _I, _I, _I]),
     "hps_down_level": (_I, [_P, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _S]),
+    "hps_static_scratch_bytes": (_S, []),
+    "hps_down_level_static": (_I, [_P, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P]),
     "hps_leaf_apply": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _P, _P]),
     "hps_residual_iti": (_I, [_P] + [_I] * 9 + [_P, _P, _P, _I] + [_P] * 7),
     "hps_residual_dtn": (_I, [_P] + [_I] * 9 + [_P, _P, _P, _I] + [_P] * 9),

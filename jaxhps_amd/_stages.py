@@ -267,6 +267,25 @@ def down_level(S, gt, g_in, tables: MergeTables):
     return g_out
 
 
+def down_level_static(S, gt, g_in, tables: MergeTables):
+    """down_level through hps_down_level_static: no workspace is allocated
+    (the interface data live in the library's static per-GPU scratch).
+    Calls that overlap on one GPU race on that scratch."""
+    lib = load_library()
+    hd = _lib.handle()
+    dev = S.device
+    dt = S.dtype
+    nc, nec, ni, ne = tables.nc, tables.nec, tables.ni, tables.ne
+    nm = S.shape[0]
+    nsrc = g_in.shape[-1]
+    src = tables.device(dev)[3]
+    g_out = torch.empty((nm * nc, nec, nsrc), dtype=dt, device=dev)
+    rc = lib.hps_down_level_static(hd, dtype_code(dt), nm, nc, nec, ni, ne, nsrc, ptr(S), ptr(gt), ptr(g_in),
+                                   ptr(src), ptr(g_out))
+    check(rc, hd, "hps_down_level_static")
+    return g_out
+
+
 def leaf_apply(Y, g, v):
     """u = Y g + v per leaf."""
     lib = load_library()

@@ -15,6 +15,8 @@
 #include <cstring>
 #include <string>
 
+#include <mutex>
+
 #include "hps_capi_internal.h"
 
 namespace hps {
@@ -192,13 +194,49 @@ __global__ __launch_bounds__(256) void down_scatter_kernel(int nc, int nec, int 
 // The entry points below get C linkage from their declarations in
 // hps_mi355x.h; the template helpers between them stay C++.
 
+// ---------------------------------------------------------------------------
+// Library-owned device scratch for the entry points that take no workspace
+// (hps_down_level_static).  One copy per GPU, held until the process ends.
+// In host code `scratch` names a host-side stand-in, not device memory: the
+// device address of each GPU's copy comes from hipGetSymbolAddress, resolved
+// once per device when the first handle on that device is created, and only
+// that address is ever handed to kernels.
+// ---------------------------------------------------------------------------
+#define HPS_MAX_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[HPS_STATIC_SCRATCH_BYTES];
+static void* g_scratch_ptr[HPS_MAX_DEVICES];
+static hipError_t g_scratch_err[HPS_MAX_DEVICES];
+static bool g_scratch_done[HPS_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+// Resolve this device's copy of `scratch` with the device current.
+static void resolve_scratch(int dev) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (dev < 0 || dev >= HPS_MAX_DEVICES || g_scratch_done[dev]) return;
+  int cur = -1;
+  hipError_t e = hipGetDevice(&cur);
+  if (e == hipSuccess && cur != dev) e = hipSetDevice(dev);
+  void* p = nullptr;
+  if (e == hipSuccess) e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (cur >= 0 && cur != dev) (void)hipSetDevice(cur);
+  g_scratch_ptr[dev] = e == hipSuccess ? p : nullptr;
+  g_scratch_err[dev] = e;
+  g_scratch_done[dev] = true;
+}
+
 int hps_create(void* stream, hps_handle_t* out) {
   if (!out) return HPS_ERR_ARG;
   hps_handle_s* h = new hps_handle_s();
   h->stream = (hipStream_t)stream;
+  int dev = -1;
+  hipError_t e = stream ? hipStreamGetDevice((hipStream_t)stream, &dev) : hipGetDevice(&dev);
+  h->device = e == hipSuccess ? dev : -1;
+  if (h->device >= 0) resolve_scratch(h->device);
   *out = h;
   return HPS_OK;
 }
+
+size_t hps_static_scratch_bytes(void) { return HPS_STATIC_SCRATCH_BYTES; }
 
 int hps_destroy(hps_handle_t h) {
   delete h;
@@ -912,6 +950,31 @@ int hps_down_level(hps_handle_t h, int dtype, int nm, int nc, int nec, int ni, i
     return HPS_ERR_WORKSPACE;
   }
   DISPATCH(dtype, down_t, h, nm, nc, nec, ni, ne, nsrc, S, gt, g_in, child_src, g_out, ws);
+}
+
+int hps_down_level_static(hps_handle_t h, int dtype, int nm, int nc, int nec, int ni, int ne, int nsrc,
+                          const void* S, const void* gt, const void* g_in, const int* child_src, void* g_out) {
+  if (!h) return HPS_ERR_ARG;
+  HPS_ARG(h, nm > 0 && nc > 0 && nec > 0 && ni > 0 && ne > 0 && nsrc > 0, "bad down-pass sizes");
+  HPS_ARG(h, S && g_in && child_src && g_out, "null pointer");
+  HPS_ARG(h, dtype >= HPS_F64 && dtype <= HPS_C64, "bad dtype");
+  const size_t need = (size_t)nm * ni * nsrc * dtype_size(dtype);
+  if (need > HPS_STATIC_SCRATCH_BYTES) {
+    h->err = "hps_down_level_static: nm * ni * nsrc elements (" + std::to_string(need) +
+             " bytes) exceed the library's static scratch of " + std::to_string((size_t)HPS_STATIC_SCRATCH_BYTES) +
+             " bytes; use hps_down_level with a workspace";
+    return HPS_ERR_WORKSPACE;
+  }
+  const int dev = h->device;
+  if (dev < 0 || dev >= HPS_MAX_DEVICES || !g_scratch_done[dev] || !g_scratch_ptr[dev]) {
+    h->err = std::string("hps_down_level_static: no static scratch on this handle's device") +
+             (dev >= 0 && dev < HPS_MAX_DEVICES && g_scratch_done[dev]
+                  ? std::string(" (hipGetSymbolAddress: ") + hipGetErrorString(g_scratch_err[dev]) + ")"
+                  : std::string());
+    return HPS_ERR_HIP;
+  }
+  // every element of g_int is written by the product before the scatter reads it
+  DISPATCH(dtype, down_t, h, nm, nc, nec, ni, ne, nsrc, S, gt, g_in, child_src, g_out, g_scratch_ptr[dev]);
 }
 
 template <typename T>

@@ -33,6 +33,7 @@ inline dim3 grid_for(long total, int batch) {
 
 struct hps_handle_s {
   hipStream_t stream = nullptr;
+  int device = -1;  // the stream's device at creation
   std::string err;
 };
 
