@@ -331,30 +331,40 @@ def adjoint_bench(L=6, warmup=1, reps=2):
         u_obs = differentiable_solve(pde, g_dev, I=I_star).clone()
     thetas_t = torch.as_tensor(thetas, device=dev)
 
-    def batch():
+    def batch(ckpt):
         grads = []
         for th in thetas_t:
             th = th.clone().requires_grad_(True)
             I = k**2 * (1.0 - sine_field(th, basis_flat).reshape(field_shape))
-            u = differentiable_solve(pde, g_dev, I=I)
+            u = differentiable_solve(pde, g_dev, checkpoint=ckpt, I=I)
             J = ((u - u_obs).abs() ** 2).sum()
             J.backward()
             grads.append(th.grad)
         return torch.stack(grads)
 
-    for _ in range(warmup):
-        batch()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        grads = batch()
-    torch.cuda.synchronize()
-    t = (time.perf_counter() - t0) / reps
+    res = {}
+    for ckpt in (False, True):
+        for _ in range(warmup):
+            batch(ckpt)
+        torch.cuda.synchronize()
+        torch.cuda.reset_peak_memory_stats(dev)
+        base = torch.cuda.memory_allocated(dev)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            grads = batch(ckpt)
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / reps
+        res[ckpt] = (t, grads, int(torch.cuda.max_memory_allocated(dev) - base))
+    t, grads, mem = res[False]
+    tc, gc, memc = res[True]
     return {"metric": "adjoint grads/s", "value": thetas.shape[0] / t, "unit": "grads/s",
             "ms_per_grad": t / thetas.shape[0] * 1e3, "batch": int(thetas.shape[0]),
             "config": {"workload": "2D Helmholtz inverse scattering ItI k=40 p=16, 25 sine coefficients",
                        "L": L, "n_leaves": dom.n_leaves},
-            "dtype": "c128", "grad_norm": float(grads.norm())}
+            "dtype": "c128", "grad_norm": float(grads.norm()), "peak_device_bytes": mem,
+            "checkpointed": {"value": thetas.shape[0] / tc, "ms_per_grad": tc / thetas.shape[0] * 1e3,
+                             "peak_device_bytes": memc,
+                             "max_rel_diff_vs_stored": float((gc - grads).abs().max() / grads.abs().max())}}
 
 
 def _line(metric, unit, value, ms, steps, warmup, dtype, config, roofline, kernels, cpu):

@@ -136,6 +136,19 @@ int hps_leaf_solve(hps_handle_t h, int dtype, int kind, int nl, int n, int nb, i
  *   (merge/_uniform_2D_DtN.py:207-348, merge/_uniform_2D_ItI.py:183-375,
  *   merge/_schur_complement.py:7-237) and _uniform_oct_merge_DtN
  *   (merge/_uniform_3D_DtN.py:128-187, merge/_schur_complement.py:294-774). */
+/* Pivoting of hps_merge_level's factorisation of D (DtN) or W (ItI) on this
+ * handle: 1 (default) = partial pivoting; 0 = the panels taller than 1024
+ * rows (512 for complex128) are factored without row exchanges, the shorter
+ * trailing panels keep partial pivoting.  The merge matrices of the uniform
+ * tree eliminate stably in their natural order (pivot growth ~1, multipliers
+ * below 4 on the Helmholtz and variable-coefficient workloads), while the
+ * tall pivoted (tournament) panels of the top levels are latency-bound; a
+ * caller that turns pivoting off checks the factors (largest multiplier,
+ * finite, nonzero diagonal) and refactors with pivoting when they fail; info
+ * reports only the pivoted panels' zero columns.  The reference factors with
+ * LAPACK's pivoted LU inside jnp.linalg.inv
+ * (merge/_schur_complement.py:107,146). */
+int hps_set_merge_pivoting(hps_handle_t h, int partial_pivoting);
 size_t hps_merge_workspace(int dtype, int kind, int nm, int ni, int ne, int nsrc);
 int hps_merge_level(hps_handle_t h, int dtype, int kind, int nm, int nc, int nec, int ni, int ne, int nsrc,
                     const void* T_in, const void* h_in, const int* int_rows, const int* int_cols, const int* ext,

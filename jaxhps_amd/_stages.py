@@ -205,9 +205,11 @@ class MergeLevel:
     tables: MergeTables
 
 
-def merge_level(T_in, h_in, tables: MergeTables, keep_factors=False, want_T=True) -> MergeLevel:
+def merge_level(T_in, h_in, tables: MergeTables, keep_factors=False, want_T=True, pivot=True) -> MergeLevel:
     """One level of 4:1 / 8:1 merges.  T_in (nm*nc, nec, nec), h_in (nm*nc, nec, nsrc) or None.
-    keep_factors also keeps B (and D12/D21 for ItI) for the up pass."""
+    keep_factors also keeps B (and D12/D21 for ItI) for the up pass.
+    pivot=False factors the interface matrices without row exchanges (see
+    merge_factors_stable for the check the caller owes)."""
     lib = load_library()
     hd = _lib.handle()
     dev = T_in.device
@@ -235,7 +237,34 @@ def merge_level(T_in, h_in, tables: MergeTables, keep_factors=False, want_T=True
     upd = np.ascontiguousarray(tables.upd, dtype=np.int32)
     d12 = np.ascontiguousarray(tables.d12_blk, dtype=np.int32)
     d21 = np.ascontiguousarray(tables.d21_blk, dtype=np.int32)
-    rc = lib.hps_merge_level(
+    if not pivot:
+        check(lib.hps_set_merge_pivoting(hd, 0), hd, "hps_set_merge_pivoting")
+    try:
+        rc = _merge_level_call(lib, hd, dt, kind, nm, nc, nec, ni, ne, nsrc, T_in, h_in, rows, cols, ext, cols_final,
+                               final_of_internal, upd, d12, d21, S, T, ho, gt, F_lu, perm, dinv, info, D12, D21, B,
+                               ws)
+    finally:
+        if not pivot:
+            lib.hps_set_merge_pivoting(hd, 1)
+    check(rc, hd, "hps_merge_level")
+    return MergeLevel(S, T, ho, gt, F_lu, perm, dinv, info, D12, D21, B, tables)
+
+
+def merge_factors_stable(F_lu, bound=64.0):
+    """Device bool: the natural-order (unpivoted) LU factors of a merge level
+    are usable -- every entry finite, no zero pivot, every multiplier at most
+    `bound` in magnitude (partial pivoting would guarantee 1; the merge
+    matrices of uniform trees give values below 4)."""
+    a = F_lu.abs()
+    finite = torch.isfinite(a.amax(dim=(-2, -1)))
+    mult = torch.tril(a, -1).amax(dim=(-2, -1))
+    dmin = a.diagonal(dim1=-2, dim2=-1).amin(dim=-1)
+    return (finite & (mult <= bound) & (dmin > 0)).all()
+
+
+def _merge_level_call(lib, hd, dt, kind, nm, nc, nec, ni, ne, nsrc, T_in, h_in, rows, cols, ext, cols_final,
+                      final_of_internal, upd, d12, d21, S, T, ho, gt, F_lu, perm, dinv, info, D12, D21, B, ws):
+    return lib.hps_merge_level(
         hd, dtype_code(dt), kind, nm, nc, nec, ni, ne, nsrc,
         ptr(T_in), ptr(h_in), ptr(rows), ptr(cols), ptr(ext), ptr(cols_final), ptr(final_of_internal),
         upd.shape[0], upd.ctypes.data_as(ctypes.c_void_p),
@@ -243,8 +272,6 @@ def merge_level(T_in, h_in, tables: MergeTables, keep_factors=False, want_T=True
         ptr(S), ptr(T), ptr(ho), ptr(gt), ptr(F_lu), ptr(perm), ptr(dinv), ptr(info),
         ptr(D12), ptr(D21), ptr(B), ptr(ws), ws.numel(),
     )
-    check(rc, hd, "hps_merge_level")
-    return MergeLevel(S, T, ho, gt, F_lu, perm, dinv, info, D12, D21, B, tables)
 
 
 def down_level(S, gt, g_in, tables: MergeTables):

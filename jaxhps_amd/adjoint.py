@@ -23,8 +23,12 @@ through the factors the build kept (nothing is refactorised):
      hbar_int = -D^{-H}(gtbar + B^H hbar_out), scattered to the children
   4. vbar = G + Q^H hbar_leaf, lambda = (leaf operator)^{-H} vbar.
 
-The factors stay resident in HBM (288 GB per GPU holds every level's for the
-configurations here), so no merge intermediate is recomputed.
+By default every factor stays resident in HBM between the forward solve and
+the backward pass (288 GB per GPU holds every level's for the configurations
+here).  With checkpoint=True the forward pass keeps only the leaf stage's
+factors and maps and drops every merge level; the backward pass recomputes
+the merge levels from the stored leaf maps (the same deterministic kernels,
+so the same factors) and frees them again after the adjoint.
 """
 
 from __future__ import annotations
@@ -177,7 +181,7 @@ class _BuildSolve(torch.autograd.Function):
     """u = solve(build_solver(pde with the given coefficient fields), g)."""
 
     @staticmethod
-    def forward(ctx, pde, boundary_data, names, *coefs):
+    def forward(ctx, pde, boundary_data, names, checkpoint, *coefs):
         from ._build_solver import build_solver
         from ._solve import solve
 
@@ -186,24 +190,42 @@ class _BuildSolve(torch.autograd.Function):
             setattr(pde, f"{nm}_coefficients", c.detach())
         build_solver(pde)
         u = solve(pde, boundary_data)
+        if checkpoint:
+            # keep the leaf factors and maps; the merge levels are recomputed in backward
+            pde._factors["levels"] = None
+            pde.S_lst = []
+            pde.g_tilde_lst = []
         ctx.pde = pde
         ctx.names = names
+        ctx.checkpoint = checkpoint
         ctx.save_for_backward(u)
         return u
 
     @staticmethod
     def backward(ctx, grad_u):
         (u,) = ctx.saved_tensors
-        lam = adjoint_lambda(ctx.pde, grad_u)
-        grads = coefficient_gradients(ctx.pde, u, lam)
-        return (None, None, None) + tuple(grads[nm] for nm in ctx.names)
+        pde = ctx.pde
+        if ctx.checkpoint:
+            from ._build_solver import _merge_levels
+
+            leaf = pde._factors["leaf"]
+            levels, _ = _merge_levels(pde, leaf.T, leaf.h, False)
+            pde._factors["levels"] = levels
+        lam = adjoint_lambda(pde, grad_u)
+        grads = coefficient_gradients(pde, u, lam)
+        if ctx.checkpoint:
+            pde._factors["levels"] = None
+        return (None, None, None, None) + tuple(grads[nm] for nm in ctx.names)
 
 
-def differentiable_solve(pde_problem, boundary_data, **coefficients):
+def differentiable_solve(pde_problem, boundary_data, checkpoint: bool = False, **coefficients):
     """Build the solver with the given coefficient fields (e.g. I=tensor)
     and solve; the result is differentiable with respect to those fields
     through the hand-written adjoint.  Coefficient tensors must be real
-    (n_leaves, p^d) tensors; other fields come from pde_problem."""
+    (n_leaves, p^d) tensors; other fields come from pde_problem.
+    checkpoint=True keeps only the leaf factors between forward and backward
+    and recomputes the merge levels in backward (less memory, one more merge
+    stage per gradient)."""
     names = tuple(coefficients)
     for nm in names:
         if getattr(pde_problem, f"{nm}_coefficients", None) is None and coefficients[nm] is None:
@@ -211,4 +233,4 @@ def differentiable_solve(pde_problem, boundary_data, **coefficients):
         if coefficients[nm].is_complex():
             raise ValueError("differentiable coefficient fields must be real")
     coefs = [coefficients[nm] for nm in names]
-    return _BuildSolve.apply(pde_problem, boundary_data, names, *coefs)
+    return _BuildSolve.apply(pde_problem, boundary_data, names, bool(checkpoint), *coefs)

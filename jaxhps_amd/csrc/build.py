@@ -17,7 +17,10 @@ PKG = HERE.parent
 REPO = PKG.parent
 LIB = PKG / "libhps_mi355x.so"
 BUILD = REPO / "_build"
-SOURCES = ["gemm.hip", "lu.hip", "hps_capi.hip", "refine.hip", "mlp.hip", "interp.hip", "scatter.hip", "adaptive.hip", "hps_prof.hip"]
+SOURCES = ["gemm.hip", "hps_capi.hip", "refine.hip", "mlp.hip", "interp.hip", "scatter.hip", "adaptive.hip", "hps_prof.hip"]
+# the LU kernels take most of the compile time: one object per element type,
+# compiled in parallel (lu.hip's HPS_LU_TYPES selects the instantiations)
+VARIANTS = [("lu.hip", f"lu_{t}", [f"-DHPS_LU_TYPES={bit}"]) for t, bit in (("f64", 1), ("f32", 2), ("c128", 4), ("c64", 8))]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
@@ -35,13 +38,13 @@ FLAGS = [
 ]
 
 
-def _compile(src: str, incremental: bool = False) -> Path:
-    obj = BUILD / (Path(src).stem + ".o")
+def _compile(src: str, incremental: bool = False, stem: str = None, defines=()) -> Path:
+    obj = BUILD / ((stem or Path(src).stem) + ".o")
     if incremental and obj.exists():
         deps = [HERE / src, *HERE.glob("*.h"), *(REPO / "include").glob("*.h")]
         if obj.stat().st_mtime > max(d.stat().st_mtime for d in deps):
             return obj
-    cmd = [HIPCC, *FLAGS, "-c", str(HERE / src), "-o", str(obj)]
+    cmd = [HIPCC, *FLAGS, *defines, "-c", str(HERE / src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-6000:]}")
@@ -52,8 +55,9 @@ def build(verbose: bool = True, incremental: bool = False) -> Path:
     """Compile every source (incremental=True keeps objects newer than their
     source and every header) and link the library."""
     BUILD.mkdir(exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
-        objs = list(ex.map(lambda f: _compile(f, incremental), SOURCES))
+    jobs = [(f, None, ()) for f in SOURCES] + VARIANTS
+    with cf.ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 8)) as ex:
+        objs = list(ex.map(lambda j: _compile(j[0], incremental, j[1], j[2]), jobs))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
     r = subprocess.run(cmd, capture_output=True, text=True)

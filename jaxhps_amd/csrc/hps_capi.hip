@@ -238,6 +238,12 @@ int hps_create(void* stream, hps_handle_t* out) {
 
 size_t hps_static_scratch_bytes(void) { return HPS_STATIC_SCRATCH_BYTES; }
 
+int hps_set_merge_pivoting(hps_handle_t h, int partial_pivoting) {
+  if (!h) return HPS_ERR_ARG;
+  h->merge_nopiv = partial_pivoting == 0;
+  return HPS_OK;
+}
+
 int hps_destroy(hps_handle_t h) {
   delete h;
   return HPS_OK;
@@ -686,7 +692,7 @@ static int merge_t(hps_handle_t h, int kind, int nm, int nc, int nec, int ni, in
     hipLaunchKernelGGL((add_identity_kernel<T>), dim3(ceil_div(nF, 256), nm), dim3(256), 0, s, Fl, nF, sNF2, nm);
     HPS_CHECK_HIP(h, hipGetLastError());
   }
-  HPS_CHECK_HIP(h, getrf_ws<T>(nF, Fl, nF, sNF2, perm, dinv, scratch, info, nm, s));
+  HPS_CHECK_HIP(h, getrf_ws<T>(nF, Fl, nF, sNF2, perm, dinv, scratch, info, nm, s, h->merge_nopiv));
 
   // ---- solves: X = -D^{-1} [C | h_int] -------------------------------------
   // one pass for the C columns (width ne) and one for h (width nsrc)

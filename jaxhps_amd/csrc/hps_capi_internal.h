@@ -34,6 +34,7 @@ inline dim3 grid_for(long total, int batch) {
 struct hps_handle_s {
   hipStream_t stream = nullptr;
   int device = -1;  // the stream's device at creation
+  bool merge_nopiv = false;  // hps_merge_level factors without row exchanges
   std::string err;
 };
 

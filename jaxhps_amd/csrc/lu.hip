@@ -36,6 +36,10 @@ namespace hps {
     if (_e != hipSuccess) return _e;    \
   } while (0)
 
+#ifndef HPS_LU_TYPES
+#define HPS_LU_TYPES 15
+#endif
+
 constexpr int LU_NB = 32;
 constexpr size_t LU_PANEL_LDS_MAX = 150 * 1024;
 constexpr int SWTAB = 1 + 4 * LU_NB;  // count, then up to 2*NB (row, src) int pairs
@@ -57,12 +61,12 @@ constexpr long LU_TSLU_MAX_WORK = 64L * 1024;
 // K = the block width.
 static int lu_outer_block(int n) { return n > LU_BLOCKED_MIN ? LU_NBO : (n >= 256 ? 128 : n); }
 
-__global__ void fill_int_kernel(int* p, long n, int v) {
+static __global__ void fill_int_kernel(int* p, long n, int v) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
 
-__global__ void iota_kernel(int* perm, int n, long total) {
+static __global__ void iota_kernel(int* perm, int n, long total) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < total) perm[i] = (int)(i % n);
 }
@@ -176,7 +180,7 @@ __device__ __forceinline__ void record_pivot(int k, int j, int fi, bool zero, in
 
 // perm = the composition of all row swaps: one workgroup per matrix, the
 // permutation built in LDS.
-__global__ __launch_bounds__(256) void perm_from_ipiv_kernel(int n, const int* ipiv, int* perm) {
+static __global__ __launch_bounds__(256) void perm_from_ipiv_kernel(int n, const int* ipiv, int* perm) {
   extern __shared__ int sp[];
   const int b = blockIdx.x;
   const int* ib = ipiv + (long)b * n;
@@ -1056,6 +1060,89 @@ static int rows_panel_max() {
   return sizeof(T) >= 16 ? 512 : 1024;
 }
 
+// ---------------------------------------------------------------------------
+// Tall panels without pivoting, for matrices whose elimination is stable in
+// the natural order (the merge matrices of the HPS tree: the ItI W = I -
+// D12 D21 and the DtN interface blocks show pivot growth ~1 and multipliers
+// below 4; the caller checks the factors and refactors with pivoting
+// otherwise).  Every workgroup factors the nb x nb diagonal block in LDS
+// (small; redundant work costs less than a second launch) and inverts its
+// U11; its 256 panel rows below the block then take L21 = A21 inv(U11), one
+// row per thread, each entry an independent dot product over the row held in
+// registers.  Workgroup 0 writes the factored block, its packed inverse
+// (dinv) and the identity pivots.  The rows are independent: grid
+// (ceil((n - k - nb) / 256), batch).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void getrf_panel_nopiv_kernel(int n, T* A, long lda, long sA, int k, int nb,
+                                                                int* ipiv, T* dinv) {
+  __shared__ T D[LU_NB][LU_NB + 1];
+  __shared__ T W[LU_NB][LU_NB + 1];  // multipliers, then inv(U11)
+  __shared__ T P[LU_NB][LU_NB + 1];  // packed inverse
+  const int b = blockIdx.y;
+  const int tid = threadIdx.x;
+  T* Ab = A + (long)b * sA;
+  for (int idx = tid; idx < LU_NB * LU_NB; idx += 256) {
+    const int r = idx / LU_NB, c = idx - r * LU_NB;
+    D[r][c] = (r < nb && c < nb) ? Ab[(long)(k + r) * lda + k + c] : (r == c ? s_one<T>() : s_zero<T>());
+  }
+  __syncthreads();
+  // elimination step j: rows r > j, columns c >= j; column j is only read in
+  // its own step, so its multipliers go to W and one barrier per step suffices
+  for (int j = 0; j < LU_NB - 1; ++j) {
+    const int w = LU_NB - j;
+    for (int idx = tid; idx < (w - 1) * w; idx += 256) {
+      const int r = j + 1 + idx / w, c = j + idx % w;
+      const T l = s_mul(D[r][j], s_inv(D[j][j]));
+      if (c == j)
+        W[r][j] = l;
+      else
+        D[r][c] = s_fnms(D[r][c], l, D[j][c]);
+    }
+    __syncthreads();
+  }
+  // the factored block (zero outside nb x nb for the inversion)
+  for (int idx = tid; idx < LU_NB * LU_NB; idx += 256) {
+    const int r = idx / LU_NB, c = idx - r * LU_NB;
+    T v = c < r ? W[r][c] : D[r][c];
+    D[r][c] = (r < nb && c < nb) ? v : s_zero<T>();
+  }
+  __syncthreads();
+  invert_diag_block<T>(&D[0][0], LU_NB + 1, &P[0][0], LU_NB + 1, nb);
+  __syncthreads();
+  for (int idx = tid; idx < LU_NB * LU_NB; idx += 256) {
+    const int r = idx / LU_NB, c = idx - r * LU_NB;
+    W[r][c] = (r <= c && c < nb) ? P[r][c] : s_zero<T>();  // P holds garbage beyond nb
+  }
+  __syncthreads();
+  // panel rows below the block: x = a inv(U11)
+  const int rg = k + nb + blockIdx.x * 256 + tid;
+  if (rg < n) {
+    T a[LU_NB];
+    T* rowp = Ab + (long)rg * lda + k;
+#pragma unroll
+    for (int i = 0; i < LU_NB; ++i) a[i] = i < nb ? rowp[i] : s_zero<T>();
+    for (int j = 0; j < nb; ++j) {
+      T acc = s_zero<T>();
+#pragma unroll
+      for (int i = 0; i < LU_NB; ++i) acc = s_fma(a[i], W[i][j], acc);
+      rowp[j] = acc;
+    }
+  }
+  if (blockIdx.x != 0) return;
+  for (int idx = tid; idx < nb * nb; idx += 256) {
+    const int rr = idx / nb, c = idx - rr * nb;
+    Ab[(long)(k + rr) * lda + k + c] = D[rr][c];
+  }
+  if (tid < nb) ipiv[(long)b * n + k + tid] = k + tid;
+  const int nblk = (n + LU_NB - 1) / LU_NB;
+  T* db = dinv + ((long)b * nblk + k / LU_NB) * LU_NB * LU_NB;
+  for (int idx = tid; idx < LU_NB * LU_NB; idx += 256) {
+    const int r = idx / LU_NB, c = idx - r * LU_NB;
+    db[idx] = (r < nb && c < nb) ? P[r][c] : s_zero<T>();
+  }
+}
+
 // *swapped: the panel kernel already applied its row permutation to the other
 // columns (the register panels do).
 template <typename T>
@@ -1123,9 +1210,11 @@ static hipError_t launch_panel(int n, T* A, long lda, long sA, int k, int nb, in
 
 // tournament-pivoting candidate lists: two buffers of batch x chunks x NB row indices
 
+#if HPS_LU_TYPES & 1
 size_t getrf_scratch_bytes(int n, int batch) {
   return ((size_t)batch * n + (size_t)batch * SWTAB + 2 * tslu_cand_ints(n, batch)) * sizeof(int);
 }
+#endif
 
 // X[k:k+nb, :] = tri(Dinv_k) X[k:k+nb, :] in place; tri = unit lower (forward)
 // or upper (backward).  With CT the conjugate transpose of the other stored
@@ -1206,7 +1295,7 @@ static hipError_t trsm_block(bool upper, int n, int ncols, const T* dinv, T* X, 
 
 template <typename T>
 hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, T* dinv, int* scratch, int* info, int batch,
-                    hipStream_t s) {
+                    hipStream_t s, bool nopiv) {
   if (n <= 0 || batch <= 0) return hipSuccess;
   int* ipiv = scratch;
   int* swtab = scratch + (long)batch * n;
@@ -1244,7 +1333,23 @@ hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, T* dinv, int* scr
     for (int k = k0; k < bend; k += LU_NB) {
       const int nb = bend - k < LU_NB ? bend - k : LU_NB;
       bool swapped = false;
-      hipError_t e = launch_panel<T>(n, A, lda, sA, k, nb, ipiv, info, dinv, swtab, cand, batch, s, &swapped);
+      hipError_t e = hipSuccess;
+      if (nopiv && n - k > rows_panel_max<T>()) {
+        static const std::string nm_np = std::string("getrf_panel_nopiv<") + tname<T>() + ">";
+        const int m = n - k;
+        prof::Scope ps(s, nm_np.c_str(), flop_mult<T>() * batch * ((double)m * nb * nb - (double)nb * nb * nb / 3.0),
+                       2.0 * sizeof(T) * (double)m * nb * batch);
+        for (int b0 = 0; b0 < batch; b0 += 65535) {
+          const int bc = batch - b0 < 65535 ? batch - b0 : 65535;
+          hipLaunchKernelGGL((getrf_panel_nopiv_kernel<T>), dim3(ceil_div(m - nb, 256) > 0 ? ceil_div(m - nb, 256) : 1, bc),
+                             dim3(256), 0, s, n, A + (long)b0 * sA, lda, sA, k, nb, ipiv + (long)b0 * n,
+                             dinv + (long)b0 * ((n + LU_NB - 1) / LU_NB) * LU_NB * LU_NB);
+        }
+        e = hipGetLastError();
+        swapped = true;  // nothing to swap: U12 is formed below like after a register panel
+      } else {
+        e = launch_panel<T>(n, A, lda, sA, k, nb, ipiv, info, dinv, swtab, cand, batch, s, &swapped);
+      }
       if (e != hipSuccess) return e;
       const int others = n - nb;
       if (swapped) {
@@ -1302,9 +1407,11 @@ __global__ void gather_rows_kernel(int n, int nrhs, const int* perm, const T* R,
 // Small systems with few right-hand sides (the leaf solves of the up pass,
 // refinement and adjoint): one workgroup per matrix keeps X (n x nrhs) in LDS
 // and streams the factors once, 32-row block by block: the diagonal block
-// through its stored inverse, the rows below (above) by per-thread dot
-// products over the block's 32 columns.  CT solves with A^H instead
-// (U^H forward, L^H backward, then X = P^T Z).
+// through its stored inverse, the rows below (above) by dot products over the
+// block's 32 columns, each row segment read once by four lanes for all the
+// right-hand sides (offblock_rows).  CT solves with A^H instead (U^H forward,
+// L^H backward, then X = P^T Z), whose column reads are coalesced across
+// threads already.
 constexpr int LU_SMALL_MAX_N = 1024;
 constexpr int LU_SMALL_MAX_RHS = 16;
 
@@ -1418,7 +1525,7 @@ __global__ __launch_bounds__(256) void getrs_small_kernel(int n, int nrhs, const
       // rows outside the block: below (pass 0) or above (pass 1)
       const int r0 = pass == 0 ? k + nb : 0;
       const int r1 = pass == 0 ? n : k;
-      if (!CT && nrhs <= 2)
+      if (!CT)
         offblock_rows(A + k, lda, r0, r1, nb, nrhs, xs, k, tid);
       else
       for (int i = r0 + tid; i < r1; i += 256) {
@@ -1501,7 +1608,7 @@ __global__ __launch_bounds__(256) void getrs_block_kernel(int n, int nrhs, const
     __syncthreads();
     const int r0 = UPPER ? 0 : kl + nb;
     const int r1 = UPPER ? kl : nbk;
-    if (nrhs <= 4)
+    if (nrhs <= LU_SMALL_MAX_RHS)
       offblock_rows(A + (long)k0 * lda + k0 + kl, lda, r0, r1, nb, nrhs, xs, kl, tid);
     else
     for (int i = r0 + tid; i < r1; i += 256) {
@@ -1738,17 +1845,31 @@ hipError_t getrs_conjt(int n, int nrhs, const T* LU, long lda, long sA, const in
   return hipGetLastError();
 }
 
+#if HPS_LU_TYPES & 1
 size_t dinv_elems(int n) { return (size_t)((n + LU_NB - 1) / LU_NB) * LU_NB * LU_NB; }
+#endif
 
 #define HPS_INST_LU(T)                                                                                          \
-  template hipError_t getrf_ws<T>(int, T*, long, long, int*, T*, int*, int*, int, hipStream_t);                 \
+  template hipError_t getrf_ws<T>(int, T*, long, long, int*, T*, int*, int*, int, hipStream_t, bool);                 \
   template hipError_t getrs_perm<T>(int, int, const T*, long, long, const int*, const T*, const T*, long, long, \
                                     T*, long, long, int, hipStream_t);                                          \
   template hipError_t getrs_conjt<T>(int, int, const T*, long, long, const int*, const T*, const T*, long, long, \
                                      T*, long, long, T*, int, hipStream_t);
+// HPS_LU_TYPES selects the element types this object instantiates (bit 0
+// double, 1 float, 2 c128, 3 c64; default all): the build compiles one
+// object per type in parallel.  The type-independent functions live in the
+// object that holds double.
+#if HPS_LU_TYPES & 1
 HPS_INST_LU(double)
+#endif
+#if HPS_LU_TYPES & 2
 HPS_INST_LU(float)
+#endif
+#if HPS_LU_TYPES & 4
 HPS_INST_LU(c128)
+#endif
+#if HPS_LU_TYPES & 8
 HPS_INST_LU(c64)
+#endif
 
 }  // namespace hps

@@ -12,42 +12,57 @@ from typing import List
 import torch
 
 from ._lib import raise_on_info
-from ._stages import cast, merge_level
+from ._stages import cast, merge_factors_stable, merge_level
 from ._tables import oct_merge_tables, quad_merge_tables
 
 
-def _levels(T_arr, h_arr, l, kind, nc, table_fn, keep_B=False, root_T=True, dtypes=None):
+def _levels(T_arr, h_arr, l, kind, nc, table_fn, keep_B=False, root_T=True, dtypes=None, pivot=False):
     """Run max(l-1, 0) batched levels plus the root merge, as the reference's
     merge stages do (``merge/_uniform_2D_DtN.py:90-203``).  root_T=False skips
     forming the root's T and h (only needed when the caller asks for them).
     dtypes[i], when given, is the precision level i merges in (the children's
-    T and h are converted on entry)."""
+    T and h are converted on entry).
+
+    The tall panels of the interface matrices (the top levels') are factored
+    without row exchanges (pivot=False): these matrices eliminate stably in
+    their natural order, and pivoted panels that tall are latency-bound.  The factors of every
+    level are checked at the end (one synchronisation); from the first level
+    that fails the check the merges are redone with partial pivoting."""
     from . import profiling
     nec = T_arr.shape[-1]
-    T = T_arr.reshape(-1, nec, nec)
+    T0 = T_arr.reshape(-1, nec, nec)
     # the reference's multi-source test (merge/_uniform_2D_DtN.py:73)
     multi = h_arr is not None and h_arr.ndim == T_arr.ndim
     if h_arr is None:
-        h = None
+        h0 = None
     else:
-        h = h_arr.reshape(T.shape[0], nec, -1) if multi else h_arr.reshape(T.shape[0], nec, 1)
-    out = []
+        h0 = h_arr.reshape(T0.shape[0], nec, -1) if multi else h_arr.reshape(T0.shape[0], nec, 1)
     n_levels = max(l - 1, 0) + 1
-    for lvl in range(n_levels):
-        if lvl == n_levels - 1:
-            T = T[:nc]
-            h = h[:nc] if h is not None else None
-        if dtypes is not None:
-            T = cast(T, dtypes[lvl])
-            h = cast(h, dtypes[lvl])
-        nec = T.shape[-1]
-        tb = table_fn(nec)
-        want_T = root_T or lvl < n_levels - 1
-        with profiling.stage(f"merge_level_{lvl + 1}"):
-            res = merge_level(T.contiguous(), h.contiguous() if h is not None else None, tb, keep_factors=keep_B,
-                              want_T=want_T)
-        out.append(res)
-        T, h = res.T, res.h
+
+    def run(start, T, h, out, piv):
+        for lvl in range(start, n_levels):
+            if lvl == n_levels - 1:
+                T = T[:nc]
+                h = h[:nc] if h is not None else None
+            if dtypes is not None:
+                T = cast(T, dtypes[lvl])
+                h = cast(h, dtypes[lvl])
+            tb = table_fn(T.shape[-1])
+            want_T = root_T or lvl < n_levels - 1
+            with profiling.stage(f"merge_level_{lvl + 1}"):
+                res = merge_level(T.contiguous(), h.contiguous() if h is not None else None, tb,
+                                  keep_factors=keep_B, want_T=want_T, pivot=piv)
+            out.append(res)
+            T, h = res.T, res.h
+        return out
+
+    out = run(0, T0, h0, [], pivot)
+    if not pivot:
+        ok = torch.stack([merge_factors_stable(r.F_lu) for r in out]).cpu()
+        if not bool(ok.all()):
+            first = int((~ok).nonzero()[0, 0])
+            T, h = (T0, h0) if first == 0 else (out[first - 1].T, out[first - 1].h)
+            out = run(first, T, h, out[:first], True)
     return out, multi
 
 

@@ -285,3 +285,38 @@ def test_nosource_dtn_build_then_source_solve(gpu):
     g = np.random.default_rng(7).standard_normal((dom.boundary_points.shape[0], 2))
     u = _np(jh.solve(pde, g, source=pde_src.source))
     assert _rel(u, _np(jh.solve(pde_src, g))) < 1e-11
+
+
+def test_unpivoted_merge_falls_back_to_pivoting(gpu):
+    """A DtN merge whose interface matrix (1120 unknowns) has an exactly zero
+    leading entry (nonsingular otherwise) breaks the unpivoted factorisation; the factor
+    check must catch it and the level be redone with partial pivoting, giving
+    the oracle's result.  The regular merge matrices of the workloads take
+    the unpivoted path (every other parity test covers it)."""
+    from jaxhps_amd._stages import merge_factors_stable
+    from jaxhps_amd.merge import _levels, merge_stage_uniform_2D_DtN
+    from jaxhps_amd._tables import quad_merge_tables
+
+    print("unpivoted merge fallback", flush=True)
+    m = 280  # 1120 interface unknowns: taller than the register panels, so unpivoted
+    ne = 4 * m
+    rng = np.random.default_rng(21)
+    T = 2.0 * np.eye(ne) + rng.standard_normal((4, ne, ne)) / np.sqrt(ne)
+    h = rng.standard_normal((4, ne, 1))
+    # D[0, 0] = a55[0, 0] + b55[0, 0]: a's interface 5 is its E side (rows m..2m),
+    # b's is its W side, reversed (rows 4m-1 .. 3m)
+    T[1, ne - 1, ne - 1] = -T[0, m, m]
+    S0, g0, Tt0, _ = hps2d.merge_stage(T, h, 1, "dtn")
+    Tg = torch.as_tensor(T, device=gpu)
+    hg = torch.as_tensor(h, device=gpu)
+    # unpivoted factors of this level fail the check ...
+    levels, _ = _levels(Tg, hg, 1, "dtn", 4, lambda nec: quad_merge_tables(nec // 4, "dtn"), pivot=True)
+    assert bool(merge_factors_stable(levels[0].F_lu))
+    from jaxhps_amd._stages import merge_level
+
+    raw = merge_level(Tg.contiguous(), hg.contiguous(), quad_merge_tables(m, "dtn"), pivot=False)
+    assert not bool(merge_factors_stable(raw.F_lu))
+    # ... and the merge stage redoes it with pivoting
+    S_lst, g_lst, Ttop = merge_stage_uniform_2D_DtN(Tg, hg, 1, return_T=True)
+    assert _rel(_np(S_lst[0]), S0[0]) < 1e-11
+    assert _rel(_np(Ttop), Tt0) < 1e-11

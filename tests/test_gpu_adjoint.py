@@ -141,3 +141,31 @@ def test_dtn_real_gradient_matches_central_fd(gpu):
     with torch.no_grad():
         fd = (float(J(c0 + eps * dc, a0 + eps * da)) - float(J(c0 - eps * dc, a0 - eps * da))) / (2 * eps)
     assert abs(dirderiv - fd) <= 1e-6 * abs(fd), (dirderiv, fd)
+
+
+def test_checkpointed_adjoint_equals_stored_factors(gpu):
+    """checkpoint=True drops the merge levels after the forward solve and
+    recomputes them in backward: the gradient is the one of the stored-factor
+    adjoint (config E's problem at p = 16, k = 40, L = 3, 25 sine terms), and
+    no merge level is held between forward and backward."""
+    from jaxhps_amd import workloads as wl
+    import jaxhps_amd as jh
+    from jaxhps_amd.adjoint import differentiable_solve
+
+    print("checkpointed adjoint", flush=True)
+    k = 40.0
+    dom, kw, g, basis, thetas, theta_star = wl.config_E(L=3, p=16, k=k)
+    flat = torch.as_tensor(basis, device=gpu).reshape(basis.shape[0], -1)
+    shape = basis.shape[1:]
+    g_dev = torch.as_tensor(g, device=gpu)
+    u_obs = torch.zeros(shape, dtype=torch.complex128, device=gpu)
+    grads = []
+    for ckpt in (False, True):
+        pde = jh.PDEProblem(dom, I_coefficients=np.ones(shape), **kw)
+        theta = torch.tensor(thetas[1], device=gpu, requires_grad=True)
+        u = differentiable_solve(pde, g_dev, checkpoint=ckpt, I=k**2 * (1.0 - (theta @ flat).reshape(shape)))
+        if ckpt:
+            assert pde._factors["levels"] is None and pde.S_lst == []
+        ((u - u_obs).abs() ** 2).sum().backward()
+        grads.append(_np(theta.grad))
+    assert np.abs(grads[1] - grads[0]).max() <= 1e-12 * np.abs(grads[0]).max()

@@ -170,3 +170,29 @@ def test_fp64_solution_has_zero_residual(gpu):
                 assert np.abs(d[0][l]).max() == 0.0, l
         if not pde.use_ItI:
             assert np.abs(d[2]).max() == 0.0
+
+
+def test_refinement_stop_estimate_matches_the_next_update(gpu):
+    """The refinement stops on an estimate of the next update (the last update
+    times the residual's reduction).  Running the same solve with a tighter
+    tolerance computes that update: the estimate must be within a factor of
+    ten of it (config C's workload, k = 100, L = 4, complex64 leaves and
+    merges)."""
+    import jaxhps_amd as jh
+    from jaxhps_amd import workloads as wl
+
+    print("refinement estimate vs actual update", flush=True)
+    dom, kw, g = wl.config_C(L=4)
+    pde = jh.PDEProblem(dom, **kw)
+    jh.build_solver(pde, precision="mixed")
+    jh.solve(pde, g, refine_tol=1e-10)
+    h1 = list(pde.refine_history)
+    assert pde.refine_converged and "next_update_estimate" in h1[-1]
+    est, n1 = h1[-1]["next_update_estimate"], len(h1)
+    jh.solve(pde, g, refine_tol=1e-15, max_refine=n1 + 1)
+    h2 = pde.refine_history
+    assert len(h2) > n1
+    for a, b in zip(h1, h2):
+        assert a["rel_update"] == b["rel_update"]  # the same iterations
+    actual = h2[n1]["rel_update"]
+    assert est / 10 <= actual <= est * 10, (est, actual)

@@ -156,3 +156,39 @@ def test_getrs_conjt_solves_adjoint_system(gpu, dtype, n, nrhs, batch):
     X = getrs_conjt(n, LU, 0, n, n * n, perm, dinv, R.to(gpu).contiguous())
     res = _ref64(A).conj().transpose(1, 2) @ _ref64(X) - _ref64(R)
     assert res.abs().max().item() < TOL[dtype] * 50 * n**0.5
+
+
+@pytest.mark.parametrize("kind,dtype", [("iti", torch.complex128), ("iti", torch.complex64),
+                                        ("dtn", torch.float64), ("dtn", torch.float32)])
+def test_unpivoted_tall_merge_matches_pivoted(gpu, kind, dtype):
+    """A merge level whose interface matrix is tall enough for the unpivoted
+    panel kernel (1792 unknowns, children with 448 Gauss nodes per side) on
+    well-conditioned synthetic maps: the unpivoted factors pass the check
+    and S, T and g_tilde equal the partially pivoted merge's.  (The panels
+    of its last 1024 (complex128: 512) rows are the pivoted register panels:
+    the factorisation is unpivoted only where the tournament panels were.)"""
+    from jaxhps_amd._stages import merge_factors_stable, merge_level
+    from jaxhps_amd._tables import quad_merge_tables
+
+    print(f"unpivoted tall merge {kind} {dtype}", flush=True)
+    m = 448
+    ne = 4 * m
+    g = torch.Generator(device=gpu).manual_seed(3)
+    real = torch.float64
+    X = torch.randn(4, ne, ne, generator=g, device=gpu, dtype=real) / ne**0.5
+    if kind == "iti":
+        # contractive children: impedance maps of lossless problems are unitary
+        Y = torch.randn(4, ne, ne, generator=g, device=gpu, dtype=real) / ne**0.5
+        Tc = 0.5 * torch.complex(X, Y)
+    else:
+        Tc = 2.0 * torch.eye(ne, device=gpu, dtype=real) + 0.1 * X
+    T = Tc.to(dtype).contiguous()
+    h = torch.randn(4, ne, 2, generator=g, device=gpu, dtype=real).to(dtype).contiguous()
+    tb = quad_merge_tables(m, kind)
+    ref = merge_level(T, h, tb, pivot=True)
+    out = merge_level(T, h, tb, pivot=False)
+    assert bool(merge_factors_stable(out.F_lu))
+    tol = 1e-11 if dtype in (torch.float64, torch.complex128) else 2e-4
+    for a, b in ((out.S, ref.S), (out.T, ref.T), (out.gt, ref.gt), (out.h, ref.h)):
+        err = float((a - b).abs().max() / b.abs().max())
+        assert err < tol, err

@@ -295,3 +295,26 @@ def test_gemm_wrapper_rejects_mismatched_operands():
         _lib.gemm(A, torch.zeros(3, 5, 6, dtype=torch.float64), C=torch.zeros(2, 4, 6, dtype=torch.float64))
     with pytest.raises(ValueError):
         _lib.gemm(A, torch.zeros(3, 5, 6, dtype=torch.float32))
+
+
+def test_unpivoted_factor_check():
+    """merge_factors_stable accepts natural-order LU factors with small
+    multipliers and rejects a zero pivot, a large multiplier or a
+    non-finite entry (CPU tensors; the check is plain torch)."""
+    import torch
+
+    from jaxhps_amd._stages import merge_factors_stable
+
+    rng = np.random.default_rng(0)
+    A = np.eye(6) + 0.1 * rng.standard_normal((6, 6))
+    Lu = torch.as_tensor(np.stack([A, A]))
+    assert bool(merge_factors_stable(Lu))
+    bad = Lu.clone()
+    bad[1, 3, 3] = 0.0
+    assert not bool(merge_factors_stable(bad))
+    bad = Lu.clone()
+    bad[0, 4, 1] = 100.0
+    assert not bool(merge_factors_stable(bad))
+    bad = Lu.clone().to(torch.complex128)
+    bad[0, 2, 5] = complex("nan")
+    assert not bool(merge_factors_stable(bad))
