@@ -58,6 +58,8 @@ def parse():
                     help="refinement stops once the (measured or contraction-estimated) relative update is below "
                          "this; 1e-10 leaves a 100x margin to the 1e-8 bar, and the line reports the measured error")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint grads/s measurement")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="skip the surrogate-in-the-gradient-loop measurement of the adjoint line")
     ap.add_argument("--adjoint-L", type=int, default=6, help="tree depth of the adjoint workload (testing only)")
     ap.add_argument("--workload", default="hps",
                     choices=["hps", "scattering", "interp", "adaptive", "adaptive3d", "subtree"],
@@ -342,29 +344,147 @@ def adjoint_bench(L=6, warmup=1, reps=2):
             grads.append(th.grad)
         return torch.stack(grads)
 
+    def held_between_forward_and_backward(ckpt):
+        """Device bytes the forward pass leaves allocated for the backward."""
+        pde.reset()
+        th = thetas_t[0].clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        m0 = torch.cuda.memory_allocated(dev)
+        u = differentiable_solve(pde, g_dev, checkpoint=ckpt,
+                                 I=k**2 * (1.0 - sine_field(th, basis_flat).reshape(field_shape)))
+        J = ((u - u_obs).abs() ** 2).sum()
+        torch.cuda.synchronize()
+        held = torch.cuda.memory_allocated(dev) - m0
+        J.backward()
+        return int(held)
+
     res = {}
     for ckpt in (False, True):
         for _ in range(warmup):
             batch(ckpt)
         torch.cuda.synchronize()
-        torch.cuda.reset_peak_memory_stats(dev)
-        base = torch.cuda.memory_allocated(dev)
         t0 = time.perf_counter()
         for _ in range(reps):
             grads = batch(ckpt)
         torch.cuda.synchronize()
         t = (time.perf_counter() - t0) / reps
-        res[ckpt] = (t, grads, int(torch.cuda.max_memory_allocated(dev) - base))
+        res[ckpt] = (t, grads, held_between_forward_and_backward(ckpt))
     t, grads, mem = res[False]
     tc, gc, memc = res[True]
     return {"metric": "adjoint grads/s", "value": thetas.shape[0] / t, "unit": "grads/s",
             "ms_per_grad": t / thetas.shape[0] * 1e3, "batch": int(thetas.shape[0]),
             "config": {"workload": "2D Helmholtz inverse scattering ItI k=40 p=16, 25 sine coefficients",
                        "L": L, "n_leaves": dom.n_leaves},
-            "dtype": "c128", "grad_norm": float(grads.norm()), "peak_device_bytes": mem,
+            "dtype": "c128", "grad_norm": float(grads.norm()), "held_device_bytes": mem,
             "checkpointed": {"value": thetas.shape[0] / tc, "ms_per_grad": tc / thetas.shape[0] * 1e3,
-                             "peak_device_bytes": memc,
+                             "held_device_bytes": memc,
                              "max_rel_diff_vs_stored": float((gc - grads).abs().max() / grads.abs().max())}}
+
+
+def surrogate_loop_bench(L=6, iters=4, trials=3, n_fields=4, train_steps=1000):
+    """The learned leaf-ItI surrogate inside config E's gradient loop.
+
+    Gradient descent on J(theta) = ||u(q_theta) - u_obs||^2 with a line
+    search: each iteration takes one gradient through the plain adjoint,
+    then evaluates J at `trials` step lengths and takes the best, evaluating
+    J either with the plain FP64 build + solve (surrogate off)
+    or with the surrogate forward solve (surrogate on: MLP leaf maps, each
+    checked on a random boundary probe with fallback to the plain leaf solve,
+    complex64 merges, plain interior leaf solves with the shared q = 0
+    factors).  The plain method judges the result: J at the final theta of
+    both loops is evaluated with the plain solve.  Also: objective
+    evaluations per second each way over the 8 seeded thetas and the largest
+    relative difference of J."""
+    import numpy as np
+    import torch
+
+    import jaxhps_amd as jh
+    from jaxhps_amd import workloads as wl
+    from jaxhps_amd.adjoint import differentiable_solve
+    from jaxhps_amd.surrogate import LeafItISurrogate, SharedLeafSolver, leaf_training_data, surrogate_solve
+
+    k = 40.0
+    dev = torch.device("cuda", 0)
+    dom, kw, g, basis, thetas, theta_star = wl.config_E(L=L, p=16, k=k)
+    basis_flat = torch.as_tensor(basis, device=dev).reshape(basis.shape[0], -1).contiguous()
+    shape = basis.shape[1:]
+    g_dev = torch.as_tensor(g, device=dev)
+    dkw = {key: (torch.as_tensor(v, device=dev) if isinstance(v, np.ndarray) else v) for key, v in kw.items()}
+    pde = jh.PDEProblem(dom, I_coefficients=torch.ones(shape, dtype=torch.float64, device=dev), **dkw)
+
+    def I_of(th):
+        return k**2 * (1.0 - (th @ basis_flat).reshape(shape))
+
+    t0 = time.perf_counter()
+    q, T, T0 = leaf_training_data(n_fields=n_fields, L=L, p=16, k=k, seed=1, device=dev)
+    model = LeafItISurrogate(n_in=256, width=512, nq=56, rank=8, seed=0, device=dev)
+    model.fit(q, T, T0, steps=train_steps, batch=256, lr=1e-3)
+    solver = SharedLeafSolver(pde, I_ref=k**2, device=dev)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    stats = {"passed": [], "fallback_leaves": 0}
+
+    with torch.no_grad():
+        u_obs = differentiable_solve(pde, g_dev, I=I_of(torch.as_tensor(theta_star, device=dev))).clone()
+
+    def J_plain(th):
+        with torch.no_grad():
+            u = differentiable_solve(pde, g_dev, I=I_of(th))
+            return float(((u - u_obs).abs() ** 2).sum())
+
+    def J_sur(th):
+        with torch.no_grad():
+            r = surrogate_solve(model, solver, pde, g_dev, I_of(th), k)
+            stats["passed"].append(float(r.passed.double().mean()))
+            stats["fallback_leaves"] += int((~r.passed).sum())
+            return float(((r.u - u_obs).abs() ** 2).sum())
+
+    def grad(th):
+        th = th.clone().requires_grad_(True)
+        u = differentiable_solve(pde, g_dev, I=I_of(th))
+        J = ((u - u_obs).abs() ** 2).sum()
+        J.backward()
+        return float(J.detach()), th.grad.detach()
+
+    thetas_t = torch.as_tensor(thetas, device=dev)
+
+    def loop(evalfn):
+        # each iteration: one gradient, then `trials` step lengths evaluated
+        # and the best one taken
+        th = thetas_t[0].clone()
+        n_eval = 0
+        for _ in range(iters):
+            J0, gr = grad(th)
+            alpha = 0.1 / float(gr.norm())
+            cands = [th - alpha * 0.5**t * gr for t in range(trials)]
+            Js = [evalfn(c) for c in cands]
+            n_eval += trials
+            th = cands[int(np.argmin(Js))]
+        return th, n_eval
+
+    out = {"training_seconds": train_s, "train_samples": int(q.shape[0]), "iters": iters, "trials": trials}
+    for name, fn in (("plain", J_plain), ("surrogate", J_sur)):
+        fn(thetas_t[1])  # warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        Js = [fn(th) for th in thetas_t]
+        torch.cuda.synchronize()
+        out[f"{name}_evals_per_s"] = len(Js) / (time.perf_counter() - t0)
+        out[f"J_{name}"] = Js
+        loop(fn)  # warm
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        th_end, n_eval = loop(fn)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        out[f"{name}_loop"] = {"iterations_per_s": iters / dt, "seconds": dt, "objective_evals": n_eval,
+                               "final_J_plain": J_plain(th_end)}
+    Jp, Js = np.array(out.pop("J_plain")), np.array(out.pop("J_surrogate"))
+    out["max_rel_J_diff"] = float(np.abs(Js - Jp).max() / np.abs(Jp).max())
+    out["mean_surrogate_leaf_fraction"] = float(np.mean(stats["passed"]))
+    out["fallback_leaves_total"] = stats["fallback_leaves"]
+    out["eval_speedup"] = out["surrogate_evals_per_s"] / out["plain_evals_per_s"]
+    return out
 
 
 def _line(metric, unit, value, ms, steps, warmup, dtype, config, roofline, kernels, cpu):
@@ -819,6 +939,9 @@ def main():
         res.clear()
         torch.cuda.empty_cache()
         out["adjoint"] = adjoint_bench(args.adjoint_L)
+        if not args.no_surrogate:
+            torch.cuda.empty_cache()
+            out["adjoint"]["surrogate_loop"] = surrogate_loop_bench(args.adjoint_L)
     ok = True
     if "mixed" in res:
         ok = bool(res["mixed"]["converged"]) and out.get("rel_err_vs_fp64", 0.0) <= 1e-8

@@ -58,11 +58,16 @@ def _levels(T_arr, h_arr, l, kind, nc, table_fn, keep_B=False, root_T=True, dtyp
 
     out = run(0, T0, h0, [], pivot)
     if not pivot:
-        ok = torch.stack([merge_factors_stable(r.F_lu) for r in out]).cpu()
-        if not bool(ok.all()):
-            first = int((~ok).nonzero()[0, 0])
-            T, h = (T0, h0) if first == 0 else (out[first - 1].T, out[first - 1].h)
-            out = run(first, T, h, out[:first], True)
+        # only levels with panels taller than the register panels (1024 rows,
+        # 512 for complex128) were factored without row exchanges
+        checked = [i for i, r in enumerate(out)
+                   if r.F_lu.shape[-1] > (512 if r.F_lu.dtype == torch.complex128 else 1024)]
+        if checked:
+            ok = torch.stack([merge_factors_stable(out[i].F_lu) for i in checked]).cpu()
+            if not bool(ok.all()):
+                first = checked[int((~ok).nonzero()[0, 0])]
+                T, h = (T0, h0) if first == 0 else (out[first - 1].T, out[first - 1].h)
+                out = run(first, T, h, out[:first], True)
     return out, multi
 
 

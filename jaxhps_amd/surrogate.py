@@ -205,3 +205,175 @@ def surrogate_leaf_maps(model: LeafItISurrogate, q, T_plain, tol=1e-2, seed=0):
     passed = err <= tol
     T = torch.where(passed[:, None, None], T_sur, T_plain)
     return SurrogateCheck(T, passed, err)
+
+
+# ---------------------------------------------------------------------------
+# The surrogate inside an optimisation loop
+# ---------------------------------------------------------------------------
+class SharedLeafSolver:
+    """Solves with every ItI leaf matrix of a uniform 2D tree whose leaves
+    differ only in the identity coefficient (the inverse-scattering family
+    I = k^2 (1 - q)): B(q) = B_ref + diag(0, d) with d = I - I_ref on the
+    collocation rows.  B_ref (the same matrix for every leaf: the tree is
+    uniform) is factored once; B(q) x = r is solved by the fixed point
+    x = B_ref^{-1} (r - d x), all leaves and right-hand sides as the columns
+    of one product with the explicit inverse of B_ref per step (formed once
+    from its LU; B_ref is row-scaled, condition ~1e4, so the fixed point
+    differs from B(q)^{-1} r by ~1e-12 relative).  It contracts by about
+    |B_ref^{-1} diag(d)| (about 0.1 for k = 40, p = 16, L = 6, |q| <= 0.5);
+    leaves that do not converge are reported."""
+
+    def __init__(self, pde, I_ref, device=None):
+        from ._stages import default_device
+
+        dom = pde.domain
+        if not (pde.use_ItI and dom.bool_2D and dom.bool_uniform):
+            raise ValueError("SharedLeafSolver needs a uniform 2D ItI problem")
+        self.device = torch.device(device) if device is not None else default_device()
+        n = dom.p**2
+        A = np.zeros((n, n))
+        for nm, arr in pde.coefficient_list():
+            if nm == "I":
+                continue
+            a = np.asarray(arr.detach().cpu() if isinstance(arr, torch.Tensor) else arr)
+            if not np.all(a == a.flat[0]):
+                raise ValueError(f"{nm} coefficients vary: the leaf matrices differ beyond the identity term")
+            A += float(np.real(a.flat[0])) * np.asarray(pde_op(pde, nm))
+        A += float(I_ref) * np.eye(n)
+        G = np.asarray(pde.G)
+        self.nb = G.shape[0]
+        self.s = float(2.0 ** np.round(np.log2(np.abs(A[self.nb:]).max() / np.abs(G).max())))
+        B = np.concatenate([self.s * G, A[self.nb:].astype(complex)])
+        LU = torch.as_tensor(B[None], dtype=torch.complex128).to(self.device).contiguous()
+        perm, dinv, info = _lib.getrf(LU)
+        _lib.raise_on_info(info, "shared leaf matrix")
+        eye = torch.eye(n, dtype=torch.complex128, device=self.device)[None].contiguous()
+        self.Binv = _lib.getrs(LU, perm, dinv, eye)[0].contiguous()
+        self.I_ref = float(I_ref)
+        self.n = n
+
+    def _apply(self, R):
+        """B_ref^{-1} applied to every leaf's columns of R (nl, n, w)."""
+        nl, n, w = R.shape
+        Rt = R.permute(1, 0, 2).reshape(n, nl * w).contiguous()
+        X = _lib.gemm(self.Binv, Rt)[0]
+        return X.reshape(n, nl, w).permute(1, 0, 2).contiguous()
+
+    def solve(self, rhs, I_coef, tol=1e-12, max_iter=60, chunk=4):
+        """x (nl, n, w) with B(q) x = rhs (impedance rows first, unscaled).
+        Returns (x, converged (nl,) bool, iterations)."""
+        nl, n, w = rhs.shape
+        d = (I_coef.to(self.device, torch.float64) - self.I_ref).to(torch.complex128)
+        d = d.clone()
+        d[:, : self.nb] = 0
+        d = d.unsqueeze(-1)
+        r = rhs.to(torch.complex128).clone()
+        r[:, : self.nb] *= self.s
+        x = self._apply(r)
+        it = 0
+        rel = None
+        while it < max_iter:
+            for _ in range(chunk):
+                x_new = self._apply(r - d * x)
+                it += 1
+                diff = (x_new - x).abs().pow(2).sum((1, 2))
+                x = x_new
+            rel = (diff / x.abs().pow(2).sum((1, 2)).clamp_min(1e-300)).sqrt()
+            if float(rel.max()) <= tol:
+                break
+        return x, torch.isfinite(rel) & (rel <= max(tol, 1e-10)), it
+
+
+def pde_op(pde, name):
+    return getattr(pde, name)
+
+
+@dataclass
+class SurrogateSolve:
+    u: torch.Tensor            # (n_leaves, n) complex128
+    passed: torch.Tensor       # (n_leaves,) bool: the surrogate map was used
+    probe_error: torch.Tensor  # (n_leaves,) relative error of the surrogate map on the probe
+    iterations: int            # fixed-point steps of the leaf solves (probe + interior)
+
+
+def surrogate_solve(model: LeafItISurrogate, solver: SharedLeafSolver, pde, boundary_data, I_coef, k,
+                    tol=1e-2, merge_dtype=torch.complex64, seed=0, leaf_tol=1e-9) -> SurrogateSolve:
+    """u for the Helmholtz problem with I = I_coef (n_leaves, n) = k^2 (1 - q)
+    and zero source, with the leaf ItI maps from the surrogate.
+
+    Every surrogate leaf map is checked on a random boundary probe t against
+    the plain leaf solve of the same probe (B(q) x = [P t; 0], T t = QH x,
+    solved with the shared factors); leaves whose relative probe error exceeds
+    tol (or whose solve does not converge) fall back to the plain leaf stage.
+    The merges run in merge_dtype; each leaf's interior comes from its plain
+    solve with the incoming impedance of the down pass.  The plain method
+    judges the end-to-end error (compare with build_solver + solve)."""
+    from . import _tables
+    from ._stages import leaf_solve, to_device
+    from .down_pass import down_levels
+    from .merge import _levels
+
+    dom = pde.domain
+    dev = solver.device
+    I_t = to_device(I_coef, dev, torch.float64)
+    q = 1.0 - I_t / k**2
+    nl, n = I_t.shape
+    ops = pde.device_ops(dev, torch.complex128)
+    P, QH = ops["P"], ops["Q"]
+    nb, nq = P.shape
+    T = model.predict_T(q)
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    t = torch.complex(torch.randn((nl, nq, 1), generator=g, dtype=torch.float64),
+                      torch.randn((nl, nq, 1), generator=g, dtype=torch.float64)).to(dev)
+    rhs = torch.zeros((nl, n, 1), dtype=torch.complex128, device=dev)
+    rhs[:, :nb] = _lib.gemm(P, t)
+    x, conv, it1 = solver.solve(rhs, I_t, tol=leaf_tol)
+    Tt = _lib.gemm(QH, x)
+    err = ((_lib.gemm(T, t) - Tt).abs().pow(2).sum((1, 2)) / Tt.abs().pow(2).sum((1, 2))).sqrt()
+    passed = conv & (err <= tol)
+    fail = (~passed).nonzero().flatten()
+    Y_fail = None
+    if fail.numel():
+        sub = _leaf_subproblem(pde, I_t, fail)
+        res = leaf_solve(sub, 1, torch.complex128, dev, nsrc_zero=True)
+        _lib.raise_on_info(res.info, "surrogate fallback leaves")
+        T = T.clone()
+        T[fail] = res.T
+        Y_fail = res.Y
+    L = dom.L
+    levels, _ = _levels(T.to(merge_dtype).contiguous(), None, L, "iti", 4,
+                        lambda nec: _tables.quad_merge_tables(nec // 4, "iti"), root_T=False)
+    g_root = to_device(boundary_data, dev, merge_dtype).reshape(-1, 1)
+    _, g_leaf = down_levels(g_root, [lv.S for lv in levels], [], None, None)
+    g_leaf = g_leaf.to(torch.complex128)
+    rhs = torch.zeros((nl, n, 1), dtype=torch.complex128, device=dev)
+    rhs[:, :nb] = _lib.gemm(P, g_leaf)
+    u, conv2, it2 = solver.solve(rhs, I_t, tol=leaf_tol)
+    if fail.numel():
+        u[fail] = _lib.gemm(Y_fail, g_leaf[fail].contiguous())
+    return SurrogateSolve(u[..., 0], passed, err, it1 + it2)
+
+
+def _leaf_subproblem(pde, I_t, idx):
+    """The PDEProblem restricted to the leaves idx (same operators), with I
+    coefficients I_t[idx]."""
+    import copy
+
+    from ._pdeproblem import PDEProblem
+
+    sub = PDEProblem.__new__(PDEProblem)
+    sub.__dict__.update({kk: v for kk, v in pde.__dict__.items() if not kk.endswith("_coefficients")})
+    for nm in ("D_xx", "D_xy", "D_yy", "D_x", "D_y", "I"):
+        a = getattr(pde, f"{nm}_coefficients", None)
+        if nm == "I":
+            a = I_t
+        if a is not None:
+            a = a[idx.cpu().numpy()] if not isinstance(a, torch.Tensor) else a[idx.to(a.device)]
+        setattr(sub, f"{nm}_coefficients", a)
+    sub.source = None
+    dom = copy.copy(pde.domain)
+    dom.n_leaves = int(idx.numel())
+    sub.domain = dom
+    sub._dev_cache = {kk: v for kk, v in pde._dev_cache.items() if not str(kk[0]).startswith(("refine", "iti_row"))}
+    sub.reset()
+    return sub

@@ -99,3 +99,65 @@ def test_surrogate_training_gain_and_fallback(gpu):
     assert float(chk.probe_error[chk.passed].max()) <= chk_tol
     fb = ~chk.passed
     assert torch.equal(chk.T[fb], Tt[fb])
+
+
+def _config_e_problem(gpu, L, seed_theta=0):
+    import jaxhps_amd as jh
+    from jaxhps_amd import workloads as wl
+
+    k = 40.0
+    dom, kw, g, basis, thetas, theta_star = wl.config_E(L=L, p=16, k=k)
+    I = k**2 * (1.0 - np.tensordot(thetas[seed_theta], basis, axes=1))
+    pde = jh.PDEProblem(dom, I_coefficients=I, **kw)
+    return k, dom, kw, g, I, pde
+
+
+def test_shared_leaf_solver_matches_plain_leaf_solves(gpu):
+    """The fixed-point leaf solver with the one shared factorisation of the
+    q = 0 leaf matrix equals the plain per-leaf LU solves (config E's leaves:
+    k = 40, p = 16, L = 6)."""
+    from jaxhps_amd._lib import HPS_KIND_ITI
+    from jaxhps_amd._stages import leaf_apply_inverse, leaf_solve
+    from jaxhps_amd.surrogate import SharedLeafSolver
+
+    print("shared leaf solver", flush=True)
+    k, dom, kw, g, I, pde = _config_e_problem(gpu, 6)
+    solver = SharedLeafSolver(pde, I_ref=k**2, device=gpu)
+    gen = torch.Generator(device="cpu").manual_seed(2)
+    rhs = torch.complex(torch.randn((dom.n_leaves, 256, 2), generator=gen, dtype=torch.float64),
+                        torch.randn((dom.n_leaves, 256, 2), generator=gen, dtype=torch.float64)).to(gpu)
+    x, conv, it = solver.solve(rhs, torch.as_tensor(I, device=gpu))
+    assert bool(conv.all()) and it <= 30, it
+    res = leaf_solve(pde, HPS_KIND_ITI, torch.complex128, gpu, nsrc_zero=True)
+    x_ref = leaf_apply_inverse(res, rhs)
+    assert float((x - x_ref).abs().max() / x_ref.abs().max()) < 1e-10
+
+
+def test_surrogate_solve_with_probe_check_and_fallback(gpu):
+    """The surrogate forward solve (MLP leaf maps, each checked on a random
+    boundary probe, complex64 merges, plain interior solves) against the plain
+    FP64 build + solve on a held-out field at config E's size (L = 6); with a
+    zero tolerance every leaf falls back and the result is the plain one."""
+    import jaxhps_amd as jh
+    from jaxhps_amd.surrogate import LeafItISurrogate, SharedLeafSolver, leaf_training_data, surrogate_solve
+
+    print("surrogate solve in the loop", flush=True)
+    q, T, T0 = leaf_training_data(n_fields=4, L=6, p=16, k=40.0, seed=1, device=gpu)
+    model = LeafItISurrogate(n_in=256, width=512, nq=56, rank=8, seed=0, device=gpu)
+    model.fit(q, T, T0, steps=1000, batch=256, lr=1e-3)
+    k, dom, kw, g, I, pde = _config_e_problem(gpu, 6, seed_theta=3)
+    jh.build_solver(pde)
+    u_plain = jh.solve(pde, g)
+    solver = SharedLeafSolver(pde, I_ref=k**2, device=gpu)
+    It = torch.as_tensor(I, device=gpu)
+    res = surrogate_solve(model, solver, pde, g, It, k, tol=2e-2)
+    frac = float(res.passed.double().mean())
+    err = float((res.u - u_plain).norm() / u_plain.norm())
+    print(f"surrogate leaves {frac:.3f}, end-to-end error vs plain {err:.2e}, fixed-point steps {res.iterations}",
+          flush=True)
+    assert frac > 0.5
+    assert float(res.probe_error[res.passed].max()) <= 2e-2
+    assert err < 0.1
+    allplain = surrogate_solve(model, solver, pde, g, It, k, tol=0.0, merge_dtype=torch.complex128)
+    assert not bool(allplain.passed.any())
+    assert float((allplain.u - u_plain).norm() / u_plain.norm()) < 1e-9
