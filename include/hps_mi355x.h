@@ -138,8 +138,9 @@ int hps_leaf_solve(hps_handle_t h, int dtype, int kind, int nl, int n, int nb, i
  *   (merge/_uniform_3D_DtN.py:128-187, merge/_schur_complement.py:294-774). */
 /* Pivoting of hps_merge_level's factorisation of D (DtN) or W (ItI) on this
  * handle: 1 (default) = partial pivoting; 0 = the panels taller than 1024
- * rows (512 for complex128) are factored without row exchanges, the shorter
- * trailing panels keep partial pivoting.  The merge matrices of the uniform
+ * rows (512 for complex128), and every panel of a level of at most 16
+ * matrices, are factored without row exchanges; the other panels keep
+ * partial pivoting.  The merge matrices of the uniform
  * tree eliminate stably in their natural order (pivot growth ~1, multipliers
  * below 4 on the Helmholtz and variable-coefficient workloads), while the
  * tall pivoted (tournament) panels of the top levels are latency-bound; a

@@ -53,11 +53,24 @@ hipError_t gemm(const GemmArgs<T>& a, hipStream_t s) {
   if (a.N <= 2) {
     static const std::string nm = std::string("gemm_skinny<") + tname<T>() + ">";
     prof::Scope ps(s, nm.c_str(), flops, bytes);
-    dim3 grid(ceil_div(a.M, 4), gy);
-    if (a.N <= 1)
+    // lanes per row: about one element of the row per lane
+    const int lpr = a.K <= 64 ? 4 : (a.K <= 512 ? 16 : 64);
+    dim3 grid(ceil_div(a.M, 256 / lpr), gy);
+    if (lpr == 4) {
+      if (a.N <= 1)
+        hipLaunchKernelGGL((gemm_skinny_kernel<T, 1, 4>), grid, dim3(256), 0, s, a);
+      else
+        hipLaunchKernelGGL((gemm_skinny_kernel<T, 2, 4>), grid, dim3(256), 0, s, a);
+    } else if (lpr == 16) {
+      if (a.N <= 1)
+        hipLaunchKernelGGL((gemm_skinny_kernel<T, 1, 16>), grid, dim3(256), 0, s, a);
+      else
+        hipLaunchKernelGGL((gemm_skinny_kernel<T, 2, 16>), grid, dim3(256), 0, s, a);
+    } else if (a.N <= 1) {
       hipLaunchKernelGGL((gemm_skinny_kernel<T, 1>), grid, dim3(256), 0, s, a);
-    else
+    } else {
       hipLaunchKernelGGL((gemm_skinny_kernel<T, 2>), grid, dim3(256), 0, s, a);
+    }
   } else if (a.N <= 16) {
     // one 16-wide MFMA column block; the four waves stack along M (64-row tiles)
     static const std::string nm = std::string("gemm_mfma_64x16<") + tname<T>() + ">";

@@ -389,15 +389,19 @@ __global__ __launch_bounds__(256) void gemv_conjt_kernel(GemmArgs<T> p) {
   }
 }
 
-template <typename T, int NMAX>
+// N <= 2: LPR lanes per output row (64 = one wave per row for long rows;
+// 16 or 4 for the short rows of the leaf apply and the lower tree levels, so
+// a wave serves 4 or 16 rows and the cross-lane sum is 4 or 2 steps).  Each
+// row's lanes read it contiguously.  256 / LPR rows per workgroup.
+template <typename T, int NMAX, int LPR = 64>
 __global__ __launch_bounds__(256) void gemm_skinny_kernel(GemmArgs<T> p) {
   using R = typename real_of<T>::type;
   constexpr bool CPLX = is_cplx<T>::value;
-  const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (row >= p.M) return;
+  const int sub = threadIdx.x % LPR;
+  const int row = blockIdx.x * (256 / LPR) + threadIdx.x / LPR;
+  const bool live = row < p.M;  // every lane takes part in the cross-lane sums
   for (int b = blockIdx.y; b < p.batch; b += gridDim.y) {
-    const T* A = p.A + (long)b * p.sA + (long)row * p.lda;
+    const T* A = p.A + (long)b * p.sA + (long)(live ? row : 0) * p.lda;
     const T* B = p.B + (long)b * p.sB;
     R acc_re[NMAX], acc_im[NMAX];
 #pragma unroll
@@ -405,7 +409,7 @@ __global__ __launch_bounds__(256) void gemm_skinny_kernel(GemmArgs<T> p) {
       acc_re[n] = R(0);
       acc_im[n] = R(0);
     }
-    for (int k = lane; k < p.K; k += 64) {
+    for (int k = sub; k < p.K && live; k += LPR) {
       T a = A[k];
       const T* Bk = B + (long)k * p.ldb;
 #pragma unroll
@@ -424,15 +428,19 @@ __global__ __launch_bounds__(256) void gemm_skinny_kernel(GemmArgs<T> p) {
 #pragma unroll
     for (int n = 0; n < NMAX; ++n) {
       if (n < p.N) {
-        acc_re[n] = wave_sum(acc_re[n]);
-        if constexpr (CPLX) acc_im[n] = wave_sum(acc_im[n]);
+#pragma unroll
+        for (int off = LPR / 2; off > 0; off >>= 1) {
+          acc_re[n] += __shfl_xor(acc_re[n], off, 64);
+          if constexpr (CPLX) acc_im[n] += __shfl_xor(acc_im[n], off, 64);
+        }
       }
     }
+    if (!live) continue;
     T* D = p.D + (long)b * p.sD + (long)row * p.ldd;
     const T* C = p.C ? p.C + (long)b * p.sC + (long)row * p.ldc : nullptr;
 #pragma unroll
     for (int n = 0; n < NMAX; ++n) {
-      if (n < p.N && lane == n) {
+      if (n < p.N && sub == n) {
         T v = make_val<T>(acc_re[n], acc_im[n]);
         v = s_scale(v, p.alpha);
         if (p.beta != 0.0) v = s_add(v, s_scale(C[n], p.beta));

@@ -41,6 +41,10 @@ namespace hps {
 #endif
 
 constexpr int LU_NB = 32;
+// with row exchanges off, batches this small take the unpivoted panel kernel
+// for every panel: the pivoted register panels then run one workgroup per
+// matrix on an otherwise idle GPU
+constexpr int LU_NOPIV_SMALL_BATCH = 16;
 constexpr size_t LU_PANEL_LDS_MAX = 150 * 1024;
 constexpr int SWTAB = 1 + 4 * LU_NB;  // count, then up to 2*NB (row, src) int pairs
 template <typename T> __device__ __forceinline__ T make_conj(T x) { return x; }
@@ -1091,9 +1095,10 @@ __global__ __launch_bounds__(256) void getrf_panel_nopiv_kernel(int n, T* A, lon
   // its own step, so its multipliers go to W and one barrier per step suffices
   for (int j = 0; j < LU_NB - 1; ++j) {
     const int w = LU_NB - j;
+    const T inv = s_inv(D[j][j]);
     for (int idx = tid; idx < (w - 1) * w; idx += 256) {
       const int r = j + 1 + idx / w, c = j + idx % w;
-      const T l = s_mul(D[r][j], s_inv(D[j][j]));
+      const T l = s_mul(D[r][j], inv);
       if (c == j)
         W[r][j] = l;
       else
@@ -1334,7 +1339,7 @@ hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, T* dinv, int* scr
       const int nb = bend - k < LU_NB ? bend - k : LU_NB;
       bool swapped = false;
       hipError_t e = hipSuccess;
-      if (nopiv && n - k > rows_panel_max<T>()) {
+      if (nopiv && (n - k > rows_panel_max<T>() || batch <= LU_NOPIV_SMALL_BATCH)) {
         static const std::string nm_np = std::string("getrf_panel_nopiv<") + tname<T>() + ">";
         const int m = n - k;
         prof::Scope ps(s, nm_np.c_str(), flop_mult<T>() * batch * ((double)m * nb * nb - (double)nb * nb * nb / 3.0),

@@ -23,9 +23,10 @@ def _levels(T_arr, h_arr, l, kind, nc, table_fn, keep_B=False, root_T=True, dtyp
     dtypes[i], when given, is the precision level i merges in (the children's
     T and h are converted on entry).
 
-    The tall panels of the interface matrices (the top levels') are factored
-    without row exchanges (pivot=False): these matrices eliminate stably in
-    their natural order, and pivoted panels that tall are latency-bound.  The factors of every
+    The interface matrices of the top levels (panels taller than the register
+    panels hold, or at most 16 matrices) are factored without row exchanges
+    (pivot=False): these matrices eliminate stably in their natural order,
+    and pivoted panels there are latency-bound.  The factors of every
     level are checked at the end (one synchronisation); from the first level
     that fails the check the merges are redone with partial pivoting."""
     from . import profiling
@@ -58,10 +59,11 @@ def _levels(T_arr, h_arr, l, kind, nc, table_fn, keep_B=False, root_T=True, dtyp
 
     out = run(0, T0, h0, [], pivot)
     if not pivot:
-        # only levels with panels taller than the register panels (1024 rows,
-        # 512 for complex128) were factored without row exchanges
+        # the levels factored without row exchanges: those with panels taller
+        # than the register panels (1024 rows, 512 for complex128) and those of
+        # at most 16 matrices
         checked = [i for i, r in enumerate(out)
-                   if r.F_lu.shape[-1] > (512 if r.F_lu.dtype == torch.complex128 else 1024)]
+                   if r.F_lu.shape[-1] > (512 if r.F_lu.dtype == torch.complex128 else 1024) or r.F_lu.shape[0] <= 16]
         if checked:
             ok = torch.stack([merge_factors_stable(out[i].F_lu) for i in checked]).cpu()
             if not bool(ok.all()):

@@ -26,7 +26,8 @@ def _ref64(t):
 @pytest.mark.parametrize(
     "batch,M,N,K",
     [(1, 16, 16, 4), (3, 64, 64, 16), (5, 56, 56, 256), (2, 130, 70, 33), (7, 196, 72, 60), (2, 100, 1, 300),
-     (3, 77, 16, 45), (4, 33, 3, 7), (1, 300, 257, 129), (6, 200, 2, 70), (3, 150, 9, 256)],
+     (3, 77, 16, 45), (4, 33, 3, 7), (1, 300, 257, 129), (6, 200, 2, 70), (3, 150, 9, 256),
+     (3, 90, 1, 56), (2, 131, 2, 40), (2, 37, 2, 700), (1, 5, 1, 3)],
 )
 def test_gemm_matches_reference(gpu, dtype, batch, M, N, K):
     from jaxhps_amd import _lib
@@ -164,9 +165,8 @@ def test_unpivoted_tall_merge_matches_pivoted(gpu, kind, dtype):
     """A merge level whose interface matrix is tall enough for the unpivoted
     panel kernel (1792 unknowns, children with 448 Gauss nodes per side) on
     well-conditioned synthetic maps: the unpivoted factors pass the check
-    and S, T and g_tilde equal the partially pivoted merge's.  (The panels
-    of its last 1024 (complex128: 512) rows are the pivoted register panels:
-    the factorisation is unpivoted only where the tournament panels were.)"""
+    and S, T and g_tilde equal the partially pivoted merge's (a single
+    matrix: every panel unpivoted)."""
     from jaxhps_amd._stages import merge_factors_stable, merge_level
     from jaxhps_amd._tables import quad_merge_tables
 
