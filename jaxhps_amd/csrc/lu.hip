@@ -1071,12 +1071,35 @@ static int rows_panel_max() {
 // below 4; the caller checks the factors and refactors with pivoting
 // otherwise).  Every workgroup factors the nb x nb diagonal block in LDS
 // (small; redundant work costs less than a second launch) and inverts its
-// U11; its 256 panel rows below the block then take L21 = A21 inv(U11), one
-// row per thread, each entry an independent dot product over the row held in
-// registers.  Workgroup 0 writes the factored block, its packed inverse
+// U11; its NOPIV_ROWS panel rows below the block then take L21 = A21
+// inv(U11), four lanes per row, each entry an independent dot product over
+// the row held in the quad's registers.  Workgroup 0 writes the factored block, its packed inverse
 // (dinv) and the identity pivots.  The rows are independent: grid
-// (ceil((n - k - nb) / 256), batch).
+// (ceil((n - k - nb) / NOPIV_ROWS), batch).
 // ---------------------------------------------------------------------------
+constexpr int NOPIV_ROWS = 64;  // panel rows per workgroup of the unpivoted panel
+
+// the value of lane CTRL & 3 of each quad (CTRL = 0x55 * lane: quad_perm [l,l,l,l])
+template <int CTRL>
+__device__ __forceinline__ float quad_bcast(float v) { return dpp_move<CTRL, 0xf>(0.0f, v); }
+template <int CTRL>
+__device__ __forceinline__ double quad_bcast(double v) { return dpp_move<CTRL, 0xf>(0.0, v); }
+template <int CTRL>
+__device__ __forceinline__ c64 quad_bcast(c64 v) { return c64{quad_bcast<CTRL>(v.re), quad_bcast<CTRL>(v.im)}; }
+template <int CTRL>
+__device__ __forceinline__ c128 quad_bcast(c128 v) { return c128{quad_bcast<CTRL>(v.re), quad_bcast<CTRL>(v.im)}; }
+
+// x[j] += sum_t a_S[t] W[8 S + t][8 sub + j], a_S = lane S's entries of the quad's row
+template <int S, typename T>
+__device__ __forceinline__ void nopiv_rows_accum(const T (&a)[8], T (&x)[8], const T (*W)[LU_NB + 1], int sub) {
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const T ai = quad_bcast<S * 0x55>(a[t]);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x[j] = s_fma(ai, W[8 * S + t][8 * sub + j], x[j]);
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void getrf_panel_nopiv_kernel(int n, T* A, long lda, long sA, int k, int nb,
                                                                 int* ipiv, T* dinv) {
@@ -1120,18 +1143,35 @@ __global__ __launch_bounds__(256) void getrf_panel_nopiv_kernel(int n, T* A, lon
     W[r][c] = (r <= c && c < nb) ? P[r][c] : s_zero<T>();  // P holds garbage beyond nb
   }
   __syncthreads();
-  // panel rows below the block: x = a inv(U11)
-  const int rg = k + nb + blockIdx.x * 256 + tid;
-  if (rg < n) {
-    T a[LU_NB];
-    T* rowp = Ab + (long)rg * lda + k;
+  // panel rows below the block: x = a inv(U11), four lanes per row, each
+  // holding eight consecutive entries of the row; entry i comes from lane
+  // i / 8 of the quad by a DPP broadcast, and each lane forms the row's
+  // outputs 8 sub .. 8 sub + 7
+  {
+    const int sub = tid & 3;
+    const int rg = k + nb + blockIdx.x * NOPIV_ROWS + (tid >> 2);
+    // a quad's lanes share their row, so whole quads are live or not and the
+    // broadcasts read live lanes only
+    if (rg < n) {
+      T* rowp = Ab + (long)rg * lda + k + 8 * sub;
+      T a[8], x[8];
 #pragma unroll
-    for (int i = 0; i < LU_NB; ++i) a[i] = i < nb ? rowp[i] : s_zero<T>();
-    for (int j = 0; j < nb; ++j) {
-      T acc = s_zero<T>();
+      for (int j = 0; j < 8; ++j) {
+        a[j] = 8 * sub + j < nb ? rowp[j] : s_zero<T>();
+        x[j] = s_zero<T>();
+      }
+      nopiv_rows_accum<0>(a, x, W, sub);
+      nopiv_rows_accum<1>(a, x, W, sub);
+      nopiv_rows_accum<2>(a, x, W, sub);
+      nopiv_rows_accum<3>(a, x, W, sub);
+      if (nb == LU_NB) {
 #pragma unroll
-      for (int i = 0; i < LU_NB; ++i) acc = s_fma(a[i], W[i][j], acc);
-      rowp[j] = acc;
+        for (int j = 0; j < 8; ++j) rowp[j] = x[j];
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          if (8 * sub + j < nb) rowp[j] = x[j];
+      }
     }
   }
   if (blockIdx.x != 0) return;
@@ -1346,7 +1386,8 @@ hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, T* dinv, int* scr
                        2.0 * sizeof(T) * (double)m * nb * batch);
         for (int b0 = 0; b0 < batch; b0 += 65535) {
           const int bc = batch - b0 < 65535 ? batch - b0 : 65535;
-          hipLaunchKernelGGL((getrf_panel_nopiv_kernel<T>), dim3(ceil_div(m - nb, 256) > 0 ? ceil_div(m - nb, 256) : 1, bc),
+          hipLaunchKernelGGL((getrf_panel_nopiv_kernel<T>),
+                             dim3(ceil_div(m - nb, NOPIV_ROWS) > 0 ? ceil_div(m - nb, NOPIV_ROWS) : 1, bc),
                              dim3(256), 0, s, n, A + (long)b0 * sA, lda, sA, k, nb, ipiv + (long)b0 * n,
                              dinv + (long)b0 * ((n + LU_NB - 1) / LU_NB) * LU_NB * LU_NB);
         }
@@ -1420,39 +1461,37 @@ __global__ void gather_rows_kernel(int n, int nrhs, const int* perm, const T* R,
 constexpr int LU_SMALL_MAX_N = 1024;
 constexpr int LU_SMALL_MAX_RHS = 16;
 
-// Sum over each aligned group of four lanes (every lane gets its group's sum).
-__device__ __forceinline__ double quad_lane_sum(double v) {
-  v += __shfl_xor(v, 1, 64);
-  return v + __shfl_xor(v, 2, 64);
-}
-__device__ __forceinline__ float quad_lane_sum(float v) {
-  v += __shfl_xor(v, 1, 64);
-  return v + __shfl_xor(v, 2, 64);
-}
-__device__ __forceinline__ c128 quad_lane_sum(c128 v) { return c128{quad_lane_sum(v.re), quad_lane_sum(v.im)}; }
-__device__ __forceinline__ c64 quad_lane_sum(c64 v) { return c64{quad_lane_sum(v.re), quad_lane_sum(v.im)}; }
 
 // xs[i][c] -= sum_q Ab[i * lda + q] xs[kx + q][c] for rows r0 <= i < r1 and
 // the nb <= 32 block columns q, by a 256-thread workgroup: four lanes per
 // row, eight block columns each, so a row's block segment is read
 // contiguously by its lanes and their products summed across them (a
-// row-per-thread loop strides lda between lanes).  The block rows of xs are
-// only read.  Every thread of the workgroup must call it.
+// row-per-thread loop strides lda between lanes).  The rows go in chunks of
+// 64; each chunk's loads are issued before the previous chunk's products, and
+// the first chunk comes in `a`, loaded by offblock_load before the caller's
+// own work of the step, so no step waits on a cold global load.  The block
+// rows of xs are only read.  Every thread of the workgroup must call it.
+template <typename T>
+__device__ __forceinline__ void offblock_load(const T* Ab, long lda, int i, int r1, int nb, int sub, T (&a)[8]) {
+  const bool live = i < r1;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int q = 8 * sub + j;
+    a[j] = live && q < nb ? Ab[(long)i * lda + q] : s_zero<T>();
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ void offblock_rows(const T* Ab, long lda, int r0, int r1, int nb, int nrhs, T* xs, int kx,
-                                              int tid) {
+                                              int tid, T (&a)[8]) {
   const int sub = tid & 3;
   const int row0 = r0 + (tid >> 2);
   const int iters = (r1 - r0 + 63) / 64;  // uniform across the workgroup
   for (int it = 0; it < iters; ++it) {
     const int i = row0 + 64 * it;
     const bool live = i < r1;
-    T a[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int q = 8 * sub + j;
-      a[j] = live && q < nb ? Ab[(long)i * lda + q] : s_zero<T>();
-    }
+    T an[8];
+    offblock_load(Ab, lda, i + 64, it + 1 < iters ? r1 : 0, nb, sub, an);
     for (int c = 0; c < nrhs; ++c) {
       T prod = s_zero<T>();
 #pragma unroll
@@ -1460,10 +1499,76 @@ __device__ __forceinline__ void offblock_rows(const T* Ab, long lda, int r0, int
         const int q = 8 * sub + j;
         if (q < nb) prod = s_add(prod, s_mul(a[j], xs[(kx + q) * nrhs + c]));
       }
-      prod = quad_lane_sum(prod);
+      prod = quad_sum(prod);  // DPP: every lane of the quad runs this
       if (live && sub == 0) xs[i * nrhs + c] = s_sub(xs[i * nrhs + c], prod);
     }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a[j] = an[j];
   }
+}
+
+// The 32 x 32 triangular factor the solve applies to a diagonal block, from
+// the packed inverse dk: unit-lower inv(L_kk) (LOWER) or inv(U_kk), or for
+// CT their conjugate transposes (inv(U_kk)^H lower, inv(L_kk)^H unit upper).
+// PER = 1024 / (threads of the workgroup) entries per thread, so the next
+// step's block can be loaded into registers while the current one is applied.
+template <typename T, bool CT, int PER>
+__device__ __forceinline__ void diag_inverse_load(const T* dk, bool lower, int tid, T (&d)[PER]) {
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    const int idx = tid + (LU_NB * LU_NB / PER) * e;
+    const int r = idx / LU_NB, q = idx - r * LU_NB;
+    T v;
+    if (lower) {
+      if (CT)
+        v = q <= r ? make_conj(dk[q * LU_NB + r]) : s_zero<T>();
+      else
+        v = q < r ? dk[idx] : (q == r ? s_one<T>() : s_zero<T>());
+    } else {
+      if (CT)
+        v = q > r ? make_conj(dk[q * LU_NB + r]) : (q == r ? s_one<T>() : s_zero<T>());
+      else
+        v = q >= r ? dk[idx] : s_zero<T>();
+    }
+    d[e] = v;
+  }
+}
+
+// xs[k + r][c] = sum_q Di[r][q] xs[k + q][c] for the nb rows of a diagonal
+// block: one quad per output (r, c), lane s summing q = s, s + 4, ... with
+// the loads of all eight terms issued together, then a DPP quad sum.  Every
+// thread of the workgroup calls it; it ends with xs updated and synchronised.
+template <typename T, int THREADS = 256>
+__device__ __forceinline__ void diag_block_apply(const T (*Di)[LU_NB + 1], T* xs, int k, int nb, int nrhs, int tid) {
+  constexpr int QUADS = THREADS / 4;
+  constexpr int MAXP = LU_NB * LU_SMALL_MAX_RHS / QUADS;
+  const int sub = tid & 3, quad = tid >> 2;
+  const int nout = nb * nrhs;
+  const int per = (nout + QUADS - 1) / QUADS;  // <= MAXP for nb <= 32, nrhs <= 16
+  T y[MAXP];
+#pragma unroll
+  for (int e = 0; e < MAXP; ++e) {
+    y[e] = s_zero<T>();
+    if (e < per) {
+      const int o = quad + QUADS * e;
+      const bool live = o < nout;
+      const int r = live ? o / nrhs : 0, c = live ? o - r * nrhs : 0;
+      T acc = s_zero<T>();
+#pragma unroll
+      for (int t = 0; t < LU_NB / 4; ++t) {
+        const int q = 4 * t + sub;
+        if (live && q < nb) acc = s_fma(Di[r][q], xs[(k + q) * nrhs + c], acc);
+      }
+      y[e] = quad_sum(acc);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < MAXP; ++e) {
+    const int o = quad + QUADS * e;
+    if (e < per && o < nout && sub == 0) xs[(k + o / nrhs) * nrhs + o % nrhs] = y[e];
+  }
+  __syncthreads();
 }
 
 template <typename T, bool CT>
@@ -1480,72 +1585,66 @@ __global__ __launch_bounds__(256) void getrs_small_kernel(int n, int nrhs, const
   const int nblk = (n + LU_NB - 1) / LU_NB;
   const T* db = dinv + (long)b * nblk * LU_NB * LU_NB;
   const T* Rb = R + (long)b * sR;
+  // steps t < nblk: lower (L, or U^H for CT), forward; then upper (U, or
+  // L^H), backward.  Step t + 1's diagonal inverse and first chunk of rows
+  // are loaded into registers at the start of step t.
+  const int sub = tid & 3;
+  auto geom = [&](int t, int& k, int& nb, int& r0, int& r1) {
+    const bool lower = t < nblk;
+    const int kb = lower ? t : 2 * nblk - 1 - t;
+    k = kb * LU_NB;
+    nb = n - k < LU_NB ? n - k : LU_NB;
+    r0 = lower ? k + nb : 0;  // rows outside the block: below (lower) or above
+    r1 = lower ? n : k;
+  };
+  T d[4], a[8];
+  {
+    int k, nb, r0, r1;
+    geom(0, k, nb, r0, r1);
+    diag_inverse_load<T, CT, 4>(db, true, tid, d);
+    if (!CT) offblock_load(A + k, lda, r0 + (tid >> 2), r1, nb, sub, a);
+  }
+  // (the first step's factor loads above go out before these)
   for (int idx = tid; idx < n * nrhs; idx += 256) {
     const int i = idx / nrhs, c = idx - i * nrhs;
     xs[idx] = CT ? Rb[(long)i * ldr + c] : Rb[(long)pb[i] * ldr + c];
   }
   __syncthreads();
-  // pass 0: lower (L, or U^H for CT), forward; pass 1: upper (U, or L^H), backward
-  for (int pass = 0; pass < 2; ++pass) {
-    for (int s = 0; s < nblk; ++s) {
-      const int kb = pass == 0 ? s : nblk - 1 - s;
-      const int k = kb * LU_NB;
-      const int nb = n - k < LU_NB ? n - k : LU_NB;
-      const T* dk = db + (long)kb * LU_NB * LU_NB;
-      for (int idx = tid; idx < LU_NB * LU_NB; idx += 256) {
-        const int r = idx / LU_NB, q = idx - r * LU_NB;
-        T v;
-        if (pass == 0) {
-          if (CT)
-            v = q <= r ? make_conj(dk[q * LU_NB + r]) : s_zero<T>();
-          else
-            v = q < r ? dk[idx] : (q == r ? s_one<T>() : s_zero<T>());
-        } else {
-          if (CT)
-            v = q > r ? make_conj(dk[q * LU_NB + r]) : (q == r ? s_one<T>() : s_zero<T>());
-          else
-            v = q >= r ? dk[idx] : s_zero<T>();
-        }
-        Di[r][q] = v;
-      }
-      __syncthreads();
-      // diagonal block: y = Di x_k
-      T y[LU_SMALL_MAX_RHS / 4];
-      const int per = (nb * nrhs + 255) / 256;  // <= 2 for nb <= 32, nrhs <= 16
-      for (int e = 0; e < per; ++e) {
-        const int idx = tid + 256 * e;
-        T acc = s_zero<T>();
-        if (idx < nb * nrhs) {
-          const int r = idx / nrhs, c = idx - r * nrhs;
-          for (int q = 0; q < nb; ++q) acc = s_add(acc, s_mul(Di[r][q], xs[(k + q) * nrhs + c]));
-        }
-        y[e] = acc;
-      }
-      __syncthreads();
-      for (int e = 0; e < per; ++e) {
-        const int idx = tid + 256 * e;
-        if (idx < nb * nrhs) xs[(k + idx / nrhs) * nrhs + idx % nrhs] = y[e];
-      }
-      __syncthreads();
-      // rows outside the block: below (pass 0) or above (pass 1)
-      const int r0 = pass == 0 ? k + nb : 0;
-      const int r1 = pass == 0 ? n : k;
-      if (!CT)
-        offblock_rows(A + k, lda, r0, r1, nb, nrhs, xs, k, tid);
-      else
+  for (int t = 0; t < 2 * nblk; ++t) {
+    int k, nb, r0, r1;
+    geom(t, k, nb, r0, r1);
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int idx = tid + 256 * e;
+      Di[idx / LU_NB][idx % LU_NB] = d[e];
+    }
+    T a_next[8];
+    if (t + 1 < 2 * nblk) {
+      int k1, nb1, r01, r11;
+      geom(t + 1, k1, nb1, r01, r11);
+      diag_inverse_load<T, CT, 4>(db + (long)(k1 / LU_NB) * LU_NB * LU_NB, t + 1 < nblk, tid, d);
+      if (!CT) offblock_load(A + k1, lda, r01 + (tid >> 2), r11, nb1, sub, a_next);
+    }
+    __syncthreads();
+    diag_block_apply<T>(Di, xs, k, nb, nrhs, tid);  // diagonal block: x_k = Di x_k
+    if constexpr (!CT) {
+      offblock_rows(A + k, lda, r0, r1, nb, nrhs, xs, k, tid, a);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) a[j] = a_next[j];
+    } else {
       for (int i = r0 + tid; i < r1; i += 256) {
         for (int c = 0; c < nrhs; ++c) {
           T acc = xs[i * nrhs + c];
           for (int q = 0; q < nb; ++q) {
-            // L[i][k+q] / U[i][k+q], or for CT conj(U[k+q][i]) / conj(L[k+q][i])
-            const T a = CT ? make_conj(A[(long)(k + q) * lda + i]) : A[(long)i * lda + k + q];
-            acc = s_fnms(acc, a, xs[(k + q) * nrhs + c]);
+            // conj(U[k+q][i]) (lower step) / conj(L[k+q][i]) (upper step)
+            const T av = make_conj(A[(long)(k + q) * lda + i]);
+            acc = s_fnms(acc, av, xs[(k + q) * nrhs + c]);
           }
           xs[i * nrhs + c] = acc;
         }
       }
-      __syncthreads();
     }
+    __syncthreads();
   }
   T* Xb = X + (long)b * sX;
   for (int idx = tid; idx < n * nrhs; idx += 256) {
@@ -1557,14 +1656,22 @@ __global__ __launch_bounds__(256) void getrs_small_kernel(int n, int nrhs, const
   }
 }
 
-// In-place triangular solve of the diagonal block [k0, k0 + nbk) of X for few
-// right-hand sides (the inner steps of the blocked solve of large systems):
-// unit-lower L forward or U backward, 32-row sub-blocks through the stored
-// inverses, the rows of the block below (above) updated by per-thread dot
-// products; the X block lives in LDS.  One workgroup per matrix.
+// In-place triangular solve of the diagonal block [k0, k0 + nbk <= 256) of X
+// for few right-hand sides (the inner steps of the blocked solve of large
+// systems): unit-lower L forward or U backward, 32-row sub-blocks through the
+// stored inverses; the X block lives in LDS.  One 1024-thread workgroup per
+// matrix: the rows of the block below (above) a sub-block, at most 224, take
+// one quad each, so a sub-block's factor entries are all in flight at once,
+// and the next sub-block's entries and inverse are loaded while the current
+// one is applied (one global latency per sub-block at most, not one per
+// 64-row chunk).
+constexpr int GETRS_BLOCK_THREADS = 1024;
+constexpr int GETRS_BLOCK_MAX = 256;
+
 template <typename T, bool UPPER>
-__global__ __launch_bounds__(256) void getrs_block_kernel(int n, int nrhs, const T* LU, long lda, long sA,
-                                                          const T* dinv, T* X, long ldx, long sX, int k0, int nbk) {
+__global__ __launch_bounds__(GETRS_BLOCK_THREADS) void getrs_block_kernel(int n, int nrhs, const T* LU, long lda,
+                                                                          long sA, const T* dinv, T* X, long ldx,
+                                                                          long sX, int k0, int nbk) {
   extern __shared__ char smem_raw[];
   T* xs = (T*)smem_raw;  // (nbk, nrhs)
   __shared__ T Di[LU_NB][LU_NB + 1];
@@ -1574,59 +1681,59 @@ __global__ __launch_bounds__(256) void getrs_block_kernel(int n, int nrhs, const
   const int nblk = (n + LU_NB - 1) / LU_NB;
   const T* db = dinv + (long)b * nblk * LU_NB * LU_NB;
   T* Xb = X + (long)b * sX;
-  for (int idx = tid; idx < nbk * nrhs; idx += 256) {
+  const int nsub = (nbk + LU_NB - 1) / LU_NB;
+  const int sub = tid & 3;
+  const T* Ak = A + (long)k0 * lda + k0;
+  auto geom = [&](int sb, int& kl, int& nb, int& r0, int& r1) {
+    kl = (UPPER ? nsub - 1 - sb : sb) * LU_NB;  // sub-block start within the block
+    nb = nbk - kl < LU_NB ? nbk - kl : LU_NB;
+    r0 = UPPER ? 0 : kl + nb;
+    r1 = UPPER ? kl : nbk;
+  };
+  // the first sub-block's factor loads go out before the X block's, so the
+  // two global latencies overlap
+  T d[1], a[8];
+  {
+    int kl, nb, r0, r1;
+    geom(0, kl, nb, r0, r1);
+    diag_inverse_load<T, false, 1>(db + (long)((k0 + kl) / LU_NB) * LU_NB * LU_NB, !UPPER, tid, d);
+    offblock_load(Ak + kl, lda, r0 + (tid >> 2), r1, nb, sub, a);
+  }
+  for (int idx = tid; idx < nbk * nrhs; idx += GETRS_BLOCK_THREADS) {
     const int i = idx / nrhs, c = idx - i * nrhs;
     xs[idx] = Xb[(long)(k0 + i) * ldx + c];
   }
-  __syncthreads();
-  const int nsub = (nbk + LU_NB - 1) / LU_NB;
   for (int sb = 0; sb < nsub; ++sb) {
-    const int kl = (UPPER ? nsub - 1 - sb : sb) * LU_NB;  // sub-block start within the block
-    const int nb = nbk - kl < LU_NB ? nbk - kl : LU_NB;
-    const T* dk = db + (long)((k0 + kl) / LU_NB) * LU_NB * LU_NB;
-    for (int idx = tid; idx < LU_NB * LU_NB; idx += 256) {
-      const int r = idx / LU_NB, q = idx - r * LU_NB;
-      T v;
-      if (UPPER)
-        v = q >= r ? dk[idx] : s_zero<T>();
-      else
-        v = q < r ? dk[idx] : (q == r ? s_one<T>() : s_zero<T>());
-      Di[r][q] = v;
+    int kl, nb, r0, r1;
+    geom(sb, kl, nb, r0, r1);
+    Di[tid / LU_NB][tid % LU_NB] = d[0];
+    T a_next[8];
+    if (sb + 1 < nsub) {
+      int kl1, nb1, r01, r11;
+      geom(sb + 1, kl1, nb1, r01, r11);
+      diag_inverse_load<T, false, 1>(db + (long)((k0 + kl1) / LU_NB) * LU_NB * LU_NB, !UPPER, tid, d);
+      offblock_load(Ak + kl1, lda, r01 + (tid >> 2), r11, nb1, sub, a_next);
     }
     __syncthreads();
-    T y[LU_SMALL_MAX_RHS / 4];
-    const int per = (nb * nrhs + 255) / 256;  // <= 2 for nb <= 32, nrhs <= 16
-    for (int e = 0; e < per; ++e) {
-      const int idx = tid + 256 * e;
-      T acc = s_zero<T>();
-      if (idx < nb * nrhs) {
-        const int r = idx / nrhs, c = idx - r * nrhs;
-        for (int q = 0; q < nb; ++q) acc = s_add(acc, s_mul(Di[r][q], xs[(kl + q) * nrhs + c]));
+    diag_block_apply<T, GETRS_BLOCK_THREADS>(Di, xs, kl, nb, nrhs, tid);
+    // the rows outside the sub-block: one quad per row, r1 - r0 <= 224
+    const int i = r0 + (tid >> 2);
+    const bool live = i < r1;
+    for (int c = 0; c < nrhs; ++c) {
+      T prod = s_zero<T>();
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int q = 8 * sub + j;
+        if (q < nb) prod = s_add(prod, s_mul(a[j], xs[(kl + q) * nrhs + c]));
       }
-      y[e] = acc;
+      prod = quad_sum(prod);  // DPP: every lane of the quad runs this
+      if (live && sub == 0) xs[i * nrhs + c] = s_sub(xs[i * nrhs + c], prod);
     }
-    __syncthreads();
-    for (int e = 0; e < per; ++e) {
-      const int idx = tid + 256 * e;
-      if (idx < nb * nrhs) xs[(kl + idx / nrhs) * nrhs + idx % nrhs] = y[e];
-    }
-    __syncthreads();
-    const int r0 = UPPER ? 0 : kl + nb;
-    const int r1 = UPPER ? kl : nbk;
-    if (nrhs <= LU_SMALL_MAX_RHS)
-      offblock_rows(A + (long)k0 * lda + k0 + kl, lda, r0, r1, nb, nrhs, xs, kl, tid);
-    else
-    for (int i = r0 + tid; i < r1; i += 256) {
-      for (int c = 0; c < nrhs; ++c) {
-        T acc = xs[i * nrhs + c];
-        const T* row = A + (long)(k0 + i) * lda + k0 + kl;
-        for (int q = 0; q < nb; ++q) acc = s_fnms(acc, row[q], xs[(kl + q) * nrhs + c]);
-        xs[i * nrhs + c] = acc;
-      }
-    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) a[j] = a_next[j];
     __syncthreads();
   }
-  for (int idx = tid; idx < nbk * nrhs; idx += 256) {
+  for (int idx = tid; idx < nbk * nrhs; idx += GETRS_BLOCK_THREADS) {
     const int i = idx / nrhs, c = idx - i * nrhs;
     Xb[(long)(k0 + i) * ldx + c] = xs[idx];
   }
@@ -1635,22 +1742,31 @@ __global__ __launch_bounds__(256) void getrs_block_kernel(int n, int nrhs, const
 template <typename T, bool UPPER>
 static hipError_t getrs_block(int n, int nrhs, const T* LU, long lda, long sA, const T* dinv, T* X, long ldx, long sX,
                               int k0, int nbk, int batch, hipStream_t s) {
+  // the kernel's quads cover at most GETRS_BLOCK_MAX rows, for few columns
+  if (nrhs > LU_SMALL_MAX_RHS || nbk > GETRS_BLOCK_MAX) return hipErrorInvalidValue;
   static const std::string nm = std::string(UPPER ? "getrs_block_upper<" : "getrs_block_lower<") + tname<T>() + ">";
   prof::Scope ps(s, nm.c_str(), flop_mult<T>() * batch * (double)nbk * nbk * nrhs,
                  sizeof(T) * (double)batch * ((double)nbk * nbk / 2 + 2.0 * nbk * nrhs));
   const int nblk = (n + LU_NB - 1) / LU_NB;
   for (int b0 = 0; b0 < batch; b0 += 65535) {
     const int bc = batch - b0 < 65535 ? batch - b0 : 65535;
-    hipLaunchKernelGGL((getrs_block_kernel<T, UPPER>), dim3(bc), dim3(256), (size_t)nbk * nrhs * sizeof(T), s, n,
+    hipLaunchKernelGGL((getrs_block_kernel<T, UPPER>), dim3(bc), dim3(GETRS_BLOCK_THREADS),
+                       (size_t)nbk * nrhs * sizeof(T), s, n,
                        nrhs, LU + (long)b0 * sA, lda, sA, dinv + (long)b0 * nblk * LU_NB * LU_NB, X + (long)b0 * sX,
                        ldx, sX, k0, nbk);
   }
   return hipGetLastError();
 }
 
+// getrs_small streams a whole matrix through one workgroup: for large
+// batches that keeps every CU busy; a few large matrices take the blocked
+// solve instead, whose GEMM updates spread each matrix over the GPU.
+constexpr int GETRS_SMALL_MIN_BATCH = 128;
+
 template <typename T>
-static bool getrs_small_ok(int n, int nrhs) {
-  return n <= LU_SMALL_MAX_N && nrhs <= LU_SMALL_MAX_RHS && (size_t)n * nrhs * sizeof(T) <= 64 * 1024;
+static bool getrs_small_ok(int n, int nrhs, int batch) {
+  return n <= LU_SMALL_MAX_N && nrhs <= LU_SMALL_MAX_RHS && (size_t)n * nrhs * sizeof(T) <= 64 * 1024 &&
+         (batch >= GETRS_SMALL_MIN_BATCH || n <= 256);
 }
 
 template <typename T, bool CT>
@@ -1673,7 +1789,7 @@ template <typename T>
 hipError_t getrs_perm(int n, int nrhs, const T* LU, long lda, long sA, const int* perm, const T* dinv, const T* R,
                       long ldr, long sR, T* X, long ldx, long sX, int batch, hipStream_t s) {
   if (n <= 0 || nrhs <= 0 || batch <= 0) return hipSuccess;
-  if (getrs_small_ok<T>(n, nrhs))
+  if (getrs_small_ok<T>(n, nrhs, batch))
     return getrs_small<T, false>(n, nrhs, LU, lda, sA, perm, dinv, R, ldr, sR, X, ldx, sX, batch, s);
   for (int b0 = 0; b0 < batch; b0 += 65535) {
     static const std::string nm = std::string("getrs_gather<") + tname<T>() + ">";
@@ -1774,7 +1890,9 @@ template <typename T>
 hipError_t getrs_conjt(int n, int nrhs, const T* LU, long lda, long sA, const int* perm, const T* dinv, const T* R,
                        long ldr, long sR, T* X, long ldx, long sX, T* W, int batch, hipStream_t s) {
   if (n <= 0 || nrhs <= 0 || batch <= 0) return hipSuccess;
-  if (getrs_small_ok<T>(n, nrhs))
+  // the blocked conjugate-transposed solve has no block kernel: its 32-row
+  // steps cost more launches than getrs_small costs latency at any batch
+  if (getrs_small_ok<T>(n, nrhs, GETRS_SMALL_MIN_BATCH))
     return getrs_small<T, true>(n, nrhs, LU, lda, sA, perm, dinv, R, ldr, sR, X, ldx, sX, batch, s);
   const long ldw = nrhs, sW = (long)n * nrhs;
   HPS_LU_CHECK(copy2d<T>(n, nrhs, 1.0, R, ldr, sR, W, ldw, sW, batch, s));

@@ -101,6 +101,25 @@ def test_getrs_many_rhs(gpu, dtype, n, batch):
     assert rel < TOL[dtype] * n * 50, rel
 
 
+@pytest.mark.parametrize("dtype", DTYPES)
+@pytest.mark.parametrize("n,batch,nrhs", [(896, 4, 1), (520, 3, 16), (1100, 1, 2), (300, 127, 1), (300, 128, 3)])
+def test_getrs_few_rhs(gpu, dtype, n, batch, nrhs):
+    """Few right-hand sides: the blocked solve with the per-block kernel for
+    small batches, the one-workgroup-per-matrix solve from 128 matrices."""
+    from jaxhps_amd import _lib
+
+    print(f"getrs few rhs dtype={dtype} n={n} batch={batch} nrhs={nrhs}", flush=True)
+    g = torch.Generator().manual_seed(3 * n + batch + nrhs)
+    A = _rand((batch, n, n), dtype, g)
+    R = _rand((batch, n, nrhs), dtype, g)
+    LU = A.to(gpu).contiguous()
+    perm, dinv, info = _lib.getrf(LU)
+    X = _lib.getrs(LU, perm, dinv, R.to(gpu).contiguous())
+    ref = torch.linalg.solve(_ref64(A), _ref64(R))
+    rel = ((_ref64(X) - ref).abs().max() / ref.abs().max()).item()
+    assert rel < TOL[dtype] * n * 50, rel
+
+
 def test_getrf_reports_singular(gpu):
     from jaxhps_amd import _lib
 
