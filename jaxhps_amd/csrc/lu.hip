@@ -1073,8 +1073,10 @@ static int rows_panel_max() {
 // (small; redundant work costs less than a second launch) and inverts its
 // U11; its NOPIV_ROWS panel rows below the block then take L21 = A21
 // inv(U11), four lanes per row, each entry an independent dot product over
-// the row held in the quad's registers.  Workgroup 0 writes the factored block, its packed inverse
-// (dinv) and the identity pivots.  The rows are independent: grid
+// the row held in the quad's registers.  The last workgroup of the matrix to
+// read the diagonal block (a per-matrix arrival counter) writes the factored
+// block, its packed inverse (dinv) and the identity pivots.  The rows are
+// independent: grid
 // (ceil((n - k - nb) / NOPIV_ROWS), batch).
 // ---------------------------------------------------------------------------
 constexpr int NOPIV_ROWS = 64;  // panel rows per workgroup of the unpivoted panel
@@ -1102,16 +1104,26 @@ __device__ __forceinline__ void nopiv_rows_accum(const T (&a)[8], T (&x)[8], con
 
 template <typename T>
 __global__ __launch_bounds__(256) void getrf_panel_nopiv_kernel(int n, T* A, long lda, long sA, int k, int nb,
-                                                                int* ipiv, T* dinv) {
+                                                                int* ipiv, T* dinv, int* arrivals) {
   __shared__ T D[LU_NB][LU_NB + 1];
   __shared__ T W[LU_NB][LU_NB + 1];  // multipliers, then inv(U11)
   __shared__ T P[LU_NB][LU_NB + 1];  // packed inverse
+  __shared__ int last;
   const int b = blockIdx.y;
   const int tid = threadIdx.x;
   T* Ab = A + (long)b * sA;
   for (int idx = tid; idx < LU_NB * LU_NB; idx += 256) {
     const int r = idx / LU_NB, c = idx - r * LU_NB;
     D[r][c] = (r < nb && c < nb) ? Ab[(long)(k + r) * lda + k + c] : (r == c ? s_one<T>() : s_zero<T>());
+  }
+  __syncthreads();
+  // every workgroup of the matrix has read the original block once the
+  // counter reaches gridDim.x: only the last to arrive may overwrite it with
+  // the factors (workgroups of one matrix need not be resident together)
+  if (tid == 0) {
+    const int before = atomicAdd(&arrivals[b], 1);
+    last = before == (int)gridDim.x - 1;
+    if (last) atomicExch(&arrivals[b], 0);  // every workgroup has arrived: ready for the next launch
   }
   __syncthreads();
   // elimination step j: rows r > j, columns c >= j; column j is only read in
@@ -1174,7 +1186,7 @@ __global__ __launch_bounds__(256) void getrf_panel_nopiv_kernel(int n, T* A, lon
       }
     }
   }
-  if (blockIdx.x != 0) return;
+  if (!last) return;
   for (int idx = tid; idx < nb * nb; idx += 256) {
     const int rr = idx / nb, c = idx - rr * nb;
     Ab[(long)(k + rr) * lda + k + c] = D[rr][c];
@@ -1257,7 +1269,9 @@ static hipError_t launch_panel(int n, T* A, long lda, long sA, int k, int nb, in
 
 #if HPS_LU_TYPES & 1
 size_t getrf_scratch_bytes(int n, int batch) {
-  return ((size_t)batch * n + (size_t)batch * SWTAB + 2 * tslu_cand_ints(n, batch)) * sizeof(int);
+  // ipiv, swap tables, tournament candidates (two sets), then one arrival
+  // counter per matrix for the unpivoted panel kernel
+  return ((size_t)batch * n + (size_t)batch * SWTAB + 2 * tslu_cand_ints(n, batch) + (size_t)batch) * sizeof(int);
 }
 #endif
 
@@ -1345,7 +1359,9 @@ hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, T* dinv, int* scr
   int* ipiv = scratch;
   int* swtab = scratch + (long)batch * n;
   int* cand = swtab + (long)batch * SWTAB;
+  int* arrivals = cand + 2 * tslu_cand_ints(n, batch);  // zero between unpivoted panel launches
   hipLaunchKernelGGL(fill_int_kernel, dim3(ceil_div(batch, 256)), dim3(256), 0, s, info, (long)batch, 0);
+  if (nopiv) hipLaunchKernelGGL(fill_int_kernel, dim3(ceil_div(batch, 256)), dim3(256), 0, s, arrivals, (long)batch, 0);
   auto gemm_update = [&](int M, int N, int K, const T* Ap, const T* Bp, T* Cp) -> hipError_t {
     if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
     GemmArgs<T> g;
@@ -1389,7 +1405,7 @@ hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, T* dinv, int* scr
           hipLaunchKernelGGL((getrf_panel_nopiv_kernel<T>),
                              dim3(ceil_div(m - nb, NOPIV_ROWS) > 0 ? ceil_div(m - nb, NOPIV_ROWS) : 1, bc),
                              dim3(256), 0, s, n, A + (long)b0 * sA, lda, sA, k, nb, ipiv + (long)b0 * n,
-                             dinv + (long)b0 * ((n + LU_NB - 1) / LU_NB) * LU_NB * LU_NB);
+                             dinv + (long)b0 * ((n + LU_NB - 1) / LU_NB) * LU_NB * LU_NB, arrivals + b0);
         }
         e = hipGetLastError();
         swapped = true;  // nothing to swap: U12 is formed below like after a register panel

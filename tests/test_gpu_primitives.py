@@ -178,6 +178,34 @@ def test_getrs_conjt_solves_adjoint_system(gpu, dtype, n, nrhs, batch):
     assert res.abs().max().item() < TOL[dtype] * 50 * n**0.5
 
 
+def test_unpivoted_panels_of_many_matrices(gpu):
+    """96 complex64 interface matrices of 1200 unknowns: the leading panels
+    (taller than 1024 rows) unpivoted, the rest partially pivoted, with more
+    unpivoted-panel workgroups than the GPU holds at once, so the workgroups of
+    one matrix do not all run together.  S, T, g_tilde and h equal the
+    pivoted merge's (the factored diagonal block is written only after every
+    workgroup of its matrix has read the original)."""
+    from jaxhps_amd._stages import merge_factors_stable, merge_level
+    from jaxhps_amd._tables import quad_merge_tables
+
+    m, nm = 300, 96
+    ne = 4 * m
+    print(f"unpivoted panels of {nm} matrices, n = {ne}", flush=True)
+    g = torch.Generator(device=gpu).manual_seed(11)
+    X = torch.randn(4 * nm, ne, ne, generator=g, device=gpu, dtype=torch.float32) / ne**0.5
+    Y = torch.randn(4 * nm, ne, ne, generator=g, device=gpu, dtype=torch.float32) / ne**0.5
+    T = (0.5 * torch.complex(X, Y)).contiguous()
+    del X, Y
+    h = torch.randn(4 * nm, ne, 1, generator=g, device=gpu, dtype=torch.float32).to(torch.complex64).contiguous()
+    tb = quad_merge_tables(m, "iti")
+    ref = merge_level(T, h, tb, pivot=True)
+    out = merge_level(T, h, tb, pivot=False)
+    assert bool(merge_factors_stable(out.F_lu))
+    for a, b in ((out.S, ref.S), (out.T, ref.T), (out.gt, ref.gt), (out.h, ref.h)):
+        err = float((a - b).abs().amax(dim=(1, 2)).max() / b.abs().max())
+        assert err < 2e-4, err
+
+
 @pytest.mark.parametrize("kind,dtype", [("iti", torch.complex128), ("iti", torch.complex64),
                                         ("dtn", torch.float64), ("dtn", torch.float32)])
 def test_unpivoted_tall_merge_matches_pivoted(gpu, kind, dtype):
