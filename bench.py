@@ -781,6 +781,7 @@ def bench_subtree(args, L=8, height=6):
                   "subtree_height": height}, None, None, None)
     line["paths"] = out
     line["rel_diff_subtree_vs_stored"] = rel
+    line["accuracy_ok"] = rel < 1e-8  # the same discrete solution by two routes
     line["memory_ratio_stored_over_subtree"] = out["stored"]["peak_device_bytes"] / out["subtree"]["peak_device_bytes"]
     return line
 
