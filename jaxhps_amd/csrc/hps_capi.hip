@@ -27,9 +27,10 @@ namespace hps {
 template <typename T>
 __global__ void copy2d_kernel(int rows, int cols, double alpha, const T* src, long lds, long ss, T* dst, long ldd,
                               long sd, int batch) {
-  long total = (long)rows * cols;
-  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
-    int r = (int)(idx / cols), c = (int)(idx - (long)r * cols);
+  // 32-bit index arithmetic (the per-matrix element count stays below 2^31)
+  const int total = rows * cols;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += gridDim.x * blockDim.x) {
+    const int r = idx / cols, c = idx - r * cols;
     for (int b = blockIdx.y; b < batch; b += gridDim.y) {
       T v = src ? s_scale(src[(long)b * ss + (long)r * lds + c], alpha) : s_zero<T>();
       dst[(long)b * sd + (long)r * ldd + c] = v;
@@ -42,6 +43,7 @@ hipError_t copy2d(int rows, int cols, double alpha, const T* src, long lds, long
                   int batch, hipStream_t s) {
   if (rows <= 0 || cols <= 0 || batch <= 0) return hipSuccess;
   long total = (long)rows * cols;
+  if (total >= (1L << 31)) return hipErrorInvalidValue;  // the kernel indexes one matrix in 32 bits
   unsigned gx = ceil_div(total, 256);
   if (gx > 4096) gx = 4096;
   unsigned gy = batch < 65535 ? batch : 65535;
@@ -83,36 +85,66 @@ hipError_t add2d(int rows, int cols, double alpha, const T* src, long lds, long 
 // leaf operator assembly: out[l][r][c] = sum_i coef_i[l][row0+r] * D_{op_i}[row0+r][col0+c]
 // op index < 0 means the identity operator.
 // ---------------------------------------------------------------------------
-// Each thread assembles its entry for ASM_LEAVES consecutive leaves, so the
-// operator entries it reads (shared by all leaves) come from cache once per
-// group instead of once per leaf.
+// One workgroup per (operator row, group of ASM_LEAVES consecutive leaves):
+// the row's coefficient values are uniform across the workgroup (scalar
+// loads, once per workgroup), each thread reads the operator entries of its
+// columns (shared by every leaf, so they come from cache) and writes them,
+// combined, for each leaf of the group: every store is a coalesced row
+// segment.
 constexpr int ASM_LEAVES = 4;
 
 template <typename T>
 __global__ __launch_bounds__(256) void assemble_kernel(int n, int nl, int ncoef, CoefOps ops_idx, const T* coeffs,
                                                        const T* diffops, int row0, int nrows, int col0, int ncols,
                                                        T* out, long ldo, long so) {
-  long total = (long)nrows * ncols;
+  const int r = blockIdx.x;
+  if (r >= nrows) return;
+  const int gr = row0 + r;
   for (int l0 = blockIdx.y * ASM_LEAVES; l0 < nl; l0 += gridDim.y * ASM_LEAVES) {
-  const int nlg = nl - l0 < ASM_LEAVES ? nl - l0 : ASM_LEAVES;
-  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
-    int r = (int)(idx / ncols), c = (int)(idx - (long)r * ncols);
-    int gr = row0 + r, gc = col0 + c;
-    T acc[ASM_LEAVES];
+    const int nlg = nl - l0 < ASM_LEAVES ? nl - l0 : ASM_LEAVES;
+    for (int c = threadIdx.x; c < ncols; c += blockDim.x) {
+      const int gc = col0 + c;
+      T acc[ASM_LEAVES];
 #pragma unroll
-    for (int j = 0; j < ASM_LEAVES; ++j) acc[j] = s_zero<T>();
-    for (int i = 0; i < ncoef; ++i) {
-      const int op = ops_idx.op[i];
-      const T d = op < 0 ? (gr == gc ? s_one<T>() : s_zero<T>()) : diffops[((long)op * n + gr) * n + gc];
-      const T* cfp = coeffs + ((long)i * nl + l0) * n + gr;
+      for (int j = 0; j < ASM_LEAVES; ++j) acc[j] = s_zero<T>();
+      for (int i = 0; i < ncoef; ++i) {
+        const int op = ops_idx.op[i];
+        const T d = op < 0 ? (gr == gc ? s_one<T>() : s_zero<T>()) : diffops[((long)op * n + gr) * n + gc];
+        const T* cfp = coeffs + ((long)i * nl + l0) * n + gr;  // uniform across the workgroup
+#pragma unroll
+        for (int j = 0; j < ASM_LEAVES; ++j)
+          if (j < nlg) acc[j] = s_add(acc[j], s_mul(cfp[(long)j * n], d));
+      }
 #pragma unroll
       for (int j = 0; j < ASM_LEAVES; ++j)
-        if (j < nlg) acc[j] = s_add(acc[j], s_mul(cfp[(long)j * n], d));
+        if (j < nlg) out[(long)(l0 + j) * so + (long)r * ldo + c] = acc[j];
     }
-#pragma unroll
-    for (int j = 0; j < ASM_LEAVES; ++j)
-      if (j < nlg) out[(long)(l0 + j) * so + (long)r * ldo + c] = acc[j];
   }
+}
+
+// ItI leaf right-hand sides, gathered by the LU's row permutation:
+// X[b][i][c] = R[b][perm[b][i]][c] with R = [P 0; 0 f_int] (the scaled
+// Gauss-to-Chebyshev map P in the first nb rows and nq columns, the source's
+// interior rows in the last nsrc columns), without forming R.
+template <typename T>
+__global__ __launch_bounds__(256) void iti_rhs_gather_kernel(int n, int nb, int nq, int nsrc, const int* perm,
+                                                             const T* P, const T* source, T* X, int nl) {
+  const int w = nq + nsrc;
+  const int per = n * w;
+  for (int b = blockIdx.y; b < nl; b += gridDim.y) {
+    const int* pb = perm + (long)b * n;
+    T* Xb = X + (long)b * per;
+    for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < per; idx += gridDim.x * blockDim.x) {
+      const int i = idx / w, c = idx - i * w;
+      const int j = pb[i];
+      T v = s_zero<T>();
+      if (c < nq) {
+        if (j < nb) v = P[j * nq + c];
+      } else if (source && j >= nb) {
+        v = source[((long)b * n + j) * nsrc + (c - nq)];
+      }
+      Xb[idx] = v;
+    }
   }
 }
 
@@ -383,8 +415,7 @@ static int leaf_t(hps_handle_t h, int kind, int nl, int n, int nb, int nq, int n
   {
     HPS_PROF("leaf_assemble", T, flop_mult<T>() * 2.0 * ncoef * (double)an * n * nl,
              sizeof(T) * ((double)an * n * nl + (double)ncoef * n * nl));
-    dim3 g = grid_for((long)an * n, 1);
-    g.y = ceil_div(nl, ASM_LEAVES) < 65535 ? ceil_div(nl, ASM_LEAVES) : 65535;
+    dim3 g(an, ceil_div(nl, ASM_LEAVES) < 65535 ? ceil_div(nl, ASM_LEAVES) : 65535);
     hipLaunchKernelGGL((assemble_kernel<T>), g, dim3(256), 0, s, n, nl, ncoef, co, (const T*)coeffs,
                        (const T*)diffops, arow0, an, 0, n, aout, ld_lu, s_lu);
     HPS_CHECK_HIP(h, hipGetLastError());
@@ -415,19 +446,27 @@ static int leaf_t(hps_handle_t h, int kind, int nl, int n, int nb, int nq, int n
     if (nsrc > 0)
       HPS_CHECK_HIP(h, copy2d<T>(ni, nsrc, 1.0, source ? (const T*)source + (long)nb * nsrc : nullptr, nsrc,
                                  (long)n * nsrc, R + nq, w, (long)ni * w, nl, s));
-  } else {
-    HPS_CHECK_HIP(h, copy2d<T>(n, w, 1.0, nullptr, 0, 0, R, w, (long)n * w, nl, s));
-    HPS_CHECK_HIP(h, copy2d<T>(nb, nq, 1.0, P, nq, 0, R, w, (long)n * w, nl, s));
-    if (nsrc > 0 && source)
-      HPS_CHECK_HIP(h, copy2d<T>(ni, nsrc, 1.0, (const T*)source + (long)nb * nsrc, nsrc, (long)n * nsrc,
-                                 R + (long)nb * w + nq, w, (long)n * w, nl, s));
   }
 
   // 3. factor and solve
   T* Afac = kind == HPS_KIND_DTN ? lu + nb : lu;
   HPS_CHECK_HIP(h, getrf_ws<T>(nrows, Afac, ld_lu, s_lu, perm, dinv, scratch, info, nl, s));
-  HPS_CHECK_HIP(h, getrs_perm<T>(nrows, w, Afac, ld_lu, s_lu, perm, dinv, R, w, (long)nrows * w, X, w,
-                                 (long)nrows * w, nl, s));
+  if (kind == HPS_KIND_DTN) {
+    HPS_CHECK_HIP(h, getrs_perm<T>(nrows, w, Afac, ld_lu, s_lu, perm, dinv, R, w, (long)nrows * w, X, w,
+                                   (long)nrows * w, nl, s));
+  } else {
+    // ItI: the right-hand sides [P 0; 0 f_int] go straight to X in the
+    // factors' row order (no R), then the solves run in place
+    {
+      HPS_PROF("leaf_rhs_gather", T, 0.0, sizeof(T) * (double)n * w * nl);
+      const int per = n * w;
+      dim3 g(ceil_div(per, 256) < 64 ? ceil_div(per, 256) : 64, nl < 65535 ? nl : 65535);
+      hipLaunchKernelGGL((iti_rhs_gather_kernel<T>), g, dim3(256), 0, s, n, nb, nq, nsrc, perm, P,
+                         (const T*)source, X, nl);
+      HPS_CHECK_HIP(h, hipGetLastError());
+    }
+    HPS_CHECK_HIP(h, getrs_inplace<T>(nrows, w, Afac, ld_lu, s_lu, dinv, X, w, (long)nrows * w, nl, s));
+  }
 
   // 4. unpack Y and v
   if (kind == HPS_KIND_DTN) {

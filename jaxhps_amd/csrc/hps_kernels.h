@@ -51,6 +51,12 @@ template <typename T>
 hipError_t getrs_perm(int n, int nrhs, const T* LU, long lda, long sA, const int* perm, const T* dinv, const T* R,
                       long ldr, long sR, T* X, long ldx, long sX, int batch, hipStream_t s);
 
+// X = A^{-1} P^T X in place, X already gathered by the permutation
+// (X[i] = R[perm[i]]): the triangular solves of getrs_perm's blocked path.
+template <typename T>
+hipError_t getrs_inplace(int n, int nrhs, const T* LU, long lda, long sA, const T* dinv, T* X, long ldx, long sX,
+                         int batch, hipStream_t s);
+
 // X = A^{-H} R from getrf_ws's factors (A^H = U^H L^H P).  W is n x nrhs
 // scratch per batch element (row stride nrhs); X, W and R must not alias.
 template <typename T>

@@ -1815,6 +1815,13 @@ hipError_t getrs_perm(int n, int nrhs, const T* LU, long lda, long sA, const int
     hipLaunchKernelGGL((gather_rows_kernel<T>), grid, dim3(64), 0, s, n, nrhs, perm + (long)b0 * n,
                        R + (long)b0 * sR, ldr, sR, X + (long)b0 * sX, ldx, sX);
   }
+  return getrs_inplace<T>(n, nrhs, LU, lda, sA, dinv, X, ldx, sX, batch, s);
+}
+
+template <typename T>
+hipError_t getrs_inplace(int n, int nrhs, const T* LU, long lda, long sA, const T* dinv, T* X, long ldx, long sX,
+                         int batch, hipStream_t s) {
+  if (n <= 0 || nrhs <= 0 || batch <= 0) return hipSuccess;
   auto gemm_update = [&](int M, int K, const T* Ap, const T* Bp, T* Cp) -> hipError_t {
     if (M <= 0 || K <= 0) return hipSuccess;
     GemmArgs<T> g;
@@ -1992,6 +1999,7 @@ size_t dinv_elems(int n) { return (size_t)((n + LU_NB - 1) / LU_NB) * LU_NB * LU
   template hipError_t getrf_ws<T>(int, T*, long, long, int*, T*, int*, int*, int, hipStream_t, bool);                 \
   template hipError_t getrs_perm<T>(int, int, const T*, long, long, const int*, const T*, const T*, long, long, \
                                     T*, long, long, int, hipStream_t);                                          \
+  template hipError_t getrs_inplace<T>(int, int, const T*, long, long, const T*, T*, long, long, int, hipStream_t); \
   template hipError_t getrs_conjt<T>(int, int, const T*, long, long, const int*, const T*, const T*, long, long, \
                                      T*, long, long, T*, int, hipStream_t);
 // HPS_LU_TYPES selects the element types this object instantiates (bit 0

@@ -78,10 +78,15 @@ def test_scattering_system_and_impedance_match_oracle(n, nsrc):
     imp = _np(get_scattering_uscat_impedance(S, D, T, dirs, bdry, k, eta))
     assert imp.shape == (n, nsrc)
     assert _rel(imp, osc.uscat_impedance(S, D, T, dirs, bdry, k, eta)) < 1e-10
-    # D = I/2: imp = i eta (T^-1 du_in/dn - u_in)
+    # D = I/2: imp = i eta (T^-1 du_in/dn - u_in).  The identity holds to the
+    # rounding of the boundary-integral system (condition ~6e2 at n = 448):
+    # LAPACK through the oracle lands at 5e-11 from the closed form, so the
+    # bound is a few times that
     u, dn = osc.plane_wave_and_normals(k, bdry, dirs)
+    closed = 1j * eta * (np.linalg.solve(T, dn) - u)
     imp2 = _np(get_scattering_uscat_impedance(S, 0.5 * np.eye(n), T, dirs, bdry, k, eta))
-    assert _rel(imp2, 1j * eta * (np.linalg.solve(T, dn) - u)) < 1e-10
+    lapack = _rel(osc.uscat_impedance(S, 0.5 * np.eye(n), T, dirs, bdry, k, eta), closed)
+    assert _rel(imp2, closed) < max(1e-10, 4 * lapack)
 
 
 def _gauss_bump(x):
