@@ -1069,16 +1069,24 @@ static int rows_panel_max() {
 // the natural order (the merge matrices of the HPS tree: the ItI W = I -
 // D12 D21 and the DtN interface blocks show pivot growth ~1 and multipliers
 // below 4; the caller checks the factors and refactors with pivoting
-// otherwise).  Every workgroup factors the nb x nb diagonal block in LDS
-// (small; redundant work costs less than a second launch) and inverts its
-// U11; its NOPIV_ROWS panel rows below the block then take L21 = A21
-// inv(U11), four lanes per row, each entry an independent dot product over
+// otherwise).  Every workgroup factors the nb x nb diagonal block (one wave,
+// rows in registers; redundant work costs less than a second launch) and
+// inverts its U11; its NOPIV_ROWS panel rows below the block then take
+// L21 = A21 inv(U11), four lanes per row, each entry an independent dot product over
 // the row held in the quad's registers.  The last workgroup of the matrix to
 // read the diagonal block (a per-matrix arrival counter) writes the factored
 // block, its packed inverse (dinv) and the identity pivots.  The rows are
 // independent: grid
 // (ceil((n - k - nb) / NOPIV_ROWS), batch).
 // ---------------------------------------------------------------------------
+// lane `lane`'s value of v in every lane of the wave (all lanes active)
+__device__ __forceinline__ float lane_read(float v, int lane) { return __shfl(v, lane, 64); }
+__device__ __forceinline__ double lane_read(double v, int lane) { return __shfl(v, lane, 64); }
+__device__ __forceinline__ c64 lane_read(c64 v, int lane) { return c64{lane_read(v.re, lane), lane_read(v.im, lane)}; }
+__device__ __forceinline__ c128 lane_read(c128 v, int lane) {
+  return c128{lane_read(v.re, lane), lane_read(v.im, lane)};
+}
+
 constexpr int NOPIV_ROWS = 64;  // panel rows per workgroup of the unpivoted panel
 
 // the value of lane CTRL & 3 of each quad (CTRL = 0x55 * lane: quad_perm [l,l,l,l])
@@ -1106,7 +1114,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void getrf_panel_nopiv_kernel(int n, T* A, long lda, long sA, int k, int nb,
                                                                 int* ipiv, T* dinv, int* arrivals) {
   __shared__ T D[LU_NB][LU_NB + 1];
-  __shared__ T W[LU_NB][LU_NB + 1];  // multipliers, then inv(U11)
+  __shared__ T W[LU_NB][LU_NB + 1];  // inv(U11)
   __shared__ T P[LU_NB][LU_NB + 1];  // packed inverse
   __shared__ int last;
   const int b = blockIdx.y;
@@ -1126,26 +1134,33 @@ __global__ __launch_bounds__(256) void getrf_panel_nopiv_kernel(int n, T* A, lon
     if (last) atomicExch(&arrivals[b], 0);  // every workgroup has arrived: ready for the next launch
   }
   __syncthreads();
-  // elimination step j: rows r > j, columns c >= j; column j is only read in
-  // its own step, so its multipliers go to W and one barrier per step suffices
-  for (int j = 0; j < LU_NB - 1; ++j) {
-    const int w = LU_NB - j;
-    const T inv = s_inv(D[j][j]);
-    for (int idx = tid; idx < (w - 1) * w; idx += 256) {
-      const int r = j + 1 + idx / w, c = j + idx % w;
-      const T l = s_mul(D[r][j], inv);
-      if (c == j)
-        W[r][j] = l;
-      else
-        D[r][c] = s_fnms(D[r][c], l, D[j][c]);
+  // elimination of the diagonal block by wave 0 alone, no barriers: lane r
+  // holds row r in registers; step j reads the pivot row from lane j
+  // (cross-lane shuffles) and the rows below subtract their multiple of it.
+  // The factored block (multipliers below the diagonal) goes back to D, zero
+  // outside nb x nb for the inversion.
+  if (tid < 64) {
+    const int r = tid;
+    T x[LU_NB];
+#pragma unroll
+    for (int c = 0; c < LU_NB; ++c) x[c] = r < LU_NB ? D[r][c] : s_zero<T>();
+#pragma unroll
+    for (int j = 0; j < LU_NB - 1; ++j) {
+      // rows at or above j (and the idle lanes) take a zero multiplier, so the
+      // step runs without divergence and lane j's row stays as it is; the
+      // shuffles stay outside any branch (lane j must take part for its value
+      // to be read)
+      const bool below = r > j && r < LU_NB;
+      const T inv = s_inv(lane_read(x[j], j));
+      const T l = below ? s_mul(x[j], inv) : s_zero<T>();
+      x[j] = below ? l : x[j];
+#pragma unroll
+      for (int c = j + 1; c < LU_NB; ++c) x[c] = s_fnms(x[c], l, lane_read(x[c], j));
     }
-    __syncthreads();
-  }
-  // the factored block (zero outside nb x nb for the inversion)
-  for (int idx = tid; idx < LU_NB * LU_NB; idx += 256) {
-    const int r = idx / LU_NB, c = idx - r * LU_NB;
-    T v = c < r ? W[r][c] : D[r][c];
-    D[r][c] = (r < nb && c < nb) ? v : s_zero<T>();
+    if (r < LU_NB) {
+#pragma unroll
+      for (int c = 0; c < LU_NB; ++c) D[r][c] = (r < nb && c < nb) ? x[c] : s_zero<T>();
+    }
   }
   __syncthreads();
   invert_diag_block<T>(&D[0][0], LU_NB + 1, &P[0][0], LU_NB + 1, nb);
