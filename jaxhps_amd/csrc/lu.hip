@@ -1112,7 +1112,8 @@ __device__ __forceinline__ void nopiv_rows_accum(const T (&a)[8], T (&x)[8], con
 
 template <typename T>
 __global__ __launch_bounds__(256) void getrf_panel_nopiv_kernel(int n, T* A, long lda, long sA, int k, int nb,
-                                                                int* ipiv, T* dinv, int* arrivals) {
+                                                                int* ipiv, T* dinv, int* arrivals, int row_wgs,
+                                                                int u12_end) {
   __shared__ T D[LU_NB][LU_NB + 1];
   __shared__ T W[LU_NB][LU_NB + 1];  // inv(U11)
   __shared__ T P[LU_NB][LU_NB + 1];  // packed inverse
@@ -1170,11 +1171,30 @@ __global__ __launch_bounds__(256) void getrf_panel_nopiv_kernel(int n, T* A, lon
     W[r][c] = (r <= c && c < nb) ? P[r][c] : s_zero<T>();  // P holds garbage beyond nb
   }
   __syncthreads();
-  // panel rows below the block: x = a inv(U11), four lanes per row, each
-  // holding eight consecutive entries of the row; entry i comes from lane
-  // i / 8 of the quad by a DPP broadcast, and each lane forms the row's
-  // outputs 8 sub .. 8 sub + 7
-  {
+  if (blockIdx.x >= row_wgs) {
+    // U12 = inv(L11) A12 for 32 columns of the block row right of the panel
+    // (columns up to u12_end): the chunk staged in W (free here), then each
+    // entry a dot product with the unit-lower inv(L11) held in P
+    const int c0 = k + nb + (int)(blockIdx.x - row_wgs) * LU_NB;
+    const int wc = u12_end - c0 < LU_NB ? u12_end - c0 : LU_NB;
+    for (int idx = tid; idx < LU_NB * LU_NB; idx += 256) {
+      const int r = idx / LU_NB, c = idx - r * LU_NB;
+      W[r][c] = (r < nb && c < wc) ? Ab[(long)(k + r) * lda + c0 + c] : s_zero<T>();
+    }
+    __syncthreads();
+    for (int idx = tid; idx < LU_NB * LU_NB; idx += 256) {
+      const int r = idx / LU_NB, c = idx - r * LU_NB;
+      if (r < nb && c < wc) {
+        T acc = W[r][c];
+        for (int q = 0; q < r; ++q) acc = s_fma(P[r][q], W[q][c], acc);
+        Ab[(long)(k + r) * lda + c0 + c] = acc;
+      }
+    }
+  } else {
+    // panel rows below the block: x = a inv(U11), four lanes per row, each
+    // holding eight consecutive entries of the row; entry i comes from lane
+    // i / 8 of the quad by a DPP broadcast, and each lane forms the row's
+    // outputs 8 sub .. 8 sub + 7
     const int sub = tid & 3;
     const int rg = k + nb + blockIdx.x * NOPIV_ROWS + (tid >> 2);
     // a quad's lanes share their row, so whole quads are live or not and the
@@ -1408,30 +1428,35 @@ hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, T* dinv, int* scr
     const int bend = n - k0 < nbo ? n : k0 + nbo;
     for (int k = k0; k < bend; k += LU_NB) {
       const int nb = bend - k < LU_NB ? bend - k : LU_NB;
-      bool swapped = false;
+      bool swapped = false, u12_done = false;
       hipError_t e = hipSuccess;
       if (nopiv && (n - k > rows_panel_max<T>() || batch <= LU_NOPIV_SMALL_BATCH)) {
         static const std::string nm_np = std::string("getrf_panel_nopiv<") + tname<T>() + ">";
         const int m = n - k;
         prof::Scope ps(s, nm_np.c_str(), flop_mult<T>() * batch * ((double)m * nb * nb - (double)nb * nb * nb / 3.0),
                        2.0 * sizeof(T) * (double)m * nb * batch);
+        // row workgroups, then one workgroup per 32 columns of the block's U12
+        const int row_wgs = ceil_div(m - nb, NOPIV_ROWS) > 0 ? ceil_div(m - nb, NOPIV_ROWS) : 1;
+        const int u12_wgs = bend - k - nb > 0 ? ceil_div(bend - k - nb, LU_NB) : 0;
         for (int b0 = 0; b0 < batch; b0 += 65535) {
           const int bc = batch - b0 < 65535 ? batch - b0 : 65535;
-          hipLaunchKernelGGL((getrf_panel_nopiv_kernel<T>),
-                             dim3(ceil_div(m - nb, NOPIV_ROWS) > 0 ? ceil_div(m - nb, NOPIV_ROWS) : 1, bc),
-                             dim3(256), 0, s, n, A + (long)b0 * sA, lda, sA, k, nb, ipiv + (long)b0 * n,
-                             dinv + (long)b0 * ((n + LU_NB - 1) / LU_NB) * LU_NB * LU_NB, arrivals + b0);
+          hipLaunchKernelGGL((getrf_panel_nopiv_kernel<T>), dim3(row_wgs + u12_wgs, bc), dim3(256), 0, s, n,
+                             A + (long)b0 * sA, lda, sA, k, nb, ipiv + (long)b0 * n,
+                             dinv + (long)b0 * ((n + LU_NB - 1) / LU_NB) * LU_NB * LU_NB, arrivals + b0, row_wgs,
+                             bend);
         }
         e = hipGetLastError();
-        swapped = true;  // nothing to swap: U12 is formed below like after a register panel
+        swapped = true;  // nothing to swap
+        u12_done = true;
       } else {
         e = launch_panel<T>(n, A, lda, sA, k, nb, ipiv, info, dinv, swtab, cand, batch, s, &swapped);
       }
       if (e != hipSuccess) return e;
       const int others = n - nb;
       if (swapped) {
-        // U12 = inv(L11) A12 for the block's columns right of the panel
-        if (bend - k - nb > 0) {
+        // U12 = inv(L11) A12 for the block's columns right of the panel (the
+        // unpivoted panel kernel forms it itself)
+        if (bend - k - nb > 0 && !u12_done) {
           static const std::string nm = std::string("getrf_u12<") + tname<T>() + ">";
           e = trsm_block<T>(false, n, bend - k - nb, dinv, A + k + nb, lda, sA, k, nb, batch, s, nm);
           if (e != hipSuccess) return e;
