@@ -64,6 +64,12 @@ constexpr long LU_TSLU_MAX_WORK = 64L * 1024;
 // update only their block; the rest is updated once per block by a GEMM with
 // K = the block width.
 static int lu_outer_block(int n) { return n > LU_BLOCKED_MIN ? LU_NBO : (n >= 256 ? 128 : n); }
+// the factorisation and the many-column solves of large matrices take column
+// blocks of 512: their trailing updates are single large GEMMs whose K = 512
+// halves the C-tile traffic per flop of K = 256 (the few-column solves keep
+// 256-blocks, which their block kernel holds)
+constexpr int LU_WIDE_MIN = 3072;
+static int lu_outer_block_wide(int n) { return n >= LU_WIDE_MIN ? 2 * LU_NBO : lu_outer_block(n); }
 
 static __global__ void fill_int_kernel(int* p, long n, int v) {
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1423,7 +1429,7 @@ hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, T* dinv, int* scr
   // Large matrices are factored in column blocks of LU_NBO: inside a block the
   // 32-wide panels update only the block's columns; the block row's U12 and the
   // trailing update then run once per block, the latter as a GEMM with K = LU_NBO.
-  const int nbo = lu_outer_block(n);
+  const int nbo = lu_outer_block_wide(n);
   for (int k0 = 0; k0 < n; k0 += nbo) {
     const int bend = n - k0 < nbo ? n : k0 + nbo;
     for (int k = k0; k < bend; k += LU_NB) {
@@ -1888,9 +1894,10 @@ hipError_t getrs_inplace(int n, int nrhs, const T* LU, long lda, long sA, const 
   // Column blocks of LU_NBO for large matrices (as in getrf): the 32-row
   // steps update only the rows of their block; the rest of X is updated once
   // per block by a GEMM with K = LU_NBO.
-  const int nbo = lu_outer_block(n);
   // few right-hand sides: each outer block's inner steps in one kernel
-  const bool few = nrhs <= LU_SMALL_MAX_RHS && (size_t)nbo * nrhs * sizeof(T) <= 64 * 1024;
+  const int nbo_few = lu_outer_block(n);
+  const bool few = nrhs <= LU_SMALL_MAX_RHS && (size_t)nbo_few * nrhs * sizeof(T) <= 64 * 1024;
+  const int nbo = few ? nbo_few : lu_outer_block_wide(n);
   for (int k0 = 0; k0 < n; k0 += nbo) {
     const int bend = n - k0 < nbo ? n : k0 + nbo;
     if (few) {
