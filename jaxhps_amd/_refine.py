@@ -93,7 +93,12 @@ class RefineOps:
         nl, n, w = u.shape
         if w == 1:
             return _lib.gemm(u.view(nl, n), self.MT).view(nl, self.nrow_R, 1)
-        return _lib.gemm(self.M, u)
+        if w >= 16:
+            return _lib.gemm(self.M, u)  # full 16-column product tiles
+        # a few sources: one product over every (leaf, source) row rather
+        # than nl products with w columns each (w of 16 tile columns used)
+        ut = u.transpose(1, 2).reshape(nl * w, n).contiguous()
+        return _lib.gemm(ut, self.MT).view(nl, w, self.nrow_R).transpose(1, 2).contiguous()
 
 
 def root_points(dom) -> int:
