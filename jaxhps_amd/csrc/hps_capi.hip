@@ -564,6 +564,8 @@ size_t hps_merge_workspace(int dtype, int kind, int nm, int ni, int ne, int nsrc
   tot += kind == HPS_KIND_ITI ? 2 * align_up((size_t)nm * nF * nF * es) : 0;
   tot += align_up((size_t)nm * ne * ni * es);
   tot += align_up(getrf_scratch_bytes(nfac, nm));
+  if ((dtype == HPS_F32 || dtype == HPS_C64) && nm <= BLOCKINV_MAX_BATCH)
+    tot += align_up(blockinv_workspace_elems(nfac, ne) * es);
   return tot;
 }
 
@@ -711,6 +713,11 @@ static int merge_t(hps_handle_t h, int kind, int nm, int nc, int nec, int ni, in
   T* D12w = iti ? (T*)take((size_t)nm * nF * nF * es) : nullptr;
   T* D21w = iti ? (T*)take((size_t)nm * nF * nF * es) : nullptr;
   T* Bw = (T*)take((size_t)nm * ne * ni * es);
+  constexpr bool lowp = std::is_same<T, float>::value || std::is_same<T, c64>::value;
+  const int nfac = iti ? nF : ni;
+  T* binv = lowp && nm <= BLOCKINV_MAX_BATCH && blockinv_workspace_elems(nfac, ne)
+                ? (T*)take(blockinv_workspace_elems(nfac, ne) * es)
+                : nullptr;
   int* scratch = (int*)wp;
   T* Fl = (T*)F_lu;
   T* dinv = (T*)F_dinv;
@@ -752,14 +759,16 @@ static int merge_t(hps_handle_t h, int kind, int nm, int nc, int nec, int ni, in
     }
     const long sY = (long)ni * w;
     if (!iti) {
-      HPS_CHECK_HIP(h, getrs_perm<T>(ni, w, Fl, ni, (long)ni * ni, perm, dinv, Yp, w, sY, out, w, sY, nm, s));
+      HPS_CHECK_HIP(h, getrs_perm_blockinv<T>(ni, w, Fl, ni, (long)ni * ni, perm, dinv, Yp, w, sY, out, w, sY, nm,
+                                              pass == 0 ? binv : nullptr, s));
     } else {
       const long sZ = (long)nF * w;
       // Z = Y_F - D12 Y_G
       HPS_CHECK_HIP(h, coupling_gemm<T>(s, n_d12, d12_blk, nF, w, -1.0, D12, sNF2, Yp + (long)nF * w, w, sY, Yp, w, sY,
                                         Zp, w, sZ, nm));
       // X_F = W^{-1} Z  (into Y_F)
-      HPS_CHECK_HIP(h, getrs_perm<T>(nF, w, Fl, nF, sNF2, perm, dinv, Zp, w, sZ, Yp, w, sY, nm, s));
+      HPS_CHECK_HIP(h, getrs_perm_blockinv<T>(nF, w, Fl, nF, sNF2, perm, dinv, Zp, w, sZ, Yp, w, sY, nm,
+                                              pass == 0 ? binv : nullptr, s));
       // X_G = Y_G - D21 X_F  (in place)
       HPS_CHECK_HIP(h, coupling_gemm<T>(s, n_d21, d21_blk, nF, w, -1.0, D21, sNF2, Yp, w, sY, Yp + (long)nF * w, w,
                                         sY, Yp + (long)nF * w, w, sY, nm));

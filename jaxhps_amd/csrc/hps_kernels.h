@@ -57,6 +57,23 @@ template <typename T>
 hipError_t getrs_inplace(int n, int nrhs, const T* LU, long lda, long sA, const T* dinv, T* X, long ldx, long sX,
                          int batch, hipStream_t s);
 
+// Many-column solves of large low-precision factors through the inverses of
+// their 512 x 512 diagonal blocks (products with K = 512 instead of 32-row
+// steps).  Eligible: n >= LU_WIDE_MIN, n a multiple of 512, more than 16
+// columns; wsp holds blockinv_workspace_elems(n, nrhs) elements (otherwise,
+// or with wsp == nullptr, this is getrs_perm).
+constexpr int LU_WIDE_MIN = 3072;
+constexpr int BLOCKINV_NB = 512;
+constexpr int BLOCKINV_MAX_BATCH = 2;  // the inverses are formed matrix by matrix: top levels only
+inline bool blockinv_eligible(int n, int nrhs) { return n >= LU_WIDE_MIN && n % BLOCKINV_NB == 0 && nrhs > 16; }
+inline size_t blockinv_workspace_elems(int n, int nrhs) {
+  return blockinv_eligible(n, nrhs) ? 2 * (size_t)n * BLOCKINV_NB + (size_t)BLOCKINV_NB * nrhs : 0;
+}
+template <typename T>
+hipError_t getrs_perm_blockinv(int n, int nrhs, const T* LU, long lda, long sA, const int* perm, const T* dinv,
+                               const T* R, long ldr, long sR, T* X, long ldx, long sX, int batch, T* wsp,
+                               hipStream_t s);
+
 // X = A^{-H} R from getrf_ws's factors (A^H = U^H L^H P).  W is n x nrhs
 // scratch per batch element (row stride nrhs); X, W and R must not alias.
 template <typename T>

@@ -68,7 +68,7 @@ static int lu_outer_block(int n) { return n > LU_BLOCKED_MIN ? LU_NBO : (n >= 25
 // blocks of 512: their trailing updates are single large GEMMs whose K = 512
 // halves the C-tile traffic per flop of K = 256 (the few-column solves keep
 // 256-blocks, which their block kernel holds)
-constexpr int LU_WIDE_MIN = 3072;
+static_assert(LU_WIDE_MIN == 3072 && BLOCKINV_NB == 2 * LU_NBO, "block sizes");
 static int lu_outer_block_wide(int n) { return n >= LU_WIDE_MIN ? 2 * LU_NBO : lu_outer_block(n); }
 
 static __global__ void fill_int_kernel(int* p, long n, int v) {
@@ -1940,6 +1940,125 @@ hipError_t getrs_inplace(int n, int nrhs, const T* LU, long lda, long sA, const 
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Many-column solves with large factors in low precision (the S solves of the
+// top merge levels, refined in FP64 afterwards): the inverses of the 512 x 512
+// diagonal blocks of L (unit lower) and U are formed once per solve (all
+// blocks of a matrix as one batched 512-column triangular solve of the
+// identity), after which every block step is a product with K = 512 instead
+// of sixteen 32-row steps whose updates stream the block's columns through
+// HBM.  The products read the explicit block inverses, so the rounding is
+// that of the inverses (cond(L_kk), cond(U_kk): pivot growth ~1 for the merge
+// matrices), corrected by the refinement.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void eye_kernel(T* X, int n, long sX, int batch) {
+  const long per = (long)n * n;
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < per * batch;
+       idx += (long)gridDim.x * blockDim.x) {
+    const long b = idx / per, e = idx - b * per;
+    const int r = (int)(e / n), c = (int)(e - (long)r * n);
+    X[b * sX + e] = r == c ? s_one<T>() : s_zero<T>();
+  }
+}
+
+template <typename T>
+hipError_t getrs_perm_blockinv(int n, int nrhs, const T* LU, long lda, long sA, const int* perm, const T* dinv,
+                               const T* R, long ldr, long sR, T* X, long ldx, long sX, int batch, T* wsp,
+                               hipStream_t s) {
+  if (!blockinv_eligible(n, nrhs) || wsp == nullptr)
+    return getrs_perm<T>(n, nrhs, LU, lda, sA, perm, dinv, R, ldr, sR, X, ldx, sX, batch, s);
+  const int NB = BLOCKINV_NB;
+  const int nblk = n / NB;
+  const long sBlk = (long)NB * NB;  // one block inverse
+  T* Linv = wsp;
+  T* Uinv = wsp + (long)nblk * sBlk;
+  T* tmp = Uinv + (long)nblk * sBlk;  // NB x nrhs
+  const long dstride = (long)((n + LU_NB - 1) / LU_NB) * LU_NB * LU_NB;  // dinv per matrix
+  for (int b0 = 0; b0 < batch; b0 += 65535) {
+    static const std::string nm = std::string("getrs_gather<") + tname<T>() + ">";
+    prof::Scope ps(s, nm.c_str(), 0.0, 2.0 * sizeof(T) * (double)n * nrhs * batch);
+    int bc = batch - b0 < 65535 ? batch - b0 : 65535;
+    dim3 grid(ceil_div(nrhs, 64), n, bc);
+    hipLaunchKernelGGL((gather_rows_kernel<T>), grid, dim3(64), 0, s, n, nrhs, perm + (long)b0 * n,
+                       R + (long)b0 * sR, ldr, sR, X + (long)b0 * sX, ldx, sX);
+  }
+  auto gemm = [&](int M, int N, int K, double alpha, double beta, const T* A, long la, long sa, const T* B, long lb,
+                  long sb, const T* C, long lc, long sc, T* D, long ld, long sd, int bt, int tri) -> hipError_t {
+    if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
+    GemmArgs<T> g;
+    g.M = M;
+    g.N = N;
+    g.K = K;
+    g.alpha = alpha;
+    g.beta = beta;
+    g.A = A;
+    g.lda = la;
+    g.sA = sa;
+    g.B = B;
+    g.ldb = lb;
+    g.sB = sb;
+    g.C = C;
+    g.ldc = lc;
+    g.sC = sc;
+    g.D = D;
+    g.ldd = ld;
+    g.sD = sd;
+    g.batch = bt;
+    g.triA = tri;
+    return hps::gemm<T>(g, s);
+  };
+  static const std::string nm_inv = std::string("blockinv_form<") + tname<T>() + ">";
+  for (int b = 0; b < batch; ++b) {
+    const T* Lb = LU + (long)b * sA;
+    const T* db = dinv + (long)b * dstride;
+    T* Xb = X + (long)b * sX;
+    const long sDiag = (long)NB * (lda + 1);  // between consecutive diagonal blocks of the factors
+    const long sDinv = (long)(NB / LU_NB) * LU_NB * LU_NB;
+    // block inverses: Linv_j = inv(L_jj), Uinv_j = inv(U_jj), all j at once
+    {
+      prof::Scope ps(s, nm_inv.c_str(), 0.0, 2.0 * sizeof(T) * (double)nblk * sBlk);
+      hipLaunchKernelGGL((eye_kernel<T>), dim3(1024), dim3(256), 0, s, Linv, NB, sBlk, nblk);
+      hipLaunchKernelGGL((eye_kernel<T>), dim3(1024), dim3(256), 0, s, Uinv, NB, sBlk, nblk);
+    }
+    HPS_LU_CHECK(hipGetLastError());
+    for (int k = 0; k < NB; k += LU_NB) {
+      // X_k = inv(L_kk) X_k, then the rows below within the block
+      HPS_LU_CHECK(gemm(LU_NB, NB, LU_NB, 1.0, 0.0, db + (long)(k / LU_NB) * LU_NB * LU_NB, LU_NB, sDinv,
+                        Linv + (long)k * NB, NB, sBlk, nullptr, 0, 0, Linv + (long)k * NB, NB, sBlk, nblk, 1));
+      HPS_LU_CHECK(gemm(NB - k - LU_NB, NB, LU_NB, -1.0, 1.0, Lb + (long)(k + LU_NB) * lda + k, lda, sDiag,
+                        Linv + (long)k * NB, NB, sBlk, Linv + (long)(k + LU_NB) * NB, NB, sBlk,
+                        Linv + (long)(k + LU_NB) * NB, NB, sBlk, nblk, 0));
+    }
+    for (int k = NB - LU_NB; k >= 0; k -= LU_NB) {
+      HPS_LU_CHECK(gemm(LU_NB, NB, LU_NB, 1.0, 0.0, db + (long)(k / LU_NB) * LU_NB * LU_NB, LU_NB, sDinv,
+                        Uinv + (long)k * NB, NB, sBlk, nullptr, 0, 0, Uinv + (long)k * NB, NB, sBlk, nblk, 2));
+      HPS_LU_CHECK(gemm(k, NB, LU_NB, -1.0, 1.0, Lb + k, lda, sDiag, Uinv + (long)k * NB, NB, sBlk, Uinv, NB, sBlk,
+                        Uinv, NB, sBlk, nblk, 0));
+    }
+    // forward: X_j = Linv_j X_j, then the rows below
+    for (int j = 0; j < nblk; ++j) {
+      const int k0 = j * NB;
+      HPS_LU_CHECK(gemm(NB, nrhs, NB, 1.0, 0.0, Linv + (long)j * sBlk, NB, 0, Xb + (long)k0 * ldx, ldx, 0, nullptr,
+                        0, 0, tmp, nrhs, 0, 1, 0));
+      HPS_LU_CHECK(copy2d<T>(NB, nrhs, 1.0, tmp, nrhs, 0, Xb + (long)k0 * ldx, ldx, 0, 1, s));
+      HPS_LU_CHECK(gemm(n - k0 - NB, nrhs, NB, -1.0, 1.0, Lb + (long)(k0 + NB) * lda + k0, lda, 0,
+                        Xb + (long)k0 * ldx, ldx, 0, Xb + (long)(k0 + NB) * ldx, ldx, 0,
+                        Xb + (long)(k0 + NB) * ldx, ldx, 0, 1, 0));
+    }
+    // backward: X_j = Uinv_j X_j, then the rows above
+    for (int j = nblk - 1; j >= 0; --j) {
+      const int k0 = j * NB;
+      HPS_LU_CHECK(gemm(NB, nrhs, NB, 1.0, 0.0, Uinv + (long)j * sBlk, NB, 0, Xb + (long)k0 * ldx, ldx, 0, nullptr,
+                        0, 0, tmp, nrhs, 0, 1, 0));
+      HPS_LU_CHECK(copy2d<T>(NB, nrhs, 1.0, tmp, nrhs, 0, Xb + (long)k0 * ldx, ldx, 0, 1, s));
+      HPS_LU_CHECK(gemm(k0, nrhs, NB, -1.0, 1.0, Lb + k0, lda, 0, Xb + (long)k0 * ldx, ldx, 0, Xb, ldx, 0, Xb, ldx,
+                        0, 1, 0));
+    }
+  }
+  return hipGetLastError();
+}
+
 // X[b][perm[b][i]][c] = W[b][i][c]  (the inverse of gather_rows_kernel)
 template <typename T>
 __global__ void scatter_rows_kernel(int n, int nrhs, const int* perm, const T* W, long ldw, long sW, T* X, long ldx,
@@ -2047,6 +2166,8 @@ size_t dinv_elems(int n) { return (size_t)((n + LU_NB - 1) / LU_NB) * LU_NB * LU
   template hipError_t getrs_perm<T>(int, int, const T*, long, long, const int*, const T*, const T*, long, long, \
                                     T*, long, long, int, hipStream_t);                                          \
   template hipError_t getrs_inplace<T>(int, int, const T*, long, long, const T*, T*, long, long, int, hipStream_t); \
+  template hipError_t getrs_perm_blockinv<T>(int, int, const T*, long, long, const int*, const T*, const T*, long,     \
+                                             long, T*, long, long, int, T*, hipStream_t);                             \
   template hipError_t getrs_conjt<T>(int, int, const T*, long, long, const int*, const T*, const T*, long, long, \
                                      T*, long, long, T*, int, hipStream_t);
 // HPS_LU_TYPES selects the element types this object instantiates (bit 0

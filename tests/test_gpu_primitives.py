@@ -178,6 +178,38 @@ def test_getrs_conjt_solves_adjoint_system(gpu, dtype, n, nrhs, batch):
     assert res.abs().max().item() < TOL[dtype] * 50 * n**0.5
 
 
+@pytest.mark.parametrize("kind,dtype,m", [("iti", torch.complex64, 1536), ("dtn", torch.float32, 768)])
+def test_block_inverse_solves_match_fp64_merge(gpu, kind, dtype, m):
+    """A merge level whose factored matrix has 3072 unknowns in low precision
+    takes its many-column solve through the inverses of the 512 x 512
+    diagonal blocks; S, T, g_tilde and h agree with the same merge in double
+    precision (the standard blocked solve) to the low precision's accuracy."""
+    from jaxhps_amd._stages import merge_level
+    from jaxhps_amd._tables import quad_merge_tables
+
+    ne = 4 * m
+    print(f"block-inverse merge {kind} {dtype} m={m}", flush=True)
+    g = torch.Generator(device=gpu).manual_seed(5)
+    X = torch.randn(4, ne, ne, generator=g, device=gpu, dtype=torch.float64) / ne**0.5
+    if kind == "iti":
+        Y = torch.randn(4, ne, ne, generator=g, device=gpu, dtype=torch.float64) / ne**0.5
+        T64 = (0.5 * torch.complex(X, Y)).contiguous()
+        high = torch.complex128
+    else:
+        T64 = (2.0 * torch.eye(ne, device=gpu, dtype=torch.float64) + 0.1 * X).contiguous()
+        high = torch.float64
+    del X
+    h64 = torch.randn(4, ne, 2, generator=g, device=gpu, dtype=torch.float64).to(T64.dtype).contiguous()
+    tb = quad_merge_tables(m, kind)
+    lo = merge_level(T64.to(dtype).contiguous(), h64.to(dtype).contiguous(), tb)
+    hi = merge_level(T64.to(high), h64.to(high), tb)
+    nfac = lo.F_lu.shape[-1]
+    assert nfac >= 3072 and nfac % 512 == 0
+    for a, b in ((lo.S, hi.S), (lo.T, hi.T), (lo.gt, hi.gt), (lo.h, hi.h)):
+        err = float((a.to(high) - b).abs().max() / b.abs().max())
+        assert err < 2e-4, err
+
+
 def test_unpivoted_panels_of_many_matrices(gpu):
     """96 complex64 interface matrices of 1200 unknowns: the leading panels
     (taller than 1024 rows) unpivoted, the rest partially pivoted, with more
