@@ -27,7 +27,8 @@ def _ref64(t):
     "batch,M,N,K",
     [(1, 16, 16, 4), (3, 64, 64, 16), (5, 56, 56, 256), (2, 130, 70, 33), (7, 196, 72, 60), (2, 100, 1, 300),
      (3, 77, 16, 45), (4, 33, 3, 7), (1, 300, 257, 129), (6, 200, 2, 70), (3, 150, 9, 256),
-     (3, 90, 1, 56), (2, 131, 2, 40), (2, 37, 2, 700), (1, 5, 1, 3)],
+     (3, 90, 1, 56), (2, 131, 2, 40), (2, 37, 2, 700), (1, 5, 1, 3), (1, 300, 4, 100), (2, 700, 3, 600),
+     (1, 129, 4, 50)],
 )
 def test_gemm_matches_reference(gpu, dtype, batch, M, N, K):
     from jaxhps_amd import _lib
