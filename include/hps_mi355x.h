@@ -150,6 +150,15 @@ int hps_leaf_solve(hps_handle_t h, int dtype, int kind, int nl, int n, int nb, i
  * LAPACK's pivoted LU inside jnp.linalg.inv
  * (merge/_schur_complement.py:107,146). */
 int hps_set_merge_pivoting(hps_handle_t h, int partial_pivoting);
+/* Pivoting of hps_leaf_solve's factorisation on this handle: 1 (default) =
+ * partial pivoting; 0 = natural order, every panel checked in the kernel:
+ * a leaf whose largest multiplier exceeds 64 (or is not finite) gets
+ * info[l] = -1, and the caller reruns the leaf stage with pivoting.  The
+ * row-scaled ItI leaf matrices and the DtN interior blocks eliminate with
+ * multipliers below 2 on the benchmark workloads.  The reference factors
+ * with LAPACK's pivoted LU inside jnp.linalg.inv
+ * (local_solve/_uniform_2D_ItI.py:167, local_solve/_uniform_2D_DtN.py:257). */
+int hps_set_leaf_pivoting(hps_handle_t h, int partial_pivoting);
 size_t hps_merge_workspace(int dtype, int kind, int nm, int ni, int ne, int nsrc);
 int hps_merge_level(hps_handle_t h, int dtype, int kind, int nm, int nc, int nec, int ni, int ne, int nsrc,
                     const void* T_in, const void* h_in, const int* int_rows, const int* int_cols, const int* ext,

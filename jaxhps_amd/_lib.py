@@ -48,6 +48,7 @@ SIGNATURES = {
         [_P, _I, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _S],
     ),
     "hps_set_merge_pivoting": (_I, [_P, _I]),
+    "hps_set_leaf_pivoting": (_I, [_P, _I]),
     "hps_merge_workspace": (_S, [_I, _I, _I, _I, _I, _I]),
     "hps_merge_level": (
         _I,

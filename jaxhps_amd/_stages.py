@@ -176,12 +176,25 @@ def leaf_solve(pde, kind: int, dtype, device, source=None, nsrc_zero=False, leaf
         rs = row_scale.to(real_dtype(dtype))
         G = (G * rs).contiguous()
         P = (P * rs).contiguous()
-    rc = lib.hps_leaf_solve(
-        h, dtype_code(dtype), kind, nl, n, nb, nq, nsrc, len(clist), cop,
-        ptr(coeffs), ptr(ops["diffops"]), ptr(P), ptr(Q), ptr(G), ptr(src),
-        ptr(Y), ptr(T), ptr(v), ptr(hv), ptr(lu), ptr(perm), ptr(dinv), ptr(info), ptr(ws), ws.numel(),
-    )
-    check(rc, h, "hps_leaf_solve")
+
+    def run():
+        rc = lib.hps_leaf_solve(
+            h, dtype_code(dtype), kind, nl, n, nb, nq, nsrc, len(clist), cop,
+            ptr(coeffs), ptr(ops["diffops"]), ptr(P), ptr(Q), ptr(G), ptr(src),
+            ptr(Y), ptr(T), ptr(v), ptr(hv), ptr(lu), ptr(perm), ptr(dinv), ptr(info), ptr(ws), ws.numel(),
+        )
+        check(rc, h, "hps_leaf_solve")
+
+    # natural-order factorisation (the leaf matrices eliminate with multipliers
+    # below 2), checked per leaf in the panel kernels: a rejected leaf
+    # (info = -1) reruns the whole stage with partial pivoting
+    lib.hps_set_leaf_pivoting(h, 0)
+    try:
+        run()
+    finally:
+        lib.hps_set_leaf_pivoting(h, 1)
+    if bool((info < 0).any()):
+        run()
     if leaf_half_sides is not None:
         scale_batched(T, inv)
         if hv is not None:

@@ -276,6 +276,12 @@ int hps_set_merge_pivoting(hps_handle_t h, int partial_pivoting) {
   return HPS_OK;
 }
 
+int hps_set_leaf_pivoting(hps_handle_t h, int partial_pivoting) {
+  if (!h) return HPS_ERR_ARG;
+  h->leaf_nopiv = partial_pivoting == 0;
+  return HPS_OK;
+}
+
 int hps_destroy(hps_handle_t h) {
   delete h;
   return HPS_OK;
@@ -450,7 +456,7 @@ static int leaf_t(hps_handle_t h, int kind, int nl, int n, int nb, int nq, int n
 
   // 3. factor and solve
   T* Afac = kind == HPS_KIND_DTN ? lu + nb : lu;
-  HPS_CHECK_HIP(h, getrf_ws<T>(nrows, Afac, ld_lu, s_lu, perm, dinv, scratch, info, nl, s));
+  HPS_CHECK_HIP(h, getrf_ws<T>(nrows, Afac, ld_lu, s_lu, perm, dinv, scratch, info, nl, s, false, h->leaf_nopiv));
   if (kind == HPS_KIND_DTN) {
     HPS_CHECK_HIP(h, getrs_perm<T>(nrows, w, Afac, ld_lu, s_lu, perm, dinv, R, w, (long)nrows * w, X, w,
                                    (long)nrows * w, nl, s));

@@ -35,6 +35,7 @@ struct hps_handle_s {
   hipStream_t stream = nullptr;
   int device = -1;  // the stream's device at creation
   bool merge_nopiv = false;  // hps_merge_level factors without row exchanges
+  bool leaf_nopiv = false;   // hps_leaf_solve factors without row exchanges (checked)
   std::string err;
 };
 

@@ -39,11 +39,14 @@ hipError_t gemm(const GemmArgs<T>& a, hipStream_t s);
 // column j whose pivot candidates are all exactly zero.
 size_t getrf_scratch_bytes(int n, int batch);
 size_t dinv_elems(int n);
-// nopiv: no row exchanges (identity perm, info untouched); for matrices whose
-// natural-order elimination is stable -- the caller checks the factors.
+// nopiv: the tall panels (and every panel of at most 16 matrices) without row
+// exchanges; for matrices whose natural-order elimination is stable -- the
+// caller checks the factors.  nopiv_all: every register-sized panel without
+// row exchanges, checked in the kernel: a matrix with a multiplier above 64
+// (or not finite) gets info = -1 and must be refactored with pivoting.
 template <typename T>
 hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, T* dinv, int* scratch, int* info, int batch,
-                    hipStream_t s, bool nopiv = false);
+                    hipStream_t s, bool nopiv = false, bool nopiv_all = false);
 
 // X = A^{-1} R from getrf_ws's factors.  R (n x nrhs, row stride ldr, batch
 // stride sR) is read only; X must not alias R.

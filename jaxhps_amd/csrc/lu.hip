@@ -45,6 +45,9 @@ constexpr int LU_NB = 32;
 // for every panel: the pivoted register panels then run one workgroup per
 // matrix on an otherwise idle GPU
 constexpr int LU_NOPIV_SMALL_BATCH = 16;
+// largest multiplier an unpivoted register panel accepts (the leaf and merge
+// matrices of the HPS tree stay below 2)
+constexpr double LU_NOPIV_BOUND = 64.0;
 constexpr size_t LU_PANEL_LDS_MAX = 150 * 1024;
 constexpr int SWTAB = 1 + 4 * LU_NB;  // count, then up to 2*NB (row, src) int pairs
 template <typename T> __device__ __forceinline__ T make_conj(T x) { return x; }
@@ -644,13 +647,42 @@ struct PanelLDS {
 // mask) runs after the step with the pivot's position pq (J when no candidate
 // is left: the column is NaN or no live row remains) and the mask of this
 // thread's rows that held it.
-template <typename T, int RPT, int NW, int J, class F>
+template <typename T, int RPT, int NW, int J, bool NOPIV, class F>
 __device__ __forceinline__ void panel_step(T (&x)[RPT][LU_NB], int (&pos)[RPT], const bool (&live)[RPT],
-                                           PanelLDS<T, NW>& sh, int nb, F& on_pivot) {
+                                           PanelLDS<T, NW>& sh, int nb, F& on_pivot,
+                                           typename real_of<T>::type& lmax) {
   using R = typename real_of<T>::type;
   constexpr int NONE = 0x7fffffff;
   constexpr int par = J & 1;
   if (J >= nb) return;
+  if constexpr (NOPIV) {
+    // natural order: the row at position J is the pivot row; lmax keeps the
+    // largest multiplier magnitude for the caller's stability check
+#pragma unroll
+    for (int r = 0; r < RPT; ++r)
+      if (live[r] && pos[r] == J) {
+#pragma unroll
+        for (int c = J; c < LU_NB; ++c) sh.crow[par][0][c] = x[r][c];
+      }
+    __syncthreads();
+    const T* prow = sh.crow[par][0];
+    const T piv = prow[J];
+    on_pivot(J, J, s_abs1(piv), 0u);
+    if (!s_is_zero(piv)) {
+      const T inv = s_inv_uniform(piv);
+#pragma unroll
+      for (int r = 0; r < RPT; ++r)
+        if (live[r] && pos[r] > J) {
+          const T l = s_mul(x[r][J], inv);
+          const R al = s_abs1(l);
+          lmax = (al > lmax || al != al) ? al : lmax;
+          x[r][J] = l;
+#pragma unroll
+          for (int c = J + 1; c < LU_NB; ++c) x[r][c] = s_fnms(x[r][c], l, prow[c]);
+        }
+    }
+    return;
+  }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   R bv = R(-1);
   int bp = NONE;
@@ -712,10 +744,11 @@ __device__ __forceinline__ void panel_step(T (&x)[RPT][LU_NB], int (&pos)[RPT], 
   }
 }
 
-template <typename T, int RPT, int NW, class F, int... J>
+template <typename T, int RPT, int NW, bool NOPIV = false, class F, int... J>
 __device__ __forceinline__ void panel_steps(std::integer_sequence<int, J...>, T (&x)[RPT][LU_NB], int (&pos)[RPT],
-                                            const bool (&live)[RPT], PanelLDS<T, NW>& sh, int nb, F& on_pivot) {
-  (panel_step<T, RPT, NW, J>(x, pos, live, sh, nb, on_pivot), ...);
+                                            const bool (&live)[RPT], PanelLDS<T, NW>& sh, int nb, F& on_pivot,
+                                            typename real_of<T>::type& lmax) {
+  (panel_step<T, RPT, NW, J, NOPIV>(x, pos, live, sh, nb, on_pivot, lmax), ...);
 }
 
 // Register panels (m <= THREADS RPT): thread t owns panel rows t + THREADS r
@@ -723,7 +756,7 @@ __device__ __forceinline__ void panel_steps(std::integer_sequence<int, J...>, T 
 // position at the end.  No panel storage in LDS.  Columns beyond nb are zero
 // and stay zero (their updates subtract l * 0).  The workgroup then applies
 // the panel's row permutation to the rest of its matrix itself.
-template <typename T, int RPT, int THREADS>
+template <typename T, int RPT, int THREADS, bool NOPIV = false>
 __global__ __launch_bounds__(THREADS) void getrf_panel_rows_kernel(int n, T* A, long lda, long sA, int k, int nb,
                                                                    int* ipiv, int* info, T* dinv, int* swtab) {
   using R = typename real_of<T>::type;
@@ -747,7 +780,30 @@ __global__ __launch_bounds__(THREADS) void getrf_panel_rows_kernel(int n, T* A, 
   auto on_pivot = [&](int J, int pq, R fv, unsigned) {
     if (t == 0) record_pivot(k, J, pq, fv == R(0), ipivb, info, b);
   };
-  panel_steps<T, RPT, THREADS / 64>(std::make_integer_sequence<int, LU_NB>{}, x, pos, live, sh, nb, on_pivot);
+  R lmax = R(0);
+  panel_steps<T, RPT, THREADS / 64, NOPIV>(std::make_integer_sequence<int, LU_NB>{}, x, pos, live, sh, nb, on_pivot,
+                                            lmax);
+  if constexpr (NOPIV) {
+    // stability check of the natural-order elimination: a multiplier above
+    // LU_NOPIV_BOUND (or not finite) marks the matrix with info = -1, and the
+    // caller refactors with partial pivoting
+    __shared__ R wmax[THREADS / 64];
+    R v = lmax;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const R o = __shfl_xor(v, off, 64);
+      v = (o > v || o != o) ? o : v;
+    }
+    if ((t & 63) == 0) wmax[t >> 6] = v;
+    __syncthreads();
+    if (t == 0) {
+      R mx = wmax[0];
+      for (int w = 1; w < THREADS / 64; ++w) mx = (wmax[w] > mx || wmax[w] != wmax[w]) ? wmax[w] : mx;
+      // a zero pivot in the natural order (info > 0) is no verdict on the
+      // matrix either: pivoting decides
+      if (!(mx <= R(LU_NOPIV_BOUND)) || info[b] > 0) info[b] = -1;
+    }
+  }
   // net row permutation of the panel: every row whose position changed
   __shared__ int tab[SWTAB];
   if (t == 0) tab[0] = 0;
@@ -773,6 +829,7 @@ __global__ __launch_bounds__(THREADS) void getrf_panel_rows_kernel(int n, T* A, 
       for (int c = 0; c < LU_NB; ++c) Dblk[pos[r]][c] = c < nb ? x[r][c] : s_zero<T>();
   }
   finish_panel_tab<T>(&Dblk[0][0], LU_NB + 1, nullptr, 0, k, nb, n, b, tab, dinv, swtab);
+  if constexpr (NOPIV) return;  // no row moved
   // the panel's row permutation on every other column of the matrix, CW
   // columns at a time: all moved rows are read before any is written
   constexpr int CW = THREADS >= 128 ? 32 : 16, RG = THREADS / CW, NV = 2 * LU_NB / RG;
@@ -876,7 +933,9 @@ __device__ __forceinline__ void tslu_select(T (&x)[1][LU_NB], int (&pos)[1], int
     if (mask) out[J] = row_id;
     if (info && threadIdx.x == 0 && fv == R(0) && info[b] == 0) info[b] = k + J + 1;
   };
-  panel_steps<T, 1, THREADS / 64>(std::make_integer_sequence<int, LU_NB>{}, x, pos, live, sh, nb, on_pivot);
+  typename real_of<T>::type lmax_unused = 0;
+  panel_steps<T, 1, THREADS / 64>(std::make_integer_sequence<int, LU_NB>{}, x, pos, live, sh, nb, on_pivot,
+                                   lmax_unused);
   __syncthreads();
 }
 
@@ -1245,10 +1304,31 @@ __global__ __launch_bounds__(256) void getrf_panel_nopiv_kernel(int n, T* A, lon
 // columns (the register panels do).
 template <typename T>
 static hipError_t launch_panel(int n, T* A, long lda, long sA, int k, int nb, int* ipiv, int* info, T* dinv,
-                               int* swtab, int* cand, int batch, hipStream_t s, bool* swapped) {
+                               int* swtab, int* cand, int batch, hipStream_t s, bool* swapped,
+                               bool unpivoted = false) {
   const int m = n - k;
   const double pf = flop_mult<T>() * batch * ((double)m * nb * nb - (double)nb * nb * nb / 3.0);
   *swapped = m <= rows_panel_max<T>();
+  if (unpivoted && m <= rows_panel_max<T>()) {
+    // natural-order register panels (checked: info = -1 past LU_NOPIV_BOUND)
+    static const std::string nm_rows = std::string("getrf_panel_rows_nopiv<") + tname<T>() + ">";
+    prof::Scope ps(s, nm_rows.c_str(), pf, 2.0 * sizeof(T) * (double)m * nb * batch);
+    if (m <= 128)
+      hipLaunchKernelGGL((getrf_panel_rows_kernel<T, 1, 128, true>), dim3(batch), dim3(128), 0, s, n, A, lda, sA, k,
+                         nb, ipiv, info, dinv, swtab);
+    else if (m <= 256)
+      hipLaunchKernelGGL((getrf_panel_rows_kernel<T, 1, 256, true>), dim3(batch), dim3(256), 0, s, n, A, lda, sA, k,
+                         nb, ipiv, info, dinv, swtab);
+    else if (m <= 512)
+      hipLaunchKernelGGL((getrf_panel_rows_kernel<T, 1, 512, true>), dim3(batch), dim3(512), 0, s, n, A, lda, sA, k,
+                         nb, ipiv, info, dinv, swtab);
+    else if constexpr (sizeof(T) < 16)
+      hipLaunchKernelGGL((getrf_panel_rows_kernel<T, 1, 1024, true>), dim3(batch), dim3(1024), 0, s, n, A, lda, sA,
+                         k, nb, ipiv, info, dinv, swtab);
+    else
+      return hipErrorInvalidValue;  // not reached: rows_panel_max<c128>() is 512
+    return hipGetLastError();
+  }
   if (m <= rows_panel_max<T>()) {
     static const std::string nm_rows = std::string("getrf_panel_rows<") + tname<T>() + ">";
     prof::Scope ps(s, nm_rows.c_str(), pf, 2.0 * sizeof(T) * (double)m * nb * batch);
@@ -1395,7 +1475,7 @@ static hipError_t trsm_block(bool upper, int n, int ncols, const T* dinv, T* X, 
 
 template <typename T>
 hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, T* dinv, int* scratch, int* info, int batch,
-                    hipStream_t s, bool nopiv) {
+                    hipStream_t s, bool nopiv, bool nopiv_all) {
   if (n <= 0 || batch <= 0) return hipSuccess;
   int* ipiv = scratch;
   int* swtab = scratch + (long)batch * n;
@@ -1455,7 +1535,7 @@ hipError_t getrf_ws(int n, T* A, long lda, long sA, int* perm, T* dinv, int* scr
         swapped = true;  // nothing to swap
         u12_done = true;
       } else {
-        e = launch_panel<T>(n, A, lda, sA, k, nb, ipiv, info, dinv, swtab, cand, batch, s, &swapped);
+        e = launch_panel<T>(n, A, lda, sA, k, nb, ipiv, info, dinv, swtab, cand, batch, s, &swapped, nopiv_all);
       }
       if (e != hipSuccess) return e;
       const int others = n - nb;
@@ -2162,7 +2242,7 @@ size_t dinv_elems(int n) { return (size_t)((n + LU_NB - 1) / LU_NB) * LU_NB * LU
 #endif
 
 #define HPS_INST_LU(T)                                                                                          \
-  template hipError_t getrf_ws<T>(int, T*, long, long, int*, T*, int*, int*, int, hipStream_t, bool);                 \
+  template hipError_t getrf_ws<T>(int, T*, long, long, int*, T*, int*, int*, int, hipStream_t, bool, bool);           \
   template hipError_t getrs_perm<T>(int, int, const T*, long, long, const int*, const T*, const T*, long, long, \
                                     T*, long, long, int, hipStream_t);                                          \
   template hipError_t getrs_inplace<T>(int, int, const T*, long, long, const T*, T*, long, long, int, hipStream_t); \

@@ -59,6 +59,35 @@ def _rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
+def test_leaf_natural_order_rejected_falls_back_to_pivoting(gpu):
+    """The leaf stage factors in natural order with a per-leaf multiplier
+    check; an I coefficient that cancels the first interior diagonal entry of
+    A_ii (a zero or tiny first pivot in natural order, while A_ii stays
+    nonsingular) must send the stage back to partial pivoting, whose result
+    matches the oracle's LAPACK inverses."""
+    import jaxhps_amd as jh
+    from jaxhps_amd.local_solve import local_solve_stage_uniform_2D_DtN
+
+    p, q, L = 8, 6, 1
+    root = jh.DiscretizationNode2D(*cs.BOUNDS)
+    dom = jh.Domain(p=p, q=q, root=root, L=L)
+    pts = dom.interior_points
+    ones = np.ones(pts.shape[:2])
+    half = (dom.root.xmax - dom.root.xmin) / 2 ** (L + 1)
+    ops = osp.leaf_ops_2D(p, q, half)
+    nb = ops["P"].shape[0]
+    I = np.zeros(pts.shape[:2])
+    I[:, nb] = -(ops["D_xx"][nb, nb] + ops["D_yy"][nb, nb])
+    pde = jh.PDEProblem(dom, source=cs.poly_source(pts), D_xx_coefficients=ones, D_yy_coefficients=ones,
+                        I_coefficients=I)
+    print("leaf natural-order fallback", flush=True)
+    ref = _oracle(pde, "dtn")
+    Y, T, v, h = local_solve_stage_uniform_2D_DtN(pde)
+    assert _rel(_np(Y), ref["Y"]) < 1e-9
+    assert _rel(_np(T), ref["T"]) < 1e-9
+    assert _rel(_np(v), ref["v"][..., 0]) < 1e-9
+
+
 @pytest.mark.parametrize("kind", ["dtn", "iti"])
 @pytest.mark.parametrize("p,L", [(6, 1), (8, 2), (16, 1)])
 def test_leaf_stage_matches_oracle(gpu, kind, p, L):

@@ -179,8 +179,9 @@ def test_getrs_conjt_solves_adjoint_system(gpu, dtype, n, nrhs, batch):
     assert res.abs().max().item() < TOL[dtype] * 50 * n**0.5
 
 
-@pytest.mark.parametrize("kind,dtype,m", [("iti", torch.complex64, 1536), ("dtn", torch.float32, 768)])
-def test_block_inverse_solves_match_fp64_merge(gpu, kind, dtype, m):
+@pytest.mark.parametrize("kind,dtype,m,nm", [("iti", torch.complex64, 1536, 1), ("iti", torch.complex64, 1536, 2),
+                                             ("dtn", torch.float32, 768, 1)])
+def test_block_inverse_solves_match_fp64_merge(gpu, kind, dtype, m, nm):
     """A merge level whose factored matrix has 3072 unknowns in low precision
     takes its many-column solve through the inverses of the 512 x 512
     diagonal blocks; S, T, g_tilde and h agree with the same merge in double
@@ -189,18 +190,18 @@ def test_block_inverse_solves_match_fp64_merge(gpu, kind, dtype, m):
     from jaxhps_amd._tables import quad_merge_tables
 
     ne = 4 * m
-    print(f"block-inverse merge {kind} {dtype} m={m}", flush=True)
+    print(f"block-inverse merge {kind} {dtype} m={m} matrices={nm}", flush=True)
     g = torch.Generator(device=gpu).manual_seed(5)
-    X = torch.randn(4, ne, ne, generator=g, device=gpu, dtype=torch.float64) / ne**0.5
+    X = torch.randn(4 * nm, ne, ne, generator=g, device=gpu, dtype=torch.float64) / ne**0.5
     if kind == "iti":
-        Y = torch.randn(4, ne, ne, generator=g, device=gpu, dtype=torch.float64) / ne**0.5
+        Y = torch.randn(4 * nm, ne, ne, generator=g, device=gpu, dtype=torch.float64) / ne**0.5
         T64 = (0.5 * torch.complex(X, Y)).contiguous()
         high = torch.complex128
     else:
         T64 = (2.0 * torch.eye(ne, device=gpu, dtype=torch.float64) + 0.1 * X).contiguous()
         high = torch.float64
     del X
-    h64 = torch.randn(4, ne, 2, generator=g, device=gpu, dtype=torch.float64).to(T64.dtype).contiguous()
+    h64 = torch.randn(4 * nm, ne, 2, generator=g, device=gpu, dtype=torch.float64).to(T64.dtype).contiguous()
     tb = quad_merge_tables(m, kind)
     lo = merge_level(T64.to(dtype).contiguous(), h64.to(dtype).contiguous(), tb)
     hi = merge_level(T64.to(high), h64.to(high), tb)
