@@ -744,7 +744,7 @@ static int merge_t(hps_handle_t h, int kind, int nm, int nc, int nec, int ni, in
     hipLaunchKernelGGL((add_identity_kernel<T>), dim3(ceil_div(nF, 256), nm), dim3(256), 0, s, Fl, nF, sNF2, nm);
     HPS_CHECK_HIP(h, hipGetLastError());
   }
-  HPS_CHECK_HIP(h, getrf_ws<T>(nF, Fl, nF, sNF2, perm, dinv, scratch, info, nm, s, h->merge_nopiv));
+  HPS_CHECK_HIP(h, getrf_ws<T>(nF, Fl, nF, sNF2, perm, dinv, scratch, info, nm, s, h->merge_nopiv, h->merge_nopiv));
 
   // ---- solves: X = -D^{-1} [C | h_int] -------------------------------------
   // one pass for the C columns (width ne) and one for h (width nsrc)

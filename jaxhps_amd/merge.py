@@ -23,12 +23,11 @@ def _levels(T_arr, h_arr, l, kind, nc, table_fn, keep_B=False, root_T=True, dtyp
     dtypes[i], when given, is the precision level i merges in (the children's
     T and h are converted on entry).
 
-    The interface matrices of the top levels (panels taller than the register
-    panels hold, or at most 16 matrices) are factored without row exchanges
-    (pivot=False): these matrices eliminate stably in their natural order,
-    and pivoted panels there are latency-bound.  The factors of every
-    level are checked at the end (one synchronisation); from the first level
-    that fails the check the merges are redone with partial pivoting."""
+    The interface matrices are factored without row exchanges (pivot=False):
+    they eliminate stably in their natural order, and pivoted panels are
+    latency-bound.  Every level is checked at the end (one synchronisation);
+    from the first level that fails the check the merges are redone with
+    partial pivoting."""
     from . import profiling
     nec = T_arr.shape[-1]
     T0 = T_arr.reshape(-1, nec, nec)
@@ -59,17 +58,19 @@ def _levels(T_arr, h_arr, l, kind, nc, table_fn, keep_B=False, root_T=True, dtyp
 
     out = run(0, T0, h0, [], pivot)
     if not pivot:
-        # the levels factored without row exchanges: those with panels taller
-        # than the register panels (1024 rows, 512 for complex128) and those of
-        # at most 16 matrices
-        checked = [i for i, r in enumerate(out)
-                   if r.F_lu.shape[-1] > (512 if r.F_lu.dtype == torch.complex128 else 1024) or r.F_lu.shape[0] <= 16]
-        if checked:
-            ok = torch.stack([merge_factors_stable(out[i].F_lu) for i in checked]).cpu()
-            if not bool(ok.all()):
-                first = checked[int((~ok).nonzero()[0, 0])]
-                T, h = (T0, h0) if first == 0 else (out[first - 1].T, out[first - 1].h)
-                out = run(first, T, h, out[:first], True)
+        # every level was factored without row exchanges: the register-sized
+        # panels check themselves (info = -1 on a rejected matrix); the tall
+        # panels of the levels with panels taller than the register panels
+        # (1024 rows, 512 for complex128) or of at most 16 matrices are
+        # checked here on the factors
+        tall = lambda r: (r.F_lu.shape[-1] > (512 if r.F_lu.dtype == torch.complex128 else 1024)  # noqa: E731
+                          or r.F_lu.shape[0] <= 16)
+        ok = torch.stack([(r.info >= 0).all() & (merge_factors_stable(r.F_lu) if tall(r) else True)
+                          for r in out]).cpu()
+        if not bool(ok.all()):
+            first = int((~ok).nonzero()[0, 0])
+            T, h = (T0, h0) if first == 0 else (out[first - 1].T, out[first - 1].h)
+            out = run(first, T, h, out[:first], True)
     return out, multi
 
 

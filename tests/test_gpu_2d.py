@@ -349,3 +349,32 @@ def test_unpivoted_merge_falls_back_to_pivoting(gpu):
     S_lst, g_lst, Ttop = merge_stage_uniform_2D_DtN(Tg, hg, 1, return_T=True)
     assert _rel(_np(S_lst[0]), S0[0]) < 1e-11
     assert _rel(_np(Ttop), Tt0) < 1e-11
+
+
+def test_unpivoted_register_panel_merge_rejected_falls_back(gpu):
+    """32 DtN merges of 128 interface unknowns (register-sized panels, more
+    than 16 matrices): merge 5's interface matrix has an exactly zero
+    leading entry.  The natural-order panel kernel must flag that matrix
+    alone (info = -1) and the merge stage redo the level with partial
+    pivoting, giving the oracle's S for every merge."""
+    from jaxhps_amd._stages import merge_level
+    from jaxhps_amd._tables import quad_merge_tables
+    from jaxhps_amd.merge import merge_stage_uniform_2D_DtN
+
+    print("unpivoted register-panel merge fallback", flush=True)
+    m = 32
+    ne = 4 * m
+    rng = np.random.default_rng(23)
+    T = 2.0 * np.eye(ne) + rng.standard_normal((128, ne, ne)) / np.sqrt(ne)
+    h = rng.standard_normal((128, ne, 1))
+    bad = 5
+    # D[0, 0] = a55[0, 0] + b55[0, 0] of merge `bad` (children 4 bad, 4 bad + 1)
+    T[4 * bad + 1, ne - 1, ne - 1] = -T[4 * bad, m, m]
+    S0, _, _, _ = hps2d.merge_stage(T, h, 1, "dtn")
+    Tg = torch.as_tensor(T, device=gpu)
+    hg = torch.as_tensor(h, device=gpu)
+    raw = merge_level(Tg, hg, quad_merge_tables(m, "dtn"), pivot=False)
+    info = _np(raw.info)
+    assert info[bad] == -1 and (np.delete(info, bad) == 0).all(), info
+    S_lst, _ = merge_stage_uniform_2D_DtN(Tg, hg, 2)
+    assert _rel(_np(S_lst[0]), S0[0]) < 1e-11
