@@ -125,9 +125,76 @@ hipError_t gemm(const GemmArgs<T>& a, hipStream_t s) {
   return hipGetLastError();
 }
 
+template <typename T>
+hipError_t gemm_grouped(const GemmArgs<T>& a, int n_blk, const int* blk, hipStream_t s) {
+  if (a.N <= 0 || a.batch <= 0 || n_blk <= 0) return hipSuccess;
+  if (a.transA || a.triA) return hipErrorInvalidValue;
+  // one launch when every group takes the 64 x 64 tile path (N > 256, K > 32);
+  // otherwise one product per group
+  bool one = a.N > 256;
+  for (int u = 0; u < n_blk && one; ++u) one = blk[4 * u + 3] > 32;
+  // groups that write the same rows run one after another (each reads the
+  // previous one's D as its C)
+  for (int u = 0; u < n_blk && one; ++u)
+    for (int v = u + 1; v < n_blk && one; ++v)
+      one = blk[4 * u] + blk[4 * u + 1] <= blk[4 * v] || blk[4 * v] + blk[4 * v + 1] <= blk[4 * u];
+  auto group_args = [&](int u) {
+    GemmArgs<T> g = a;
+    const int r0 = blk[4 * u], k0 = blk[4 * u + 2];
+    g.M = blk[4 * u + 1];
+    g.K = blk[4 * u + 3];
+    g.A = a.A + (long)r0 * a.lda + k0;
+    g.B = a.B + (long)k0 * a.ldb;
+    g.C = a.C ? a.C + (long)r0 * a.ldc : nullptr;
+    g.D = a.D + (long)r0 * a.ldd;
+    return g;
+  };
+  if (!one) {
+    for (int u = 0; u < n_blk; ++u) {
+      hipError_t e = gemm<T>(group_args(u), s);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
+  static const std::string nm = std::string("gemm_mfma_64x64<") + tname<T>() + ">";
+  const int tiles_n = ceil_div(a.N, 64);
+  const unsigned gy = (unsigned)(a.batch < 65535 ? a.batch : 65535);
+  for (int u0 = 0; u0 < n_blk; u0 += GEMM_MAX_GROUPS) {
+    GemmGroups gr;
+    gr.n = n_blk - u0 < GEMM_MAX_GROUPS ? n_blk - u0 : GEMM_MAX_GROUPS;
+    double flops = 0, bytes = 0;
+    for (int i = 0; i < gr.n; ++i) {
+      const int u = u0 + i;
+      gr.r0[i] = blk[4 * u];
+      gr.nr[i] = blk[4 * u + 1];
+      gr.k0[i] = blk[4 * u + 2];
+      gr.nk[i] = blk[4 * u + 3];
+      gr.start[i + 1] = gr.start[i] + ceil_div(gr.nr[i], 64) * tiles_n;
+      flops += 2.0 * gr.nr[i] * a.N * (double)gr.nk[i] * a.batch * flop_mult<T>();
+      bytes += sizeof(T) * ((double)gr.nr[i] * gr.nk[i] * (a.sA ? a.batch : 1) +
+                            (double)gr.nk[i] * a.N * (a.sB ? a.batch : 1) +
+                            (double)gr.nr[i] * a.N * a.batch * (a.beta != 0.0 ? 2 : 1));
+    }
+    prof::Scope ps(s, nm.c_str(), flops, bytes);
+    dim3 grid((unsigned)gr.start[gr.n], gy);
+    if constexpr (sizeof(T) == 16)
+      hipLaunchKernelGGL((gemm_mfma_grouped_kernel<T, 2, 2, 2, false>), grid, dim3(256), 0, s, a, gr, tiles_n);
+    else
+      hipLaunchKernelGGL((gemm_mfma_grouped_kernel<T, 2, 2, 3, is_cplx<T>::value>), grid, dim3(256), 0, s, a, gr,
+                         tiles_n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 template hipError_t gemm<double>(const GemmArgs<double>&, hipStream_t);
 template hipError_t gemm<float>(const GemmArgs<float>&, hipStream_t);
 template hipError_t gemm<c128>(const GemmArgs<c128>&, hipStream_t);
 template hipError_t gemm<c64>(const GemmArgs<c64>&, hipStream_t);
+template hipError_t gemm_grouped<double>(const GemmArgs<double>&, int, const int*, hipStream_t);
+template hipError_t gemm_grouped<float>(const GemmArgs<float>&, int, const int*, hipStream_t);
+template hipError_t gemm_grouped<c128>(const GemmArgs<c128>&, int, const int*, hipStream_t);
+template hipError_t gemm_grouped<c64>(const GemmArgs<c64>&, int, const int*, hipStream_t);
 
 }  // namespace hps

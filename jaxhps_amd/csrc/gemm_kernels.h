@@ -66,8 +66,8 @@ template <> __device__ __forceinline__ c64 s_conj<c64>(c64 x) { return c64{x.re,
 // K3M (complex only): three real products per complex one (P1 = Ar Br,
 // P2 = Ai Bi, P3 = (Ar + Ai)(Br + Bi); Re = P1 - P2, Im = P3 - P1 - P2), a
 // quarter fewer MFMAs for one more accumulator per block.
-template <typename T, int TM, int TN, int MINB = 1, int WR = 2, bool TA = false, bool K3M = false>
-__global__ __launch_bounds__(256, MINB) void gemm_mfma_kernel(GemmArgs<T> p, int tiles_n) {
+template <typename T, int TM, int TN, int WR, bool TA, bool K3M>
+__device__ __forceinline__ void gemm_mfma_tile(const GemmArgs<T>& p, int bx, int by, int tiles_n) {
   using R = typename real_of<T>::type;
   using M = Mfma<R>;
   using acc_t = typename M::acc_t;
@@ -86,8 +86,6 @@ __global__ __launch_bounds__(256, MINB) void gemm_mfma_kernel(GemmArgs<T> p, int
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wr = wave / WC, wc = wave % WC;
-  int bx, by;
-  xcd_block(bx, by);
   const int tm = bx / tiles_n, tn = bx % tiles_n;
   const int m0 = tm * BM, n0 = tn * BN;
   const int li = lane & 15, lk = lane >> 4;
@@ -217,6 +215,43 @@ __global__ __launch_bounds__(256, MINB) void gemm_mfma_kernel(GemmArgs<T> p, int
           }
         }
   }
+}
+
+template <typename T, int TM, int TN, int MINB = 1, int WR = 2, bool TA = false, bool K3M = false>
+__global__ __launch_bounds__(256, MINB) void gemm_mfma_kernel(GemmArgs<T> p, int tiles_n) {
+  int bx, by;
+  xcd_block(bx, by);
+  gemm_mfma_tile<T, TM, TN, WR, TA, K3M>(p, bx, by, tiles_n);
+}
+
+// Several products in one launch: group g is the product of rows
+// r0[g] .. r0[g] + nr[g] of A (columns k0[g] .. k0[g] + nk[g]) with rows
+// k0[g] .. of B, into rows r0[g] .. of C and D; every group shares N, the
+// leading dimensions, the batch strides, alpha and beta (the block-sparse
+// coupling and update products of the merges).  Workgroup x-ids
+// start[g] .. start[g + 1] - 1 hold group g's tiles, so the groups fill the
+// GPU together instead of one short launch after another.
+constexpr int GEMM_MAX_GROUPS = 32;
+struct GemmGroups {
+  int n = 0;
+  int start[GEMM_MAX_GROUPS + 1] = {};
+  int r0[GEMM_MAX_GROUPS] = {}, nr[GEMM_MAX_GROUPS] = {}, k0[GEMM_MAX_GROUPS] = {}, nk[GEMM_MAX_GROUPS] = {};
+};
+
+template <typename T, int TM, int TN, int MINB, bool K3M>
+__global__ __launch_bounds__(256, MINB) void gemm_mfma_grouped_kernel(GemmArgs<T> p, GemmGroups g, int tiles_n) {
+  int bx, by;
+  xcd_block(bx, by);
+  int gi = 0;
+  while (gi + 1 < g.n && bx >= g.start[gi + 1]) ++gi;
+  bx -= g.start[gi];
+  p.M = g.nr[gi];
+  p.K = g.nk[gi];
+  p.A += (long)g.r0[gi] * p.lda + g.k0[gi];
+  p.B += (long)g.k0[gi] * p.ldb;
+  if (p.C) p.C += (long)g.r0[gi] * p.ldc;
+  p.D += (long)g.r0[gi] * p.ldd;
+  gemm_mfma_tile<T, TM, TN, 2, false, K3M>(p, bx, by, tiles_n);
 }
 
 // Short-K updates (K <= 32: the 32-wide steps of the blocked LU and solves):

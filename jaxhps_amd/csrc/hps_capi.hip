@@ -667,6 +667,32 @@ static hipError_t gemm_call(hipStream_t s, int M, int N, int K, double alpha, co
   return gemm<T>(g, s);
 }
 
+// The row-block products of a block list (r0, nr, k0, nk) in one grouped
+// launch: D[r0:] = alpha A[r0:, k0:k0+nk] B[k0:k0+nk] + beta C[r0:].
+template <typename T>
+static hipError_t block_gemm(hipStream_t s, int n_blk, const int* blk, int N, double alpha, const T* A, long lda,
+                             long sA, const T* B, long ldb, long sB, double beta, const T* C, long ldc, long sC, T* D,
+                             long ldd, long sD, int batch) {
+  GemmArgs<T> g;
+  g.N = N;
+  g.alpha = alpha;
+  g.beta = beta;
+  g.A = A;
+  g.lda = lda;
+  g.sA = sA;
+  g.B = B;
+  g.ldb = ldb;
+  g.sB = sB;
+  g.C = C;
+  g.ldc = ldc;
+  g.sC = sC;
+  g.D = D;
+  g.ldd = ldd;
+  g.sD = sD;
+  g.batch = batch;
+  return gemm_grouped<T>(g, n_blk, blk, s);
+}
+
 }  // namespace hps
 
 // Products with the block-sparse ItI couplings: D = I + [[0, D12], [D21, 0]]
@@ -679,14 +705,7 @@ static hipError_t coupling_gemm(hipStream_t s, int n_blk, const int* blk, int nF
                                 long ldo, long sO, int nm) {
   if (n_blk <= 0)
     return gemm_call<T>(s, nF, w, nF, alpha, Dxy, nF, sD, X, ldx, sX, C ? 1.0 : 0.0, C, ldc, sC, Out, ldo, sO, nm);
-  for (int u = 0; u < n_blk; ++u) {
-    const int r0 = blk[4 * u], nr = blk[4 * u + 1], k0 = blk[4 * u + 2], nk = blk[4 * u + 3];
-    hipError_t e = gemm_call<T>(s, nr, w, nk, alpha, Dxy + (long)r0 * nF + k0, nF, sD, X + (long)k0 * ldx, ldx, sX,
-                                C ? 1.0 : 0.0, C ? C + (long)r0 * ldc : nullptr, ldc, sC, Out + (long)r0 * ldo, ldo,
-                                sO, nm);
-    if (e != hipSuccess) return e;
-  }
-  return hipSuccess;
+  return block_gemm<T>(s, n_blk, blk, w, alpha, Dxy, nF, sD, X, ldx, sX, C ? 1.0 : 0.0, C, ldc, sC, Out, ldo, sO, nm);
 }
 
 template <typename T>
@@ -811,12 +830,8 @@ static int merge_t(hps_handle_t h, int kind, int nm, int nc, int nec, int ni, in
       HPS_CHECK_HIP(h, gemm_call<T>(s, ne, w, ni, 1.0, Bm, ni, (long)ne * ni, Sp, w, sS, 1.0, outp, w, sO, outp, w, sO,
                                     nm));
     } else {
-      for (int u = 0; u < n_upd; ++u) {
-        const int r0 = upd[4 * u], nr = upd[4 * u + 1], k0 = upd[4 * u + 2], nk = upd[4 * u + 3];
-        HPS_CHECK_HIP(h, gemm_call<T>(s, nr, w, nk, 1.0, Bm + (long)r0 * ni + k0, ni, (long)ne * ni,
-                                      Sp + (long)k0 * w, w, sS, 1.0, outp + (long)r0 * w, w, sO,
-                                      outp + (long)r0 * w, w, sO, nm));
-      }
+      HPS_CHECK_HIP(h, block_gemm<T>(s, n_upd, upd, w, 1.0, Bm, ni, (long)ne * ni, Sp, w, sS, 1.0, outp, w, sO, outp,
+                                     w, sO, nm));
     }
   }
   return HPS_OK;
@@ -929,11 +944,8 @@ static int up_t(hps_handle_t h, int kind, int nm, int nc, int nec, int ni, int n
     if (n_upd <= 0) {
       HPS_CHECK_HIP(h, gemm_call<T>(s, ne, w, ni, 1.0, Bm, ni, (long)ne * ni, out, w, sY, 1.0, ho, w, sO, ho, w, sO, nm));
     } else {
-      for (int u = 0; u < n_upd; ++u) {
-        const int r0 = upd[4 * u], nr = upd[4 * u + 1], k0 = upd[4 * u + 2], nk = upd[4 * u + 3];
-        HPS_CHECK_HIP(h, gemm_call<T>(s, nr, w, nk, 1.0, Bm + (long)r0 * ni + k0, ni, (long)ne * ni, out + (long)k0 * w,
-                                      w, sY, 1.0, ho + (long)r0 * w, w, sO, ho + (long)r0 * w, w, sO, nm));
-      }
+      HPS_CHECK_HIP(h, block_gemm<T>(s, n_upd, upd, w, 1.0, Bm, ni, (long)ne * ni, out, w, sY, 1.0, ho, w, sO, ho, w,
+                                     sO, nm));
     }
   }
   return HPS_OK;

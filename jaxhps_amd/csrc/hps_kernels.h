@@ -31,6 +31,14 @@ struct GemmArgs {
 template <typename T>
 hipError_t gemm(const GemmArgs<T>& a, hipStream_t s);
 
+// The products of row blocks of one A with row blocks of one B: blk holds
+// n_blk host-side rectangles (r0, nr, k0, nk); block u is
+// D[r0 : r0 + nr] = alpha A[r0 : r0 + nr, k0 : k0 + nk] B[k0 : k0 + nk] + beta C[r0 : r0 + nr]
+// with a's N, leading dimensions, batch strides, alpha and beta (a.M and
+// a.K are not read).  The blocks must write disjoint rows of D.
+template <typename T>
+hipError_t gemm_grouped(const GemmArgs<T>& a, int n_blk, const int* blk, hipStream_t s);
+
 // In-place batched LU with partial pivoting of the n x n block of each A[b]
 // (row stride lda, batch stride sA): P A = L U.  perm[b*n + i] is the row of
 // A that became row i of P A.  dinv[b] (dinv_elems(n) values) receives the
