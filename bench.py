@@ -18,8 +18,11 @@ against the FP64 path or the oracle, the FP64 path misses 1e-10 against the
 oracle, or the refinement did not converge.
 
 Default workload: 2D Helmholtz, ItI merges, k = 100, p = 16, L = 7 (16384
-leaves, 4,194,304 DOF).  --config B selects the 2D variable-coefficient DtN
-workload (p = 16, L = 6, 16 RHS).
+leaves, 4,194,304 DOF).  --config A, B or D selects the 2D Poisson DtN
+workload (p = 8, L = 3), the 2D variable-coefficient DtN workload (p = 16,
+L = 6, 16 RHS) or the 3D variable-coefficient DtN workload (p = 10, L = 3,
+4 RHS).  The default line also carries short timed runs of A, B and D
+(other_configs) and the adjoint's gradients per second.
 """
 
 from __future__ import annotations
@@ -58,6 +61,8 @@ def parse():
                     help="refinement stops once the (measured or contraction-estimated) relative update is below "
                          "this; 1e-10 leaves a 100x margin to the 1e-8 bar, and the line reports the measured error")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint grads/s measurement")
+    ap.add_argument("--no-other-configs", action="store_true",
+                    help="skip the short timed runs of configs A, B and D on the default line")
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the surrogate-in-the-gradient-loop measurement of the adjoint line")
     ap.add_argument("--adjoint-L", type=int, default=6, help="tree depth of the adjoint workload (testing only)")
@@ -88,6 +93,59 @@ def make_workload(cfg, L=None):
         dom, kw, g = wl.config_D(L=L or 3)
         desc = dict(workload="3D var-coeff Poisson DtN p=10", L=dom.L, n_leaves=dom.n_leaves, p=10, q=8, nsrc=4)
     return dom, kw, g, desc
+
+
+def other_configs_bench(refine_tol):
+    """Short timed runs of the other workloads for the default line: config A
+    (FP64, the plain method), B and D (mixed with FP64 refinement, and FP64),
+    each at its full size, inputs resident in HBM, one warmup step, then a
+    few timed build+solve steps; the mixed result's error against the FP64
+    one and the refinement's iteration count come with each."""
+    import numpy as np
+    import torch
+
+    import jaxhps_amd as jh
+
+    dev = torch.device("cuda", 0)
+    out = {}
+    for cfg, steps in (("A", 5), ("B", 3), ("D", 2)):
+        dom, kw, g, desc = make_workload(cfg)
+        dkw = {k: (torch.as_tensor(v).to(dev) if isinstance(v, np.ndarray) else v) for k, v in kw.items()}
+        g_dev = torch.as_tensor(g).to(dev)
+        dof = dom.n_leaves * dom.p ** (2 if dom.bool_2D else 3)
+        rec = dict(desc, dof=dof, steps=steps, warmup=1)
+        us = {}
+        for prec in (["fp64"] if cfg == "A" else ["mixed", "fp64"]):
+            pde = jh.PDEProblem(dom, **dkw)
+            bkw = {} if prec == "fp64" else dict(precision="mixed", fp64_top_levels=0)
+
+            def step():
+                jh.build_solver(pde, **bkw)
+                if prec == "mixed":
+                    return jh.solve(pde, g_dev, refine_tol=refine_tol)
+                return jh.solve(pde, g_dev)
+
+            step()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                u = step()
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) / steps * 1e3
+            rec[prec] = {"ms_per_step": ms, "value": dof / (ms / 1e3), "unit": "DOF/s"}
+            if prec == "mixed":
+                rec[prec]["refine_iterations"] = len(pde.refine_history)
+                rec[prec]["converged"] = bool(pde.refine_converged)
+            us[prec] = u
+            del pde
+        if "mixed" in us:
+            rec["speedup_vs_fp64"] = rec["fp64"]["ms_per_step"] / rec["mixed"]["ms_per_step"]
+            rec["rel_err_vs_fp64"] = float(torch.linalg.vector_norm(us["mixed"] - us["fp64"])
+                                           / torch.linalg.vector_norm(us["fp64"]))
+        out[cfg] = rec
+        del us
+        torch.cuda.empty_cache()
+    return out
 
 
 def _oracle_setup_2d(dom, kw):
@@ -934,18 +992,21 @@ def main():
         out["cpu_baseline"] = cpu_baseline(args.config)
         if dom.bool_2D and not args.no_oracle_check:
             out["oracle_check"] = oracle_check(args.config, build_kwargs("mixed"), args.refine_tol)
-    if args.config == "C" and not args.no_adjoint and not args.profile_only:
+    converged = bool(res["mixed"]["converged"]) if "mixed" in res else True
+    if args.config == "C" and not args.profile_only and not (args.no_adjoint and args.no_other_configs):
         for prec in res.values():
             prec.pop("pde", None)
         res.clear()
         torch.cuda.empty_cache()
+    if args.config == "C" and not args.no_other_configs and not args.profile_only:
+        out["other_configs"] = other_configs_bench(args.refine_tol)
+        torch.cuda.empty_cache()
+    if args.config == "C" and not args.no_adjoint and not args.profile_only:
         out["adjoint"] = adjoint_bench(args.adjoint_L)
         if not args.no_surrogate:
             torch.cuda.empty_cache()
             out["adjoint"]["surrogate_loop"] = surrogate_loop_bench(args.adjoint_L)
-    ok = True
-    if "mixed" in res:
-        ok = bool(res["mixed"]["converged"]) and out.get("rel_err_vs_fp64", 0.0) <= 1e-8
+    ok = converged and out.get("rel_err_vs_fp64", 0.0) <= 1e-8
     oc = out.get("oracle_check")
     if oc:
         ok = ok and oc["fp64_vs_oracle"] <= 1e-10 and oc["fp64_vs_exact"] <= 1e-10
