@@ -137,17 +137,17 @@ int hps_leaf_solve(hps_handle_t h, int dtype, int kind, int nl, int n, int nb, i
  *   merge/_schur_complement.py:7-237) and _uniform_oct_merge_DtN
  *   (merge/_uniform_3D_DtN.py:128-187, merge/_schur_complement.py:294-774). */
 /* Pivoting of hps_merge_level's factorisation of D (DtN) or W (ItI) on this
- * handle: 1 (default) = partial pivoting; 0 = the panels taller than 1024
- * rows (512 for complex128), and every panel of a level of at most 16
- * matrices, are factored without row exchanges; the other panels keep
- * partial pivoting.  The merge matrices of the uniform
- * tree eliminate stably in their natural order (pivot growth ~1, multipliers
- * below 4 on the Helmholtz and variable-coefficient workloads), while the
- * tall pivoted (tournament) panels of the top levels are latency-bound; a
- * caller that turns pivoting off checks the factors (largest multiplier,
- * finite, nonzero diagonal) and refactors with pivoting when they fail; info
- * reports only the pivoted panels' zero columns.  The reference factors with
- * LAPACK's pivoted LU inside jnp.linalg.inv
+ * handle: 1 (default) = partial pivoting; 0 = every panel is factored
+ * without row exchanges.  The merge matrices of the uniform tree eliminate
+ * stably in their natural order (pivot growth ~1, multipliers below 4 on the
+ * Helmholtz and variable-coefficient workloads), while pivoted panels are
+ * latency-bound.  With 0, the register panels (up to 1024 rows, 512 for
+ * complex128, in levels of more than 16 matrices) check each matrix's
+ * largest multiplier and set info = -1 when it exceeds 64 or is not finite;
+ * the other panels are not checked, and a caller that turns pivoting off
+ * checks those factors (largest multiplier, finite, nonzero diagonal) and
+ * refactors with pivoting when either check fails.  The reference factors
+ * with LAPACK's pivoted LU inside jnp.linalg.inv
  * (merge/_schur_complement.py:107,146). */
 int hps_set_merge_pivoting(hps_handle_t h, int partial_pivoting);
 /* Pivoting of hps_leaf_solve's factorisation on this handle: 1 (default) =
