@@ -1713,10 +1713,14 @@ __device__ __forceinline__ void diag_block_apply(const T (*Di)[LU_NB + 1], T* xs
   __syncthreads();
 }
 
-template <typename T, bool CT>
-__global__ __launch_bounds__(256) void getrs_small_kernel(int n, int nrhs, const T* LU, long lda, long sA,
+// ONE: a single right-hand side (nrhs == 1 at compile time: the refinement's
+// leaf solves), which needs fewer registers, so more workgroups per CU
+// stream their factors at once.
+template <typename T, bool CT, bool ONE = false>
+__global__ __launch_bounds__(256) void getrs_small_kernel(int n, int nrhs_arg, const T* LU, long lda, long sA,
                                                           const int* perm, const T* dinv, const T* R, long ldr,
                                                           long sR, T* X, long ldx, long sX) {
+  const int nrhs = ONE ? 1 : nrhs_arg;
   extern __shared__ char smem_raw[];
   T* xs = (T*)smem_raw;  // (n, nrhs)
   __shared__ T Di[LU_NB][LU_NB + 1];
@@ -1919,7 +1923,8 @@ static hipError_t getrs_small(int n, int nrhs, const T* LU, long lda, long sA, c
                  sizeof(T) * (double)batch * ((double)n * n + 2.0 * n * nrhs));
   for (int b0 = 0; b0 < batch; b0 += 65535) {
     const int bc = batch - b0 < 65535 ? batch - b0 : 65535;
-    hipLaunchKernelGGL((getrs_small_kernel<T, CT>), dim3(bc), dim3(256), (size_t)n * nrhs * sizeof(T), s, n, nrhs,
+    auto kern = nrhs == 1 ? getrs_small_kernel<T, CT, true> : getrs_small_kernel<T, CT, false>;
+    hipLaunchKernelGGL(kern, dim3(bc), dim3(256), (size_t)n * nrhs * sizeof(T), s, n, nrhs,
                        LU + (long)b0 * sA, lda, sA, perm + (long)b0 * n,
                        dinv + (long)b0 * ((n + LU_NB - 1) / LU_NB) * LU_NB * LU_NB, R + (long)b0 * sR, ldr, sR,
                        X + (long)b0 * sX, ldx, sX);

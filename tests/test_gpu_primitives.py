@@ -103,10 +103,12 @@ def test_getrs_many_rhs(gpu, dtype, n, batch):
 
 
 @pytest.mark.parametrize("dtype", DTYPES)
-@pytest.mark.parametrize("n,batch,nrhs", [(896, 4, 1), (520, 3, 16), (1100, 1, 2), (300, 127, 1), (300, 128, 3)])
+@pytest.mark.parametrize("n,batch,nrhs", [(896, 4, 1), (520, 3, 16), (1100, 1, 2), (300, 127, 1), (300, 128, 3),
+                                          (256, 5, 1), (300, 130, 1)])
 def test_getrs_few_rhs(gpu, dtype, n, batch, nrhs):
     """Few right-hand sides: the blocked solve with the per-block kernel for
-    small batches, the one-workgroup-per-matrix solve from 128 matrices."""
+    small batches, the one-workgroup-per-matrix solve from 128 matrices (or
+    n <= 256), with its single-column form."""
     from jaxhps_amd import _lib
 
     print(f"getrs few rhs dtype={dtype} n={n} batch={batch} nrhs={nrhs}", flush=True)
