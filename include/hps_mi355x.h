@@ -173,14 +173,31 @@ int hps_merge_level(hps_handle_t h, int dtype, int kind, int nm, int nc, int nec
  *   h_int = gather(h_in, int_rows) + inj,  gt_out = -D^{-1} h_int (final
  *   order),  h_out = h_ext + B gt_out (h_out may be NULL at the root).
  * inj (nm, ni, nsrc), internal equation order, or NULL: extra interface data
- * (the interface residual of iterative refinement).
+ * (the interface residual of iterative refinement).  F_inv: the block
+ * inverses hps_block_inverses formed from F_lu, or NULL; with them (and
+ * nsrc <= 16) the solve with D (W) runs through them.
  * Replaces up_pass_uniform_2D_ItI / _DtN with assemble_boundary_data
  * (up_pass/_uniform_2D_ItI.py:8-219, up_pass/_uniform_2D_DtN.py:8-173). */
 size_t hps_up_workspace(int dtype, int kind, int nm, int ni, int nsrc);
 int hps_up_level(hps_handle_t h, int dtype, int kind, int nm, int nc, int nec, int ni, int ne, int nsrc,
                  const void* h_in, const void* inj, const int* int_rows, const int* ext, const int* final_of_internal,
-                 int n_upd, const int* upd, const void* F_lu, const int* perm, const void* F_dinv, const void* D12,
-                 const void* D21, const void* B, void* gt_out, void* h_out, void* ws, size_t ws_bytes);
+                 int n_upd, const int* upd, const void* F_lu, const int* perm, const void* F_dinv,
+                 const void* F_inv, const void* D12, const void* D21, const void* B, void* gt_out, void* h_out,
+                 void* ws, size_t ws_bytes);
+
+/* Inverses of the 256 x 256 diagonal blocks of L (unit lower) and U of the
+ * nm factored n x n merge matrices F_lu / F_dinv of hps_merge_level, for
+ * hps_up_level's few-column solves: each 256-row block step becomes one
+ * product with a block inverse instead of eight dependent 32-row steps.
+ * inv holds nm * hps_block_inverses_elems(n) elements (0 when n is not
+ * eligible: n > 256, a multiple of 32).  No counterpart in the reference,
+ * whose up pass applies the explicit inverse D^{-1} it stores
+ * (up_pass/_uniform_2D_ItI.py:180-219); the rounding of the inverted blocks
+ * is that of the low-precision factors, corrected by the refinement. */
+size_t hps_block_inverses_elems(int n);
+size_t hps_block_inverses_workspace(int dtype, int n, int nm);
+int hps_block_inverses(hps_handle_t h, int dtype, int n, int nm, const void* F_lu, const void* F_dinv, void* inv,
+                       void* ws, size_t ws_bytes);
 
 /* ---- down pass (one tree level) -------------------------------------------
  * g_int = S g_in + gt; children's boundary data g_out (nm*nc, nec, nsrc) from

@@ -58,8 +58,12 @@ SIGNATURES = {
     "hps_up_workspace": (_S, [_I, _I, _I, _I, _I]),
     "hps_up_level": (
         _I,
-        [_P, _I, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _S],
+        [_P, _I, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P,
+         _S],
     ),
+    "hps_block_inverses_elems": (_S, [_I]),
+    "hps_block_inverses_workspace": (_S, [_I, _I, _I]),
+    "hps_block_inverses": (_I, [_P, _I, _I, _I, _P, _P, _P, _P, _S]),
     "hps_down_workspace": (_S, [_I, _I, _I, _I]),
     "hps_down_level": (_I, [_P, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _S]),
     "hps_static_scratch_bytes": (_S, []),

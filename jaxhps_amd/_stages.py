@@ -216,6 +216,7 @@ class MergeLevel:
     D21: Optional[torch.Tensor]
     B: Optional[torch.Tensor]
     tables: MergeTables
+    Finv: Optional[torch.Tensor] = None  # block inverses for few-column up-pass solves (form_block_inverses)
 
 
 def merge_level(T_in, h_in, tables: MergeTables, keep_factors=False, want_T=True, pivot=True) -> MergeLevel:
@@ -261,6 +262,27 @@ def merge_level(T_in, h_in, tables: MergeTables, keep_factors=False, want_T=True
             lib.hps_set_merge_pivoting(hd, 1)
     check(rc, hd, "hps_merge_level")
     return MergeLevel(S, T, ho, gt, F_lu, perm, dinv, info, D12, D21, B, tables)
+
+
+def form_block_inverses(lvl: MergeLevel) -> None:
+    """Store in lvl.Finv the inverses of the 256 x 256 diagonal blocks of the
+    level's low-precision factors when its up-pass solves would otherwise take
+    the blocked few-column path (fewer than 128 matrices of more than 256
+    unknowns): the refinement's up passes then solve through them."""
+    nm, n = lvl.F_lu.shape[0], lvl.F_lu.shape[-1]
+    if lvl.F_lu.dtype not in (torch.complex64, torch.float32) or nm >= 128:
+        return
+    lib = load_library()
+    elems = lib.hps_block_inverses_elems(n)
+    if elems == 0:
+        return
+    hd = _lib.handle()
+    dt = lvl.F_lu.dtype
+    inv = torch.empty(nm * elems, dtype=dt, device=lvl.F_lu.device)
+    ws = workspace(lib.hps_block_inverses_workspace(dtype_code(dt), n, nm), lvl.F_lu.device)
+    rc = lib.hps_block_inverses(hd, dtype_code(dt), n, nm, ptr(lvl.F_lu), ptr(lvl.dinv), ptr(inv), ptr(ws), ws.numel())
+    check(rc, hd, "hps_block_inverses")
+    lvl.Finv = inv
 
 
 def merge_factors_stable(F_lu, bound=64.0):
@@ -393,7 +415,8 @@ def up_level(h_in, lvl: MergeLevel, inj=None, want_h=True):
         hd, dtype_code(dt), kind, nm, tb.nc, tb.nec, tb.ni, tb.ne, nsrc, ptr(h_in.contiguous()),
         ptr(inj.contiguous()) if inj is not None else None, ptr(rows), ptr(ext), ptr(final_of_internal),
         upd.shape[0], upd.ctypes.data_as(ctypes.c_void_p), ptr(lvl.F_lu), ptr(lvl.perm), ptr(lvl.dinv),
-        ptr(lvl.D12), ptr(lvl.D21), ptr(lvl.B), ptr(gt), ptr(ho), ptr(ws), ws.numel(),
+        ptr(lvl.Finv) if lvl.Finv is not None else None, ptr(lvl.D12), ptr(lvl.D21), ptr(lvl.B), ptr(gt), ptr(ho),
+        ptr(ws), ws.numel(),
     )
     check(rc, hd, "hps_up_level")
     return gt, ho

@@ -888,14 +888,21 @@ int hps_merge_level(hps_handle_t h, int dtype, int kind, int nm, int nc, int nec
 size_t hps_up_workspace(int dtype, int kind, int nm, int ni, int nsrc) {
   size_t es = dtype_size(dtype);
   const int nF = kind == HPS_KIND_ITI ? ni / 2 : ni;
-  return align_up((size_t)nm * ni * nsrc * es) + align_up((size_t)nm * nF * nsrc * es);
+  // h_int, Z (ItI), and the scratch of the block-inverse solve
+  return align_up((size_t)nm * ni * nsrc * es) + 2 * align_up((size_t)nm * nF * nsrc * es);
+}
+
+// the up pass solves through the block inverses when they were formed and the
+// solve has few columns
+static bool up_uses_fewinv(const void* F_inv, int nfac, int nsrc) {
+  return F_inv != nullptr && nsrc <= 16 && fewinv_eligible(nfac);
 }
 
 template <typename T>
 static int up_t(hps_handle_t h, int kind, int nm, int nc, int nec, int ni, int ne, int nsrc, const void* h_in,
                 const void* inj, const int* int_rows, const int* ext, const int* final_of_internal, int n_upd,
-                const int* upd, const void* F_lu, const int* perm, const void* F_dinv, const void* D12v,
-                const void* D21v, const void* Bv, void* gt_out, void* h_out, void* ws) {
+                const int* upd, const void* F_lu, const int* perm, const void* F_dinv, const void* F_inv,
+                const void* D12v, const void* D21v, const void* Bv, void* gt_out, void* h_out, void* ws) {
   hipStream_t s = h->stream;
   const int4* rows = (const int4*)int_rows;
   const int4* ex = (const int4*)ext;
@@ -907,8 +914,12 @@ static int up_t(hps_handle_t h, int kind, int nm, int nc, int nec, int ni, int n
   T* Yh = (T*)wp;
   wp += align_up((size_t)nm * ni * w * es);
   T* Zh = (T*)wp;
+  wp += align_up((size_t)nm * nF * w * es);
+  T* Ysc = (T*)wp;
   const T* Fl = (const T*)F_lu;
   const T* dinv = (const T*)F_dinv;
+  const T* finv = (const T*)F_inv;
+  const bool fewinv = up_uses_fewinv(F_inv, nF, w);
   const long sNF2 = (long)nF * nF;
   const long sY = (long)ni * w;
   {
@@ -922,12 +933,19 @@ static int up_t(hps_handle_t h, int kind, int nm, int nc, int nec, int ni, int n
   if (inj) HPS_CHECK_HIP(h, add2d<T>(ni, w, -1.0, (const T*)inj, w, sY, Yh, w, sY, nm, s));
   T* out = (T*)gt_out;
   if (!iti) {
-    HPS_CHECK_HIP(h, getrs_perm<T>(ni, w, Fl, ni, (long)ni * ni, perm, dinv, Yh, w, sY, out, w, sY, nm, s));
+    if (fewinv)
+      HPS_CHECK_HIP(h, getrs_perm_fewinv<T>(ni, w, Fl, ni, (long)ni * ni, perm, finv, Yh, w, sY, out, w, sY, Ysc, nm,
+                                            s));
+    else
+      HPS_CHECK_HIP(h, getrs_perm<T>(ni, w, Fl, ni, (long)ni * ni, perm, dinv, Yh, w, sY, out, w, sY, nm, s));
   } else {
     const long sZ = (long)nF * w;
     HPS_CHECK_HIP(h, gemm_call<T>(s, nF, w, nF, -1.0, (const T*)D12v, nF, sNF2, Yh + (long)nF * w, w, sY, 1.0, Yh, w,
                                   sY, Zh, w, sZ, nm));
-    HPS_CHECK_HIP(h, getrs_perm<T>(nF, w, Fl, nF, sNF2, perm, dinv, Zh, w, sZ, Yh, w, sY, nm, s));
+    if (fewinv)
+      HPS_CHECK_HIP(h, getrs_perm_fewinv<T>(nF, w, Fl, nF, sNF2, perm, finv, Zh, w, sZ, Yh, w, sY, Ysc, nm, s));
+    else
+      HPS_CHECK_HIP(h, getrs_perm<T>(nF, w, Fl, nF, sNF2, perm, dinv, Zh, w, sZ, Yh, w, sY, nm, s));
     HPS_CHECK_HIP(h, gemm_call<T>(s, nF, w, nF, -1.0, (const T*)D21v, nF, sNF2, Yh, w, sY, 1.0, Yh + (long)nF * w, w,
                                   sY, Yh + (long)nF * w, w, sY, nm));
     hipLaunchKernelGGL((row_scatter_kernel<T>), grid_for((long)ni * w, nm), dim3(256), 0, s, Yh, out,
@@ -953,8 +971,9 @@ static int up_t(hps_handle_t h, int kind, int nm, int nc, int nec, int ni, int n
 
 int hps_up_level(hps_handle_t h, int dtype, int kind, int nm, int nc, int nec, int ni, int ne, int nsrc,
                  const void* h_in, const void* inj, const int* int_rows, const int* ext, const int* final_of_internal,
-                 int n_upd, const int* upd, const void* F_lu, const int* perm, const void* F_dinv, const void* D12,
-                 const void* D21, const void* B, void* gt_out, void* h_out, void* ws, size_t ws_bytes) {
+                 int n_upd, const int* upd, const void* F_lu, const int* perm, const void* F_dinv,
+                 const void* F_inv, const void* D12, const void* D21, const void* B, void* gt_out, void* h_out,
+                 void* ws, size_t ws_bytes) {
   if (!h) return HPS_ERR_ARG;
   HPS_ARG(h, kind == HPS_KIND_DTN || kind == HPS_KIND_ITI, "kind must be DtN or ItI");
   HPS_ARG(h, nm > 0 && nc > 0 && nec > 0 && ni > 0 && ne > 0 && nsrc > 0, "bad up-pass sizes");
@@ -967,7 +986,36 @@ int hps_up_level(hps_handle_t h, int dtype, int kind, int nm, int nc, int nec, i
     return HPS_ERR_WORKSPACE;
   }
   DISPATCH(dtype, up_t, h, kind, nm, nc, nec, ni, ne, nsrc, h_in, inj, int_rows, ext, final_of_internal, n_upd, upd,
-           F_lu, perm, F_dinv, D12, D21, B, gt_out, h_out, ws);
+           F_lu, perm, F_dinv, F_inv, D12, D21, B, gt_out, h_out, ws);
+}
+
+// ---------------------------------------------------------------------------
+// block inverses for the few-column up-pass solves
+// ---------------------------------------------------------------------------
+size_t hps_block_inverses_elems(int n) { return fewinv_elems(n); }
+
+size_t hps_block_inverses_workspace(int dtype, int n, int nm) {
+  return align_up(fewinv_workspace_elems(n, nm) * dtype_size(dtype));
+}
+
+template <typename T>
+static int fewinv_t(hps_handle_t h, int n, int nm, const void* F_lu, const void* F_dinv, void* inv, void* ws) {
+  HPS_CHECK_HIP(h, form_fewinv<T>(n, (const T*)F_lu, n, (long)n * n, (const T*)F_dinv, (T*)inv, (T*)ws, nm,
+                                  h->stream));
+  return HPS_OK;
+}
+
+int hps_block_inverses(hps_handle_t h, int dtype, int n, int nm, const void* F_lu, const void* F_dinv, void* inv,
+                       void* ws, size_t ws_bytes) {
+  if (!h) return HPS_ERR_ARG;
+  HPS_ARG(h, fewinv_eligible(n), "block inverses need n > 256, a multiple of 32");
+  HPS_ARG(h, nm > 0 && (long)nm * fewinv_blocks(n) <= 65535, "bad matrix count");
+  HPS_ARG(h, F_lu && F_dinv && inv && ws, "null pointer");
+  if (ws_bytes < hps_block_inverses_workspace(dtype, n, nm)) {
+    h->err = "block-inverse workspace too small";
+    return HPS_ERR_WORKSPACE;
+  }
+  DISPATCH(dtype, fewinv_t, h, n, nm, F_lu, F_dinv, inv, ws);
 }
 
 // ---------------------------------------------------------------------------

@@ -85,6 +85,30 @@ hipError_t getrs_perm_blockinv(int n, int nrhs, const T* LU, long lda, long sA, 
                                const T* R, long ldr, long sR, T* X, long ldx, long sX, int batch, T* wsp,
                                hipStream_t s);
 
+// Few-column solves (at most 16 columns) of matrices too large for one
+// workgroup, in fewer than 128 matrices: the refinement's up passes through
+// the top merge levels' low-precision factors.  form_fewinv writes the
+// inverses of the FEWINV_NB x FEWINV_NB diagonal blocks of L (unit lower) and
+// U of every matrix: inv holds batch * fewinv_elems(n) elements (all the L
+// block inverses, then all the U ones), ws fewinv_workspace_elems(n, batch)
+// elements.  getrs_perm_fewinv solves X = A^{-1} R with them (Y: n x nrhs
+// scratch per matrix, row stride nrhs); R, X and Y must not alias.  Eligible:
+// n > 256 and a multiple of 32.
+constexpr int FEWINV_NB = 256;
+inline bool fewinv_eligible(int n) { return n > 256 && n % 32 == 0; }
+inline int fewinv_blocks(int n) { return (n + FEWINV_NB - 1) / FEWINV_NB; }
+inline size_t fewinv_elems(int n) {
+  return fewinv_eligible(n) ? 2 * (size_t)fewinv_blocks(n) * FEWINV_NB * FEWINV_NB : 0;
+}
+inline size_t fewinv_workspace_elems(int n, int batch) {
+  return fewinv_eligible(n) ? (size_t)batch * fewinv_blocks(n) * (FEWINV_NB * FEWINV_NB + FEWINV_NB * 32) : 0;
+}
+template <typename T>
+hipError_t form_fewinv(int n, const T* LU, long lda, long sA, const T* dinv, T* inv, T* ws, int batch, hipStream_t s);
+template <typename T>
+hipError_t getrs_perm_fewinv(int n, int nrhs, const T* LU, long lda, long sA, const int* perm, const T* inv,
+                             const T* R, long ldr, long sR, T* X, long ldx, long sX, T* Y, int batch, hipStream_t s);
+
 // X = A^{-H} R from getrf_ws's factors (A^H = U^H L^H P).  W is n x nrhs
 // scratch per batch element (row stride nrhs); X, W and R must not alias.
 template <typename T>

@@ -2047,6 +2047,65 @@ __global__ void eye_kernel(T* X, int n, long sX, int batch) {
   }
 }
 
+// Linv_j = inv(L_jj) (unit lower) and Uinv_j = inv(U_jj) of `count` NB x NB
+// diagonal blocks at once: block j's factor entries at Lb + j sDiag (row
+// stride lda), its packed 32 x 32 diagonal inverses at db + j sDinv; the
+// inverses go to Linv / Uinv + j NB^2 (row stride NB).  Each 32-row step
+// applies the packed inverse (a triangular product) and updates the rows
+// below (L) or above (U) with K = 32.
+template <typename T>
+static hipError_t form_diag_inverses(int NB, int count, const T* Lb, long lda, long sDiag, const T* db, long sDinv,
+                                     T* Linv, T* Uinv, hipStream_t s) {
+  const long sBlk = (long)NB * NB;
+  auto gemm = [&](int M, int N, int K, double alpha, double beta, const T* A, long la, long sa, const T* B, long lb,
+                  long sb, const T* C, long lc, long sc, T* D, long ld, long sd, int bt, int tri) -> hipError_t {
+    if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
+    GemmArgs<T> g;
+    g.M = M;
+    g.N = N;
+    g.K = K;
+    g.alpha = alpha;
+    g.beta = beta;
+    g.A = A;
+    g.lda = la;
+    g.sA = sa;
+    g.B = B;
+    g.ldb = lb;
+    g.sB = sb;
+    g.C = C;
+    g.ldc = lc;
+    g.sC = sc;
+    g.D = D;
+    g.ldd = ld;
+    g.sD = sd;
+    g.batch = bt;
+    g.triA = tri;
+    return hps::gemm<T>(g, s);
+  };
+  static const std::string nm_inv = std::string("blockinv_form<") + tname<T>() + ">";
+  {
+    prof::Scope ps(s, nm_inv.c_str(), 0.0, 2.0 * sizeof(T) * (double)count * sBlk);
+    hipLaunchKernelGGL((eye_kernel<T>), dim3(1024), dim3(256), 0, s, Linv, NB, sBlk, count);
+    hipLaunchKernelGGL((eye_kernel<T>), dim3(1024), dim3(256), 0, s, Uinv, NB, sBlk, count);
+  }
+  HPS_LU_CHECK(hipGetLastError());
+  for (int k = 0; k < NB; k += LU_NB) {
+    // rows k .. k + 32 of Linv: inv(L_kk) times them, then the rows below
+    HPS_LU_CHECK(gemm(LU_NB, NB, LU_NB, 1.0, 0.0, db + (long)(k / LU_NB) * LU_NB * LU_NB, LU_NB, sDinv,
+                      Linv + (long)k * NB, NB, sBlk, nullptr, 0, 0, Linv + (long)k * NB, NB, sBlk, count, 1));
+    HPS_LU_CHECK(gemm(NB - k - LU_NB, NB, LU_NB, -1.0, 1.0, Lb + (long)(k + LU_NB) * lda + k, lda, sDiag,
+                      Linv + (long)k * NB, NB, sBlk, Linv + (long)(k + LU_NB) * NB, NB, sBlk,
+                      Linv + (long)(k + LU_NB) * NB, NB, sBlk, count, 0));
+  }
+  for (int k = NB - LU_NB; k >= 0; k -= LU_NB) {
+    HPS_LU_CHECK(gemm(LU_NB, NB, LU_NB, 1.0, 0.0, db + (long)(k / LU_NB) * LU_NB * LU_NB, LU_NB, sDinv,
+                      Uinv + (long)k * NB, NB, sBlk, nullptr, 0, 0, Uinv + (long)k * NB, NB, sBlk, count, 2));
+    HPS_LU_CHECK(gemm(k, NB, LU_NB, -1.0, 1.0, Lb + k, lda, sDiag, Uinv + (long)k * NB, NB, sBlk, Uinv, NB, sBlk,
+                      Uinv, NB, sBlk, count, 0));
+  }
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t getrs_perm_blockinv(int n, int nrhs, const T* LU, long lda, long sA, const int* perm, const T* dinv,
                                const T* R, long ldr, long sR, T* X, long ldx, long sX, int batch, T* wsp,
@@ -2093,34 +2152,13 @@ hipError_t getrs_perm_blockinv(int n, int nrhs, const T* LU, long lda, long sA, 
     g.triA = tri;
     return hps::gemm<T>(g, s);
   };
-  static const std::string nm_inv = std::string("blockinv_form<") + tname<T>() + ">";
   for (int b = 0; b < batch; ++b) {
     const T* Lb = LU + (long)b * sA;
     const T* db = dinv + (long)b * dstride;
     T* Xb = X + (long)b * sX;
-    const long sDiag = (long)NB * (lda + 1);  // between consecutive diagonal blocks of the factors
-    const long sDinv = (long)(NB / LU_NB) * LU_NB * LU_NB;
     // block inverses: Linv_j = inv(L_jj), Uinv_j = inv(U_jj), all j at once
-    {
-      prof::Scope ps(s, nm_inv.c_str(), 0.0, 2.0 * sizeof(T) * (double)nblk * sBlk);
-      hipLaunchKernelGGL((eye_kernel<T>), dim3(1024), dim3(256), 0, s, Linv, NB, sBlk, nblk);
-      hipLaunchKernelGGL((eye_kernel<T>), dim3(1024), dim3(256), 0, s, Uinv, NB, sBlk, nblk);
-    }
-    HPS_LU_CHECK(hipGetLastError());
-    for (int k = 0; k < NB; k += LU_NB) {
-      // X_k = inv(L_kk) X_k, then the rows below within the block
-      HPS_LU_CHECK(gemm(LU_NB, NB, LU_NB, 1.0, 0.0, db + (long)(k / LU_NB) * LU_NB * LU_NB, LU_NB, sDinv,
-                        Linv + (long)k * NB, NB, sBlk, nullptr, 0, 0, Linv + (long)k * NB, NB, sBlk, nblk, 1));
-      HPS_LU_CHECK(gemm(NB - k - LU_NB, NB, LU_NB, -1.0, 1.0, Lb + (long)(k + LU_NB) * lda + k, lda, sDiag,
-                        Linv + (long)k * NB, NB, sBlk, Linv + (long)(k + LU_NB) * NB, NB, sBlk,
-                        Linv + (long)(k + LU_NB) * NB, NB, sBlk, nblk, 0));
-    }
-    for (int k = NB - LU_NB; k >= 0; k -= LU_NB) {
-      HPS_LU_CHECK(gemm(LU_NB, NB, LU_NB, 1.0, 0.0, db + (long)(k / LU_NB) * LU_NB * LU_NB, LU_NB, sDinv,
-                        Uinv + (long)k * NB, NB, sBlk, nullptr, 0, 0, Uinv + (long)k * NB, NB, sBlk, nblk, 2));
-      HPS_LU_CHECK(gemm(k, NB, LU_NB, -1.0, 1.0, Lb + k, lda, sDiag, Uinv + (long)k * NB, NB, sBlk, Uinv, NB, sBlk,
-                        Uinv, NB, sBlk, nblk, 0));
-    }
+    HPS_LU_CHECK(form_diag_inverses<T>(NB, nblk, Lb, lda, (long)NB * (lda + 1), db,
+                                       (long)(NB / LU_NB) * LU_NB * LU_NB, Linv, Uinv, s));
     // forward: X_j = Linv_j X_j, then the rows below
     for (int j = 0; j < nblk; ++j) {
       const int k0 = j * NB;
@@ -2140,6 +2178,127 @@ hipError_t getrs_perm_blockinv(int n, int nrhs, const T* LU, long lda, long sA, 
       HPS_LU_CHECK(gemm(k0, nrhs, NB, -1.0, 1.0, Lb + k0, lda, 0, Xb + (long)k0 * ldx, ldx, 0, Xb, ldx, 0, Xb, ldx,
                         0, 1, 0));
     }
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Few-column solves of the top merge levels' low-precision factors (the
+// refinement's up passes) through explicit inverses of the FEWINV_NB x
+// FEWINV_NB diagonal blocks of L and U, formed once per build: each block
+// step is one product with the block inverse instead of eight dependent
+// 32-row steps in one workgroup.
+// ---------------------------------------------------------------------------
+
+// The diagonal blocks of every matrix, copied contiguously (block bb = b nblk
+// + j): G[bb] = the NB x NB factor entries (zero past n), Gd[bb] = its packed
+// 32 x 32 inverses (identity past n), so all blocks form their inverses in one
+// batched pass and the last, partial block acts as [[A_r, 0], [0, I]].
+template <typename T>
+__global__ void diag_blocks_gather_kernel(int n, int nblk, const T* LU, long lda, long sA, const T* dinv, T* G,
+                                          T* Gd) {
+  constexpr int NB = FEWINV_NB;
+  const int bb = blockIdx.y;
+  const int b = bb / nblk, j = bb - b * nblk;
+  const int k0 = j * NB;
+  const T* A = LU + (long)b * sA;
+  const long dstride = (long)((n + LU_NB - 1) / LU_NB) * LU_NB * LU_NB;
+  T* g = G + (long)bb * NB * NB;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < NB * NB; idx += gridDim.x * blockDim.x) {
+    const int r = idx / NB, c = idx - r * NB;
+    const int gr = k0 + r, gc = k0 + c;
+    g[idx] = gr < n && gc < n ? A[(long)gr * lda + gc] : s_zero<T>();
+  }
+  constexpr int ND = (NB / LU_NB) * LU_NB * LU_NB;
+  T* gd = Gd + (long)bb * ND;
+  for (int idx = blockIdx.x * blockDim.x + threadIdx.x; idx < ND; idx += gridDim.x * blockDim.x) {
+    const int sb = idx / (LU_NB * LU_NB), e = idx - sb * LU_NB * LU_NB;
+    const int r = e / LU_NB, c = e - r * LU_NB;
+    gd[idx] = k0 + sb * LU_NB < n ? dinv[(long)b * dstride + (long)(k0 / LU_NB + sb) * LU_NB * LU_NB + e]
+                                  : (r == c ? s_one<T>() : s_zero<T>());
+  }
+}
+
+template <typename T>
+hipError_t form_fewinv(int n, const T* LU, long lda, long sA, const T* dinv, T* inv, T* ws, int batch,
+                       hipStream_t s) {
+  if (!fewinv_eligible(n) || batch <= 0) return hipErrorInvalidValue;
+  constexpr int NB = FEWINV_NB;
+  const int nblk = fewinv_blocks(n);
+  const int count = batch * nblk;
+  if (count > 65535) return hipErrorInvalidValue;
+  T* G = ws;
+  T* Gd = ws + (long)count * NB * NB;
+  {
+    static const std::string nm = std::string("blockinv_form<") + tname<T>() + ">";
+    prof::Scope ps(s, nm.c_str(), 0.0, 2.0 * sizeof(T) * (double)count * NB * NB);
+    hipLaunchKernelGGL((diag_blocks_gather_kernel<T>), dim3(64, count), dim3(256), 0, s, n, nblk, LU, lda, sA, dinv,
+                       G, Gd);
+    HPS_LU_CHECK(hipGetLastError());
+  }
+  return form_diag_inverses<T>(NB, count, G, NB, (long)NB * NB, Gd, (long)(NB / LU_NB) * LU_NB * LU_NB, inv,
+                               inv + (long)count * NB * NB, s);
+}
+
+template <typename T>
+hipError_t getrs_perm_fewinv(int n, int nrhs, const T* LU, long lda, long sA, const int* perm, const T* inv,
+                             const T* R, long ldr, long sR, T* X, long ldx, long sX, T* Y, int batch,
+                             hipStream_t s) {
+  if (n <= 0 || nrhs <= 0 || batch <= 0) return hipSuccess;
+  if (!fewinv_eligible(n)) return hipErrorInvalidValue;
+  constexpr int NB = FEWINV_NB;
+  const int nblk = fewinv_blocks(n);
+  const long sI = (long)nblk * NB * NB;  // one matrix's block inverses of L (or U)
+  const T* Li = inv;
+  const T* Ui = inv + (long)batch * sI;
+  const long ldy = nrhs, sY = (long)n * nrhs;
+  for (int b0 = 0; b0 < batch; b0 += 65535) {
+    static const std::string nm = std::string("getrs_gather<") + tname<T>() + ">";
+    prof::Scope ps(s, nm.c_str(), 0.0, 2.0 * sizeof(T) * (double)n * nrhs * batch);
+    int bc = batch - b0 < 65535 ? batch - b0 : 65535;
+    dim3 grid(ceil_div(nrhs, 64), n, bc);
+    hipLaunchKernelGGL((gather_rows_kernel<T>), grid, dim3(64), 0, s, n, nrhs, perm + (long)b0 * n,
+                       R + (long)b0 * sR, ldr, sR, X + (long)b0 * sX, ldx, sX);
+  }
+  auto gemm = [&](int M, int K, double beta, const T* A, long la, long sa, const T* B, long lb, long sb, T* D, long ld,
+                  long sd) -> hipError_t {
+    if (M <= 0 || K <= 0) return hipSuccess;
+    GemmArgs<T> g;
+    g.M = M;
+    g.N = nrhs;
+    g.K = K;
+    g.alpha = beta != 0.0 ? -1.0 : 1.0;
+    g.beta = beta;
+    g.A = A;
+    g.lda = la;
+    g.sA = sa;
+    g.B = B;
+    g.ldb = lb;
+    g.sB = sb;
+    g.C = beta != 0.0 ? D : nullptr;
+    g.ldc = ld;
+    g.sC = sd;
+    g.D = D;
+    g.ldd = ld;
+    g.sD = sd;
+    g.batch = batch;
+    return hps::gemm<T>(g, s);
+  };
+  // forward (L): Y_j = inv(L_jj) X_j, then the rows below: X_>j -= L[>j, j] Y_j
+  // (every update spans all remaining rows, so its products fill the GPU)
+  for (int j = 0; j < nblk; ++j) {
+    const int k0 = j * NB, nb = n - k0 < NB ? n - k0 : NB, kend = k0 + nb;
+    HPS_LU_CHECK(gemm(nb, nb, 0.0, Li + (long)j * NB * NB, NB, sI, X + (long)k0 * ldx, ldx, sX, Y + (long)k0 * ldy,
+                      ldy, sY));
+    HPS_LU_CHECK(gemm(n - kend, nb, 1.0, LU + (long)kend * lda + k0, lda, sA, Y + (long)k0 * ldy, ldy, sY,
+                      X + (long)kend * ldx, ldx, sX));
+  }
+  // backward (U): X_j = inv(U_jj) Y_j, then the rows above: Y_<j -= U[<j, j] X_j
+  for (int j = nblk - 1; j >= 0; --j) {
+    const int k0 = j * NB, nb = n - k0 < NB ? n - k0 : NB;
+    HPS_LU_CHECK(gemm(nb, nb, 0.0, Ui + (long)j * NB * NB, NB, sI, Y + (long)k0 * ldy, ldy, sY, X + (long)k0 * ldx,
+                      ldx, sX));
+    HPS_LU_CHECK(gemm(k0, nb, 1.0, LU + k0, lda, sA, X + (long)k0 * ldx, ldx, sX, Y, ldy, sY));
   }
   return hipGetLastError();
 }
@@ -2254,7 +2413,10 @@ size_t dinv_elems(int n) { return (size_t)((n + LU_NB - 1) / LU_NB) * LU_NB * LU
   template hipError_t getrs_perm_blockinv<T>(int, int, const T*, long, long, const int*, const T*, const T*, long,     \
                                              long, T*, long, long, int, T*, hipStream_t);                             \
   template hipError_t getrs_conjt<T>(int, int, const T*, long, long, const int*, const T*, const T*, long, long, \
-                                     T*, long, long, T*, int, hipStream_t);
+                                     T*, long, long, T*, int, hipStream_t);                                     \
+  template hipError_t form_fewinv<T>(int, const T*, long, long, const T*, T*, T*, int, hipStream_t);            \
+  template hipError_t getrs_perm_fewinv<T>(int, int, const T*, long, long, const int*, const T*, const T*, long,  \
+                                           long, T*, long, long, T*, int, hipStream_t);
 // HPS_LU_TYPES selects the element types this object instantiates (bit 0
 // double, 1 float, 2 c128, 3 c64; default all): the build compiles one
 // object per type in parallel.  The type-independent functions live in the

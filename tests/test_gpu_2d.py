@@ -378,3 +378,39 @@ def test_unpivoted_register_panel_merge_rejected_falls_back(gpu):
     assert info[bad] == -1 and (np.delete(info, bad) == 0).all(), info
     S_lst, _ = merge_stage_uniform_2D_DtN(Tg, hg, 2)
     assert _rel(_np(S_lst[0]), S0[0]) < 1e-11
+
+
+@pytest.mark.parametrize("kind,m,nsrc", [("dtn", 80, 1), ("dtn", 80, 4), ("iti", 72, 1), ("iti", 72, 16)])
+def test_up_pass_through_block_inverses_matches_blocked_solve(gpu, kind, m, nsrc):
+    """A low-precision merge level of 3 matrices of 320 (DtN) / 288 (ItI)
+    unknowns -- one full 256-block and a partial one -- gets block inverses;
+    the up pass through them gives the blocked triangular solve's g_tilde and
+    h to single precision, and both match the complex128 / float64 up pass."""
+    from jaxhps_amd._stages import form_block_inverses, merge_level, up_level
+    from jaxhps_amd._tables import quad_merge_tables
+
+    print(f"up pass block inverses kind={kind} m={m} nsrc={nsrc}", flush=True)
+    rng = np.random.default_rng(31)
+    nm, nec = 3, 4 * m
+    T = 2.0 * np.eye(nec) + rng.standard_normal((4 * nm, nec, nec)) / np.sqrt(nec)
+    h = rng.standard_normal((4 * nm, nec, nsrc))
+    if kind == "iti":
+        T = T + 1j * rng.standard_normal(T.shape) / np.sqrt(nec)
+        h = h + 1j * rng.standard_normal(h.shape)
+    lo, hi = (torch.float32, torch.float64) if kind == "dtn" else (torch.complex64, torch.complex128)
+    tb = quad_merge_tables(m, kind)
+    res = {}
+    for dt in (lo, hi):
+        Tg = torch.as_tensor(T, device=gpu).to(dt).contiguous()
+        hg = torch.as_tensor(h, device=gpu).to(dt).contiguous()
+        lvl = merge_level(Tg, None, tb, keep_factors=True)
+        form_block_inverses(lvl)
+        assert (lvl.Finv is not None) == (dt == lo)
+        res[dt, True] = up_level(hg, lvl)
+        lvl.Finv = None
+        res[dt, False] = up_level(hg, lvl)
+    for i in range(2):  # g_tilde, h_out
+        ref = _np(res[hi, False][i])
+        assert _rel(_np(res[hi, True][i]), ref) < 1e-14
+        assert _rel(_np(res[lo, True][i]), _np(res[lo, False][i])) < 1e-4
+        assert _rel(_np(res[lo, True][i]), ref) < 1e-4
