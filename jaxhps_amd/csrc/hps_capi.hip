@@ -557,6 +557,12 @@ int hps_leaf_solve(hps_handle_t h, int dtype, int kind, int nl, int n, int nb, i
 //   D12, D21 nF x nF ItI blocks (when the caller does not keep them)
 //   B   ne x ni      (when the caller does not keep it)
 //   getrf scratch
+// the merge's few-column solve (its sources' g_tilde) goes through block
+// inverses of the low-precision factors, formed in the workspace
+static bool merge_few_inverses(int nfac, int nm, int nsrc) {
+  return nsrc > 0 && nsrc <= 16 && nm < 128 && fewinv_eligible(nfac);
+}
+
 size_t hps_merge_workspace(int dtype, int kind, int nm, int ni, int ne, int nsrc) {
   size_t es = dtype_size(dtype);
   const int nF = kind == HPS_KIND_ITI ? ni / 2 : ni;
@@ -572,6 +578,9 @@ size_t hps_merge_workspace(int dtype, int kind, int nm, int ni, int ne, int nsrc
   tot += align_up(getrf_scratch_bytes(nfac, nm));
   if ((dtype == HPS_F32 || dtype == HPS_C64) && nm <= BLOCKINV_MAX_BATCH)
     tot += align_up(blockinv_workspace_elems(nfac, ne) * es);
+  if ((dtype == HPS_F32 || dtype == HPS_C64) && merge_few_inverses(nfac, nm, nsrc))
+    tot += align_up(nm * fewinv_elems(nfac) * es) + align_up(fewinv_workspace_elems(nfac, nm) * es) +
+           align_up((size_t)nm * nfac * nsrc * es);
   return tot;
 }
 
@@ -743,6 +752,10 @@ static int merge_t(hps_handle_t h, int kind, int nm, int nc, int nec, int ni, in
   T* binv = lowp && nm <= BLOCKINV_MAX_BATCH && blockinv_workspace_elems(nfac, ne)
                 ? (T*)take(blockinv_workspace_elems(nfac, ne) * es)
                 : nullptr;
+  const bool few = lowp && merge_few_inverses(nfac, nm, nsrc);
+  T* finv = few ? (T*)take(nm * fewinv_elems(nfac) * es) : nullptr;
+  T* fws = few ? (T*)take(fewinv_workspace_elems(nfac, nm) * es) : nullptr;
+  T* fsc = few ? (T*)take((size_t)nm * nfac * nsrc * es) : nullptr;
   int* scratch = (int*)wp;
   T* Fl = (T*)F_lu;
   T* dinv = (T*)F_dinv;
@@ -764,6 +777,7 @@ static int merge_t(hps_handle_t h, int kind, int nm, int nc, int nec, int ni, in
     HPS_CHECK_HIP(h, hipGetLastError());
   }
   HPS_CHECK_HIP(h, getrf_ws<T>(nF, Fl, nF, sNF2, perm, dinv, scratch, info, nm, s, h->merge_nopiv, h->merge_nopiv));
+  if (few) HPS_CHECK_HIP(h, form_fewinv<T>(nF, Fl, nF, sNF2, dinv, finv, fws, nm, s));
 
   // ---- solves: X = -D^{-1} [C | h_int] -------------------------------------
   // one pass for the C columns (width ne) and one for h (width nsrc)
@@ -784,16 +798,23 @@ static int merge_t(hps_handle_t h, int kind, int nm, int nc, int nec, int ni, in
     }
     const long sY = (long)ni * w;
     if (!iti) {
-      HPS_CHECK_HIP(h, getrs_perm_blockinv<T>(ni, w, Fl, ni, (long)ni * ni, perm, dinv, Yp, w, sY, out, w, sY, nm,
-                                              pass == 0 ? binv : nullptr, s));
+      if (pass == 1 && few)
+        HPS_CHECK_HIP(h, getrs_perm_fewinv<T>(ni, w, Fl, ni, (long)ni * ni, perm, finv, Yp, w, sY, out, w, sY, fsc,
+                                              nm, s));
+      else
+        HPS_CHECK_HIP(h, getrs_perm_blockinv<T>(ni, w, Fl, ni, (long)ni * ni, perm, dinv, Yp, w, sY, out, w, sY, nm,
+                                                pass == 0 ? binv : nullptr, s));
     } else {
       const long sZ = (long)nF * w;
       // Z = Y_F - D12 Y_G
       HPS_CHECK_HIP(h, coupling_gemm<T>(s, n_d12, d12_blk, nF, w, -1.0, D12, sNF2, Yp + (long)nF * w, w, sY, Yp, w, sY,
                                         Zp, w, sZ, nm));
       // X_F = W^{-1} Z  (into Y_F)
-      HPS_CHECK_HIP(h, getrs_perm_blockinv<T>(nF, w, Fl, nF, sNF2, perm, dinv, Zp, w, sZ, Yp, w, sY, nm,
-                                              pass == 0 ? binv : nullptr, s));
+      if (pass == 1 && few)
+        HPS_CHECK_HIP(h, getrs_perm_fewinv<T>(nF, w, Fl, nF, sNF2, perm, finv, Zp, w, sZ, Yp, w, sY, fsc, nm, s));
+      else
+        HPS_CHECK_HIP(h, getrs_perm_blockinv<T>(nF, w, Fl, nF, sNF2, perm, dinv, Zp, w, sZ, Yp, w, sY, nm,
+                                                pass == 0 ? binv : nullptr, s));
       // X_G = Y_G - D21 X_F  (in place)
       HPS_CHECK_HIP(h, coupling_gemm<T>(s, n_d21, d21_blk, nF, w, -1.0, D21, sNF2, Yp, w, sY, Yp + (long)nF * w, w,
                                         sY, Yp + (long)nF * w, w, sY, nm));

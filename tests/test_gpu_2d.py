@@ -385,7 +385,9 @@ def test_up_pass_through_block_inverses_matches_blocked_solve(gpu, kind, m, nsrc
     """A low-precision merge level of 3 matrices of 320 (DtN) / 288 (ItI)
     unknowns -- one full 256-block and a partial one -- gets block inverses;
     the up pass through them gives the blocked triangular solve's g_tilde and
-    h to single precision, and both match the complex128 / float64 up pass."""
+    h to single precision, and both match the complex128 / float64 up pass.
+    The merge's own source solve (through inverses it forms in its
+    workspace) matches the double-precision merge too."""
     from jaxhps_amd._stages import form_block_inverses, merge_level, up_level
     from jaxhps_amd._tables import quad_merge_tables
 
@@ -403,13 +405,16 @@ def test_up_pass_through_block_inverses_matches_blocked_solve(gpu, kind, m, nsrc
     for dt in (lo, hi):
         Tg = torch.as_tensor(T, device=gpu).to(dt).contiguous()
         hg = torch.as_tensor(h, device=gpu).to(dt).contiguous()
-        lvl = merge_level(Tg, None, tb, keep_factors=True)
+        lvl = merge_level(Tg, hg, tb, keep_factors=True)
+        res[dt, "merge"] = (lvl.gt, lvl.h)
         form_block_inverses(lvl)
         assert (lvl.Finv is not None) == (dt == lo)
         res[dt, True] = up_level(hg, lvl)
         lvl.Finv = None
         res[dt, False] = up_level(hg, lvl)
     for i in range(2):  # g_tilde, h_out
+        ref = _np(res[hi, "merge"][i])
+        assert _rel(_np(res[lo, "merge"][i]), ref) < 1e-4
         ref = _np(res[hi, False][i])
         assert _rel(_np(res[hi, True][i]), ref) < 1e-14
         assert _rel(_np(res[lo, True][i]), _np(res[lo, False][i])) < 1e-4
