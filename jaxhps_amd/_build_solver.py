@@ -33,8 +33,13 @@ def _compute_dtypes(pde_problem, precision, low_leaf=None):
     factorisation loses the k^2 (1 - q) term next to the O((p^2/h)^2)
     differentiation entries badly enough that refinement stops contracting;
     scaled, it is ~1e4 and the complex64 leaf maps are accurate to ~5e-6,
-    which refinement corrects in two or three steps.  low_leaf=False keeps
-    the leaves in FP64 / complex128."""
+    which refinement corrects in two or three steps.  A 2D DtN leaf factors
+    only its interior block A_ii (196 unknowns at p = 16), which costs little
+    in FP64, while FP32 leaves leave the first solution accurate to only
+    ~2e-3 and cost two more refinement steps (config B, 16 right-hand sides:
+    34.3 ms with FP32 leaves, 29.4 ms with FP64 leaves); low_leaf=None
+    therefore takes FP64 leaves for 2D DtN problems and low-precision leaves
+    otherwise.  low_leaf=False / True forces FP64 / low-precision leaves."""
     from .local_solve import _compute_dtype
     from ._refine import LOW
 
@@ -45,7 +50,7 @@ def _compute_dtypes(pde_problem, precision, low_leaf=None):
         return hi, hi, hi, hi
     lo = LOW[hi]
     if low_leaf is None:
-        low_leaf = True
+        low_leaf = pde_problem.use_ItI or not pde_problem.domain.bool_2D
     return (lo if low_leaf else hi), lo, hi, lo
 
 
@@ -87,8 +92,9 @@ def build_solver(pde_problem, return_top_T: bool = False, compute_device=None, h
 
     precision="mixed" (uniform 2D and 3D trees) runs all but the top
     ``fp64_top_levels`` merge levels in FP32 (complex64 for ItI) and stores
-    Y, v in that precision; the leaves are factored in FP32 / complex64 unless
-    ``low_precision_leaves=False`` keeps them in FP64.  solve() then
+    Y, v in that precision; the leaves are factored in FP32 / complex64
+    (ItI and 3D; 2D DtN leaves stay in FP64, which saves refinement steps)
+    unless ``low_precision_leaves`` says otherwise.  solve() then
     refines its solution to FP64 accuracy.  The default keeps no merge level
     in FP64: the refined error is set by the FP64 residual, and FP32 merges
     leave the first solution accurate enough (about 1e-6 at p = 16, L = 7,

@@ -4,6 +4,7 @@ in test_gpu_2d) and against the oracle directly on small trees."""
 
 import numpy as np
 import pytest
+import torch
 
 import cases2d as cs
 from oracle import global_system as gs
@@ -54,12 +55,13 @@ def test_mixed_iti_helmholtz_refines_to_fp64(gpu, L, k, top):
     assert _rel2(_np(u), u64) < 1e-9, (hist, _rel2(_np(u), u64))
 
 
-@pytest.mark.parametrize("nsrc", [1, 3])
-def test_mixed_dtn_variable_coefficient_refines_to_fp64(gpu, nsrc):
+@pytest.mark.parametrize("nsrc,low_leaves", [(1, None), (3, None), (3, True)])
+def test_mixed_dtn_variable_coefficient_refines_to_fp64(gpu, nsrc, low_leaves):
+    """2D DtN mixed build (FP64 leaves by default, FP32 leaves on request)."""
     import jaxhps_amd as jh
     from jaxhps_amd import workloads as wl
 
-    print(f"mixed dtn nsrc={nsrc}", flush=True)
+    print(f"mixed dtn nsrc={nsrc} low_leaves={low_leaves}", flush=True)
     dom, pde64, pdemx, g = _pair(wl.config_B, L=3, p=16, nsrc=nsrc, seed=2)
     rng = np.random.default_rng(4)
     f = rng.standard_normal(pde64.source.shape)
@@ -67,7 +69,9 @@ def test_mixed_dtn_variable_coefficient_refines_to_fp64(gpu, nsrc):
     pdemx.source = f
     jh.build_solver(pde64)
     u64 = _np(jh.solve(pde64, g))
-    jh.build_solver(pdemx, precision="mixed")
+    jh.build_solver(pdemx, precision="mixed", low_precision_leaves=low_leaves)
+    leaf_dt = pdemx._factors["leaf"].lu.dtype
+    assert leaf_dt == (torch.float32 if low_leaves else torch.float64), leaf_dt
     u = jh.solve(pdemx, g, refine_tol=1e-12)
     assert str(u.dtype) == "torch.float64"
     hist = pdemx.refine_history
