@@ -126,6 +126,11 @@ int hps_leaf_solve(hps_handle_t h, int dtype, int kind, int nl, int n, int nb, i
  *   gt_out (nm, ni, nsrc) = -D^{-1} h_int,  T_out (nm, ne, ne) = A + B S,
  *   h_out (nm, ne, nsrc) = h_ext + B gt,  perm/F_dinv/info of F,
  *   B_out (nm, ne, ni) optional copy of B.  T_out/h_out may be NULL.
+ *   F_inv_out (nm * hps_block_inverses_elems(n), n = ni (DtN) or ni/2 (ItI))
+ *   or NULL: for complex64 / float32 levels of fewer than 128 matrices with
+ *   hps_block_inverses_elems(n) > 0, the inverses of F's diagonal blocks (as
+ *   hps_block_inverses forms them), which the level's few-column source
+ *   solve uses too; otherwise not written.
  * upd (host int[4*n_upd]): the nonzero blocks (r0, nr, k0, nk) of B; the
  *   products B S and B gt use only these (n_upd = 0: B dense).
  * d12_blk / d21_blk (host int[4*n], ItI only): the nonzero rectangles of D12
@@ -165,7 +170,7 @@ int hps_merge_level(hps_handle_t h, int dtype, int kind, int nm, int nc, int nec
                     const int* cols_final, const int* final_of_internal, int n_upd, const int* upd, int n_d12,
                     const int* d12_blk, int n_d21, const int* d21_blk, void* S_out, void* T_out, void* h_out,
                     void* gt_out, void* F_lu, int* perm, void* F_dinv, int* info, void* D12_out, void* D21_out,
-                    void* B_out, void* ws, size_t ws_bytes);
+                    void* B_out, void* F_inv_out, void* ws, size_t ws_bytes);
 
 /* ---- up pass (one tree level) --------------------------------------------
  * New outgoing data through the factors a merge level kept (hps_merge_level

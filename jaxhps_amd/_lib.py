@@ -53,7 +53,7 @@ SIGNATURES = {
     "hps_merge_level": (
         _I,
         [_P, _I, _I, _I, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P, _P, _P, _I, _P, _I, _P, _I, _P, _P, _P, _P, _P,
-         _P, _P, _P, _P, _P, _P, _P, _P, _S],
+         _P, _P, _P, _P, _P, _P, _P, _P, _P, _S],
     ),
     "hps_up_workspace": (_S, [_I, _I, _I, _I, _I]),
     "hps_up_level": (

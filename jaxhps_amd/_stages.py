@@ -216,7 +216,7 @@ class MergeLevel:
     D21: Optional[torch.Tensor]
     B: Optional[torch.Tensor]
     tables: MergeTables
-    Finv: Optional[torch.Tensor] = None  # block inverses for few-column up-pass solves (form_block_inverses)
+    Finv: Optional[torch.Tensor] = None  # block inverses for few-column up-pass solves (hps_block_inverses)
 
 
 def merge_level(T_in, h_in, tables: MergeTables, keep_factors=False, want_T=True, pivot=True) -> MergeLevel:
@@ -247,6 +247,9 @@ def merge_level(T_in, h_in, tables: MergeTables, keep_factors=False, want_T=True
     D12 = torch.empty((nm, nf, nf), dtype=dt, device=dev) if keep12 else None
     D21 = torch.empty((nm, nf, nf), dtype=dt, device=dev) if keep12 else None
     B = torch.empty((nm, ne, ni), dtype=dt, device=dev) if keep_factors else None
+    # the block inverses of low-precision levels for the refinement's up passes
+    n_inv = lib.hps_block_inverses_elems(nf) if (dt in (torch.complex64, torch.float32) and nm < 128) else 0
+    Finv = torch.empty((nm * n_inv,), dtype=dt, device=dev) if n_inv else None
     ws = workspace(lib.hps_merge_workspace(dtype_code(dt), kind, nm, ni, ne, nsrc), dev)
     upd = np.ascontiguousarray(tables.upd, dtype=np.int32)
     d12 = np.ascontiguousarray(tables.d12_blk, dtype=np.int32)
@@ -256,19 +259,21 @@ def merge_level(T_in, h_in, tables: MergeTables, keep_factors=False, want_T=True
     try:
         rc = _merge_level_call(lib, hd, dt, kind, nm, nc, nec, ni, ne, nsrc, T_in, h_in, rows, cols, ext, cols_final,
                                final_of_internal, upd, d12, d21, S, T, ho, gt, F_lu, perm, dinv, info, D12, D21, B,
-                               ws)
+                               Finv, ws)
     finally:
         if not pivot:
             lib.hps_set_merge_pivoting(hd, 1)
     check(rc, hd, "hps_merge_level")
-    return MergeLevel(S, T, ho, gt, F_lu, perm, dinv, info, D12, D21, B, tables)
+    return MergeLevel(S, T, ho, gt, F_lu, perm, dinv, info, D12, D21, B, tables, Finv)
 
 
 def form_block_inverses(lvl: MergeLevel) -> None:
     """Store in lvl.Finv the inverses of the 256 x 256 diagonal blocks of the
     level's low-precision factors when its up-pass solves would otherwise take
     the blocked few-column path (fewer than 128 matrices of more than 256
-    unknowns): the refinement's up passes then solve through them."""
+    unknowns): the refinement's up passes then solve through them.
+    merge_level keeps them already; this forms them for factors from
+    elsewhere."""
     nm, n = lvl.F_lu.shape[0], lvl.F_lu.shape[-1]
     if lvl.F_lu.dtype not in (torch.complex64, torch.float32) or nm >= 128:
         return
@@ -298,14 +303,14 @@ def merge_factors_stable(F_lu, bound=64.0):
 
 
 def _merge_level_call(lib, hd, dt, kind, nm, nc, nec, ni, ne, nsrc, T_in, h_in, rows, cols, ext, cols_final,
-                      final_of_internal, upd, d12, d21, S, T, ho, gt, F_lu, perm, dinv, info, D12, D21, B, ws):
+                      final_of_internal, upd, d12, d21, S, T, ho, gt, F_lu, perm, dinv, info, D12, D21, B, Finv, ws):
     return lib.hps_merge_level(
         hd, dtype_code(dt), kind, nm, nc, nec, ni, ne, nsrc,
         ptr(T_in), ptr(h_in), ptr(rows), ptr(cols), ptr(ext), ptr(cols_final), ptr(final_of_internal),
         upd.shape[0], upd.ctypes.data_as(ctypes.c_void_p),
         d12.shape[0], d12.ctypes.data_as(ctypes.c_void_p), d21.shape[0], d21.ctypes.data_as(ctypes.c_void_p),
         ptr(S), ptr(T), ptr(ho), ptr(gt), ptr(F_lu), ptr(perm), ptr(dinv), ptr(info),
-        ptr(D12), ptr(D21), ptr(B), ptr(ws), ws.numel(),
+        ptr(D12), ptr(D21), ptr(B), ptr(Finv), ptr(ws), ws.numel(),
     )
 
 

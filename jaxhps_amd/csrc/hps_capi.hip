@@ -557,10 +557,13 @@ int hps_leaf_solve(hps_handle_t h, int dtype, int kind, int nl, int n, int nb, i
 //   D12, D21 nF x nF ItI blocks (when the caller does not keep them)
 //   B   ne x ni      (when the caller does not keep it)
 //   getrf scratch
-// the merge's few-column solve (its sources' g_tilde) goes through block
-// inverses of the low-precision factors, formed in the workspace
-static bool merge_few_inverses(int nfac, int nm, int nsrc) {
-  return nsrc > 0 && nsrc <= 16 && nm < 128 && fewinv_eligible(nfac);
+// Low-precision merge levels of fewer than 128 matrices of more than 256
+// unknowns get inverses of their factors' diagonal blocks: the merge's own
+// few-column source solve (1 .. 16 columns) goes through them, and a caller
+// that passes F_inv_out keeps them for its up passes.  The workspace holds
+// them (and the forming and solve scratch) whenever the level qualifies.
+static bool merge_block_inverses(int dtype, int nfac, int nm) {
+  return (dtype == HPS_F32 || dtype == HPS_C64) && nm < 128 && fewinv_eligible(nfac);
 }
 
 size_t hps_merge_workspace(int dtype, int kind, int nm, int ni, int ne, int nsrc) {
@@ -578,9 +581,9 @@ size_t hps_merge_workspace(int dtype, int kind, int nm, int ni, int ne, int nsrc
   tot += align_up(getrf_scratch_bytes(nfac, nm));
   if ((dtype == HPS_F32 || dtype == HPS_C64) && nm <= BLOCKINV_MAX_BATCH)
     tot += align_up(blockinv_workspace_elems(nfac, ne) * es);
-  if ((dtype == HPS_F32 || dtype == HPS_C64) && merge_few_inverses(nfac, nm, nsrc))
+  if (merge_block_inverses(dtype, nfac, nm))
     tot += align_up(nm * fewinv_elems(nfac) * es) + align_up(fewinv_workspace_elems(nfac, nm) * es) +
-           align_up((size_t)nm * nfac * nsrc * es);
+           align_up((size_t)nm * nfac * ns * es);
   return tot;
 }
 
@@ -723,7 +726,7 @@ static int merge_t(hps_handle_t h, int kind, int nm, int nc, int nec, int ni, in
                    const int* cols_final, const int* final_of_internal, int n_upd, const int* upd, int n_d12,
                    const int* d12_blk, int n_d21, const int* d21_blk, void* S_out, void* T_out, void* h_out,
                    void* gt_out, void* F_lu, int* perm, void* F_dinv, int* info, void* D12_out, void* D21_out,
-                   void* B_out, void* ws) {
+                   void* B_out, void* F_inv_out, void* ws) {
   hipStream_t s = h->stream;
   const T* Tin = (const T*)T_in;
   const int4* rows = (const int4*)int_rows;
@@ -752,10 +755,15 @@ static int merge_t(hps_handle_t h, int kind, int nm, int nc, int nec, int ni, in
   T* binv = lowp && nm <= BLOCKINV_MAX_BATCH && blockinv_workspace_elems(nfac, ne)
                 ? (T*)take(blockinv_workspace_elems(nfac, ne) * es)
                 : nullptr;
-  const bool few = lowp && merge_few_inverses(nfac, nm, nsrc);
-  T* finv = few ? (T*)take(nm * fewinv_elems(nfac) * es) : nullptr;
-  T* fws = few ? (T*)take(fewinv_workspace_elems(nfac, nm) * es) : nullptr;
-  T* fsc = few ? (T*)take((size_t)nm * nfac * nsrc * es) : nullptr;
+  // block inverses (see merge_block_inverses): into F_inv_out when the caller
+  // keeps them, else into the workspace when the source solve uses them
+  const bool inv_ok = lowp && nm < 128 && fewinv_eligible(nfac);  // merge_block_inverses for this type
+  T* finv_ws = inv_ok ? (T*)take(nm * fewinv_elems(nfac) * es) : nullptr;
+  T* fws = inv_ok ? (T*)take(fewinv_workspace_elems(nfac, nm) * es) : nullptr;
+  T* fsc = inv_ok ? (T*)take((size_t)nm * nfac * ns * es) : nullptr;
+  const bool few = inv_ok && nsrc >= 1 && nsrc <= 16;  // the source solve goes through them
+  const bool form_inv = inv_ok && (few || F_inv_out);
+  T* finv = F_inv_out && inv_ok ? (T*)F_inv_out : finv_ws;
   int* scratch = (int*)wp;
   T* Fl = (T*)F_lu;
   T* dinv = (T*)F_dinv;
@@ -777,7 +785,7 @@ static int merge_t(hps_handle_t h, int kind, int nm, int nc, int nec, int ni, in
     HPS_CHECK_HIP(h, hipGetLastError());
   }
   HPS_CHECK_HIP(h, getrf_ws<T>(nF, Fl, nF, sNF2, perm, dinv, scratch, info, nm, s, h->merge_nopiv, h->merge_nopiv));
-  if (few) HPS_CHECK_HIP(h, form_fewinv<T>(nF, Fl, nF, sNF2, dinv, finv, fws, nm, s));
+  if (form_inv) HPS_CHECK_HIP(h, form_fewinv<T>(nF, Fl, nF, sNF2, dinv, finv, fws, nm, s));
 
   // ---- solves: X = -D^{-1} [C | h_int] -------------------------------------
   // one pass for the C columns (width ne) and one for h (width nsrc)
@@ -863,7 +871,7 @@ int hps_merge_level(hps_handle_t h, int dtype, int kind, int nm, int nc, int nec
                     const int* cols_final, const int* final_of_internal, int n_upd, const int* upd, int n_d12,
                     const int* d12_blk, int n_d21, const int* d21_blk, void* S_out, void* T_out, void* h_out,
                     void* gt_out, void* F_lu, int* perm, void* F_dinv, int* info, void* D12_out, void* D21_out,
-                    void* B_out, void* ws, size_t ws_bytes) {
+                    void* B_out, void* F_inv_out, void* ws, size_t ws_bytes) {
   if (!h) return HPS_ERR_ARG;
   HPS_ARG(h, kind == HPS_KIND_DTN || kind == HPS_KIND_ITI, "kind must be DtN or ItI");
   HPS_ARG(h, nm > 0 && nc > 0 && nec > 0 && ni > 0 && ne > 0 && nsrc >= 0, "bad merge sizes");
@@ -900,7 +908,7 @@ int hps_merge_level(hps_handle_t h, int dtype, int kind, int nm, int nc, int nec
   }
   DISPATCH(dtype, merge_t, h, kind, nm, nc, nec, ni, ne, nsrc, T_in, h_in, int_rows, int_cols, ext, cols_final,
            final_of_internal, n_upd, upd, n_d12, d12_blk, n_d21, d21_blk, S_out, T_out, h_out, gt_out, F_lu, perm,
-           F_dinv, info, D12_out, D21_out, B_out, ws);
+           F_dinv, info, D12_out, D21_out, B_out, F_inv_out, ws);
 }
 
 // ---------------------------------------------------------------------------

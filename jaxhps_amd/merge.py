@@ -12,7 +12,7 @@ from typing import List
 import torch
 
 from ._lib import raise_on_info
-from ._stages import cast, form_block_inverses, merge_factors_stable, merge_level
+from ._stages import cast, merge_factors_stable, merge_level
 from ._tables import oct_merge_tables, quad_merge_tables
 
 
@@ -28,8 +28,8 @@ def _levels(T_arr, h_arr, l, kind, nc, table_fn, keep_B=False, root_T=True, dtyp
     latency-bound.  Every level is checked at the end (one synchronisation);
     from the first level that fails the check the merges are redone with
     partial pivoting.  Low-precision levels of fewer than 128 matrices of more
-    than 256 unknowns then get the inverses of their factors' diagonal blocks
-    for the refinement's few-column up passes (form_block_inverses)."""
+    than 256 unknowns keep the inverses of their factors' diagonal blocks for
+    the refinement's few-column up passes (MergeLevel.Finv)."""
     from . import profiling
     nec = T_arr.shape[-1]
     T0 = T_arr.reshape(-1, nec, nec)
@@ -73,8 +73,6 @@ def _levels(T_arr, h_arr, l, kind, nc, table_fn, keep_B=False, root_T=True, dtyp
             first = int((~ok).nonzero()[0, 0])
             T, h = (T0, h0) if first == 0 else (out[first - 1].T, out[first - 1].h)
             out = run(first, T, h, out[:first], True)
-    for r in out:
-        form_block_inverses(r)
     return out, multi
 
 

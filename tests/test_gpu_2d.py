@@ -383,7 +383,7 @@ def test_unpivoted_register_panel_merge_rejected_falls_back(gpu):
 @pytest.mark.parametrize("kind,m,nsrc", [("dtn", 80, 1), ("dtn", 80, 4), ("iti", 72, 1), ("iti", 72, 16)])
 def test_up_pass_through_block_inverses_matches_blocked_solve(gpu, kind, m, nsrc):
     """A low-precision merge level of 3 matrices of 320 (DtN) / 288 (ItI)
-    unknowns -- one full 256-block and a partial one -- gets block inverses;
+    unknowns -- one full 256-block and a partial one -- keeps block inverses;
     the up pass through them gives the blocked triangular solve's g_tilde and
     h to single precision, and both match the complex128 / float64 up pass.
     The merge's own source solve (through inverses it forms in its
@@ -407,8 +407,11 @@ def test_up_pass_through_block_inverses_matches_blocked_solve(gpu, kind, m, nsrc
         hg = torch.as_tensor(h, device=gpu).to(dt).contiguous()
         lvl = merge_level(Tg, hg, tb, keep_factors=True)
         res[dt, "merge"] = (lvl.gt, lvl.h)
-        form_block_inverses(lvl)
         assert (lvl.Finv is not None) == (dt == lo)
+        if lvl.Finv is not None:  # the merge's inverses are hps_block_inverses'
+            from_merge = lvl.Finv.clone()
+            form_block_inverses(lvl)
+            assert torch.equal(lvl.Finv, from_merge)
         res[dt, True] = up_level(hg, lvl)
         lvl.Finv = None
         res[dt, False] = up_level(hg, lvl)
