@@ -444,21 +444,38 @@ __global__ __launch_bounds__(256) void gemm_skinny_kernel(GemmArgs<T> p) {
       acc_re[n] = R(0);
       acc_im[n] = R(0);
     }
-    for (int k = sub; k < p.K && live; k += LPR) {
+    auto fma_into = [&](const T& a, const T& bv, int n) {
+      if constexpr (CPLX) {
+        acc_re[n] += s_re(a) * s_re(bv) - s_im(a) * s_im(bv);
+        acc_im[n] += s_re(a) * s_im(bv) + s_im(a) * s_re(bv);
+      } else {
+        acc_re[n] += s_re(a) * s_re(bv);
+      }
+    };
+    // four k steps per iteration: their loads are all issued before the
+    // products, so each lane has four rows' worth of A and B in flight
+    int k = sub;
+    for (; k + 3 * LPR < p.K && live; k += 4 * LPR) {
+      T a[4], bv[4][NMAX];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a[u] = A[k + u * LPR];
+        const T* Bk = B + (long)(k + u * LPR) * p.ldb;
+#pragma unroll
+        for (int n = 0; n < NMAX; ++n) bv[u][n] = n < p.N ? Bk[n] : s_zero<T>();
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int n = 0; n < NMAX; ++n)
+          if (n < p.N) fma_into(a[u], bv[u][n], n);
+    }
+    for (; k < p.K && live; k += LPR) {
       T a = A[k];
       const T* Bk = B + (long)k * p.ldb;
 #pragma unroll
-      for (int n = 0; n < NMAX; ++n) {
-        if (n < p.N) {
-          T bv = Bk[n];
-          if constexpr (CPLX) {
-            acc_re[n] += s_re(a) * s_re(bv) - s_im(a) * s_im(bv);
-            acc_im[n] += s_re(a) * s_im(bv) + s_im(a) * s_re(bv);
-          } else {
-            acc_re[n] += s_re(a) * s_re(bv);
-          }
-        }
-      }
+      for (int n = 0; n < NMAX; ++n)
+        if (n < p.N) fma_into(a, Bk[n], n);
     }
 #pragma unroll
     for (int n = 0; n < NMAX; ++n) {
