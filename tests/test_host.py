@@ -235,6 +235,33 @@ def test_library_argument_errors_without_gpu():
     assert lib.hps_destroy(h) == 0
 
 
+def test_block_inverse_sizes_and_argument_errors():
+    """hps_block_inverses: eligible sizes (n > 256, a multiple of 32) hold two
+    256 x 256 inverses per diagonal block (the last one padded), others none;
+    bad sizes and pointers are refused before any device work."""
+    from jaxhps_amd import _lib
+
+    lib = _lib.load_library()
+    assert lib.hps_block_inverses_elems(256) == 0
+    assert lib.hps_block_inverses_elems(300) == 0
+    assert lib.hps_block_inverses_elems(288) == 2 * 2 * 256 * 256
+    assert lib.hps_block_inverses_elems(3584) == 2 * 14 * 256 * 256
+    ws = lib.hps_block_inverses_workspace(_lib.HPS_C64, 3584, 2)
+    assert ws >= 2 * 14 * (256 * 256 + 256 * 32) * 8
+    h = ctypes.c_void_p()
+    assert lib.hps_create(None, ctypes.byref(h)) == 0
+    p = ctypes.c_void_p(8)
+    rc = lib.hps_block_inverses(h, _lib.HPS_C64, 256, 1, p, p, p, p, 1 << 30)
+    assert rc == -1 and b"n > 256" in lib.hps_last_error(h)
+    rc = lib.hps_block_inverses(h, _lib.HPS_C64, 288, 0, p, p, p, p, 1 << 30)
+    assert rc == -1 and b"count" in lib.hps_last_error(h)
+    rc = lib.hps_block_inverses(h, _lib.HPS_C64, 288, 1, None, p, p, p, 1 << 30)
+    assert rc == -1 and b"null" in lib.hps_last_error(h)
+    rc = lib.hps_block_inverses(h, _lib.HPS_C64, 288, 1, p, p, p, p, 16)
+    assert rc != 0 and b"workspace" in lib.hps_last_error(h)
+    assert lib.hps_destroy(h) == 0
+
+
 def test_product_fails_loudly_without_gpu():
     import torch
 
