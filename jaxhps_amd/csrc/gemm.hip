@@ -175,6 +175,7 @@ hipError_t gemm_grouped(const GemmArgs<T>& a, int n_blk, const int* blk, hipStre
                             (double)gr.nk[i] * a.N * (a.sB ? a.batch : 1) +
                             (double)gr.nr[i] * a.N * a.batch * (a.beta != 0.0 ? 2 : 1));
     }
+    if (gr.start[gr.n] == 0) continue;  // blocks without rows: nothing to launch
     prof::Scope ps(s, nm.c_str(), flops, bytes);
     dim3 grid((unsigned)gr.start[gr.n], gy);
     if constexpr (sizeof(T) == 16)
